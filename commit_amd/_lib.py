@@ -1,0 +1,68 @@
+"""ctypes binding of libcommit_hip.so (declared in include/commit_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C commit_amd/csrc``).  There is no fallback: if the library is missing
+or fails to load, every operator call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcommit_hip.so")
+SYNTH_PATH = os.path.join(_HERE, "libcommit_synth.so")
+
+FP64 = 0
+FP32 = 1
+
+# every symbol include/commit_hip.h declares, with its ctypes signature
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U = ctypes.c_uint32
+_I64P = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = {
+    "commit_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _I, _I, _I,
+                                    _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _U, _U, _U, _I, _I, _I]),
+    "commit_hip_plan_destroy": (_I, [_P]),
+    "commit_hip_plan_set_eval": (_I, [_P, _P]),
+    "commit_hip_plan_shape": (_I, [_P, _I64P, _I64P]),
+    "commit_hip_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "commit_hip_A": (_I, [_P, _P, _P, _P]),
+    "commit_hip_At": (_I, [_P, _P, _P, _P]),
+    "commit_hip_last_error": (ctypes.c_char_p, []),
+    "commit_hip_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class CommitHipError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libcommit_hip.so once; raise if it is absent (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CommitHipError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() or `make -C commit_amd/csrc`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().commit_hip_last_error().decode(errors="replace")
+        raise CommitHipError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(a):
+    """Host pointer of a numpy array (None for empty/None)."""
+    if a is None or a.size == 0:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)

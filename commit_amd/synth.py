@@ -1,0 +1,330 @@
+"""Synthetic COMMIT problems: tractogram dictionaries, acquisition schemes,
+Stick-Zeppelin-Ball kernel LUTs, ground-truth weights and noisy signals.
+
+The dictionary arrays come out in the layout trk2dictionary writes
+(trk2dictionary_c.cpp:517-547): fibre-major, per fibre ordered by the
+(orientation, z, y, x) segment key, voxel ids linear ``x + X*(y + Y*z)``.
+``to_dictionary`` then applies what Evaluation.load_dictionary does
+(core.pyx:505-580) to produce the in-memory DICTIONARY.
+
+Named workloads:
+  ``small``  32x32x20 voxels, 10,000 fibres (5 random control points each),
+             nS=60 (b=1000,2000 s/mm^2 x 30 directions), 500 directions.
+  ``brain``  145x174x145 voxels at 1.25 mm with a ~600k-voxel ellipsoidal
+             white-matter mask, 1M fibres of 20-200 mm, nS=288
+             (18 b0 + b=1000/2000/3000 x 90 directions).
+  ``brain200k``  the ``brain`` geometry with 200k fibres.
+Sizes can be overridden (tests use scaled-down versions).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._lib import SYNTH_PATH
+
+D_PAR = 1.7e-3
+D_PERP = 0.6e-3
+D_ISOS = (1.7e-3, 3.0e-3)
+
+_synth = None
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("nF", ctypes.c_int), ("dim", ctypes.c_int * 3),
+                ("pixdim", ctypes.c_double), ("mask", ctypes.c_void_p), ("mode", ctypes.c_int),
+                ("len_min_mm", ctypes.c_double), ("len_max_mm", ctypes.c_double),
+                ("step_vox", ctypes.c_double), ("hash", ctypes.c_void_p), ("nthreads", ctypes.c_int)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("IC_f", ctypes.c_void_p), ("IC_v", ctypes.c_void_p), ("IC_o", ctypes.c_void_p),
+                ("IC_len", ctypes.c_void_p), ("TRK_norm", ctypes.c_void_p), ("TRK_len", ctypes.c_void_p),
+                ("TRK_lenTot", ctypes.c_void_p), ("n", ctypes.c_size_t), ("nF", ctypes.c_int)]
+
+
+def _lib():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} is not built; run `make -C commit_amd/csrc`")
+        lib = ctypes.CDLL(SYNTH_PATH)
+        lib.synth_generate.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Result)]
+        lib.synth_generate.restype = ctypes.c_int
+        lib.synth_free.argtypes = [ctypes.POINTER(_Result)]
+        lib.synth_hash_table.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.synth_stable_argsort_u32.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                                                 ctypes.c_void_p, ctypes.c_void_p]
+        lib.synth_stable_argsort_u32.restype = ctypes.c_int
+        _synth = lib
+    return _synth
+
+
+def _threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+# ------------------------------------------------------------- directions --
+def hemisphere_directions(n=500):
+    """n quasi-uniform unit vectors with y > 0 (the hemisphere the kernels are
+    computed on, trk2dictionary_c.cpp:770-781): a Fibonacci lattice around +y."""
+    i = np.arange(n, dtype=np.float64)
+    yv = 1.0 - (i + 0.5) / n
+    r = np.sqrt(np.maximum(0.0, 1.0 - yv * yv))
+    phi = i * np.pi * (3.0 - np.sqrt(5.0))
+    return np.stack([r * np.cos(phi), yv, r * np.sin(phi)], axis=1)
+
+
+def sphere_directions(n):
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - 2.0 * (i + 0.5) / n
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = i * np.pi * (3.0 - np.sqrt(5.0))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def hash_table(dirs):
+    """181x181 (colatitude, longitude) degree table -> nearest direction index."""
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+    table = np.zeros(181 * 181, dtype=np.int16)
+    _lib().synth_hash_table(dirs.ctypes.data, dirs.shape[0], table.ctypes.data)
+    return table
+
+
+# ----------------------------------------------------------------- scheme --
+def scheme(shells, ndirs_per_shell, n_b0=0):
+    """b-values (s/mm^2) and unit gradient directions, b0 volumes first."""
+    g = sphere_directions(ndirs_per_shell)
+    bvals = [np.zeros(n_b0)]
+    bvecs = [np.tile([1.0, 0.0, 0.0], (n_b0, 1))]
+    for b in shells:
+        bvals.append(np.full(ndirs_per_shell, float(b)))
+        bvecs.append(g)
+    return np.concatenate(bvals), np.concatenate(bvecs)
+
+
+def szb_kernels(bvals, bvecs, dirs, d_par=D_PAR, d_perp=D_PERP, d_isos=D_ISOS):
+    """Stick (IC), Zeppelin (EC) and Ball (ISO) LUTs, float32:
+    wmr[1,ndirs,nS] = exp(-b d_par (g.u)^2),
+    wmh[1,ndirs,nS] = exp(-b (d_perp + (d_par - d_perp)(g.u)^2)),
+    iso[k,nS]       = exp(-b d_iso_k).  ``d_perp=None`` drops the EC compartment."""
+    c2 = (dirs @ bvecs.T) ** 2
+    b = bvals[None, :]
+    wmr = np.exp(-b * d_par * c2)[None].astype(np.float32)
+    if d_perp is None:
+        wmh = np.zeros((0, dirs.shape[0], bvals.size), dtype=np.float32)
+    else:
+        wmh = np.exp(-b * (d_perp + (d_par - d_perp) * c2))[None].astype(np.float32)
+    iso = np.stack([np.exp(-bvals * d) for d in d_isos]).astype(np.float32) if len(d_isos) else \
+        np.zeros((0, bvals.size), dtype=np.float32)
+    return {"model": "StickZeppelinBall", "wmr": np.ascontiguousarray(wmr),
+            "wmh": np.ascontiguousarray(wmh), "iso": np.ascontiguousarray(iso)}
+
+
+def normalize_kernels(K):
+    """doNormalizeKernels (core.pyx:412-431): each RF divided by the norm of its
+    orientation-0 row; ISO by its own norm.  Norms kept for renormalisation."""
+    K = dict(K)
+    for key in ("wmr", "wmh"):
+        a = np.array(K[key], dtype=np.float32, copy=True)
+        nrm = np.array([np.linalg.norm(a[i, 0, :]) for i in range(a.shape[0])])
+        for i in range(a.shape[0]):
+            a[i] /= nrm[i]
+        K[key], K[key + "_norm"] = np.ascontiguousarray(a), nrm
+    a = np.array(K["iso"], dtype=np.float32, copy=True)
+    nrm = np.array([np.linalg.norm(a[i]) for i in range(a.shape[0])])
+    for i in range(a.shape[0]):
+        a[i] /= nrm[i]
+    K["iso"], K["iso_norm"] = np.ascontiguousarray(a), nrm
+    return K
+
+
+# ------------------------------------------------------------- tractogram --
+def ellipsoid_mask(dim, semi_axes):
+    x, y, z = np.meshgrid(*[np.arange(d) + 0.5 for d in dim], indexing="ij")
+    c = [d / 2.0 for d in dim]
+    r = ((x - c[0]) / semi_axes[0]) ** 2 + ((y - c[1]) / semi_axes[1]) ** 2 + ((z - c[2]) / semi_axes[2]) ** 2
+    return (r <= 1.0).astype(np.uint8)
+
+
+def tractogram(seed, nF, dim, pixdim=1.0, mask=None, mode=0, len_mm=(20.0, 200.0), step_vox=0.4,
+               dirs=None, nthreads=None):
+    """Seeded random cubic B-spline fibres voxelised into a raw dictionary
+    (trk2dictionary layout).  ``mask`` is an (X,Y,Z) uint8 array; mode 0 draws
+    5 control points uniformly in the box, mode 1 draws a smooth path of length
+    U(len_mm) centred inside the mask."""
+    if dirs is None:
+        dirs = hemisphere_directions(500)
+    table = hash_table(dirs)
+    p = _Params()
+    p.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    p.nF = int(nF)
+    p.dim[0], p.dim[1], p.dim[2] = (int(d) for d in dim)
+    p.pixdim = float(pixdim)
+    # generator indexes the mask as x + X*(y + Y*z): Fortran order of (X,Y,Z)
+    mflat = None
+    if mask is not None:
+        mflat = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).ravel(order="F"))
+        p.mask = mflat.ctypes.data
+    p.mode = int(mode)
+    p.len_min_mm, p.len_max_mm = float(len_mm[0]), float(len_mm[1])
+    p.step_vox = float(step_vox)
+    p.hash = table.ctypes.data
+    p.nthreads = int(nthreads or _threads())
+    r = _Result()
+    lib = _lib()
+    if lib.synth_generate(ctypes.byref(p), ctypes.byref(r)) != 0:
+        raise RuntimeError("synthetic tractogram generation failed")
+    try:
+        n = int(r.n)
+
+        def take(addr, dtype, count):
+            if count == 0:
+                return np.zeros(0, dtype=dtype)
+            buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(addr)
+            return np.frombuffer(buf, dtype=dtype).copy()
+
+        raw = {
+            "IC_f": take(r.IC_f, np.uint32, n), "IC_v": take(r.IC_v, np.uint32, n),
+            "IC_o": take(r.IC_o, np.uint16, n), "IC_len": take(r.IC_len, np.float32, n),
+            "TRK_norm": take(r.TRK_norm, np.float32, int(nF)),
+            "TRK_len": take(r.TRK_len, np.float32, int(nF)),
+            "TRK_lenTot": take(r.TRK_lenTot, np.float32, int(nF)),
+        }
+    finally:
+        lib.synth_free(ctypes.byref(r))
+    raw["TRK_kept"] = np.ones(int(nF), dtype=np.uint8)
+    raw["dim"] = tuple(int(d) for d in dim)
+    raw["pixdim"] = float(pixdim)
+    raw["dirs"] = dirs
+    return raw
+
+
+def add_peaks_and_iso(raw):
+    """EC: one peak per traversed voxel, along the voxel's length-weighted mean
+    segment direction; ISO: every traversed voxel, ascending linear id
+    (ISOcompartments, trk2dictionary_c.cpp:400-437)."""
+    dirs = raw["dirs"]
+    nvox_grid = int(np.prod(raw["dim"]))
+    v, o, l = raw["IC_v"], raw["IC_o"], raw["IC_len"].astype(np.float64)
+    tdi = np.bincount(v, weights=l, minlength=nvox_grid)
+    vox = np.nonzero(tdi > 0)[0].astype(np.uint32)
+    acc = np.stack([np.bincount(v, weights=l * dirs[o, c], minlength=nvox_grid)[vox] for c in range(3)], 1)
+    nrm = np.linalg.norm(acc, axis=1, keepdims=True)
+    nrm[nrm == 0] = 1.0
+    mean = acc / nrm
+    raw["EC_v"] = vox.copy()
+    raw["EC_o"] = np.argmax(np.abs(mean @ dirs.T), axis=1).astype(np.uint16)
+    raw["ISO_v"] = vox.copy()
+    raw["tdi"] = tdi
+    return raw
+
+
+def stable_argsort_u32(keys, nkeys):
+    """Stable argsort of uint32 keys < nkeys (numpy kind='mergesort' order), O(n)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    perm = np.empty(keys.size, dtype=np.uint32)
+    offs = np.empty(int(nkeys) + 1, dtype=np.uint64)
+    if _lib().synth_stable_argsort_u32(keys.ctypes.data, keys.size, int(nkeys), perm.ctypes.data,
+                                       offs.ctypes.data) != 0:
+        raise ValueError("key out of range")
+    return perm
+
+
+def to_dictionary(raw, normalize=True, use_all_voxels_in_mask=False, mask=None):
+    """In-memory equivalent of Evaluation.load_dictionary (core.pyx:436-582):
+    stable sort of IC/EC/ISO by voxel, lengths divided by TRK_norm, linear voxel
+    ids remapped to compact mask indices (Fortran order)."""
+    dim = raw["dim"]
+    nvox_grid = int(np.prod(dim))
+    if use_all_voxels_in_mask and mask is not None:
+        m = np.asarray(mask) > 0
+    else:
+        tdi = raw.get("tdi")
+        if tdi is None:
+            tdi = np.bincount(raw["IC_v"], weights=raw["IC_len"].astype(np.float64), minlength=nvox_grid)
+        m = (np.reshape(tdi, dim, order="F") > 0)
+    D = {"ndirs": raw["dirs"].shape[0], "MASK": m.astype(np.uint8), "TRK": {}, "IC": {}, "EC": {}, "ISO": {}}
+    for k in ("kept", "norm", "len", "lenTot"):
+        D["TRK"][k] = raw["TRK_" + k]
+    idx = stable_argsort_u32(raw["IC_v"], nvox_grid)
+    D["IC"]["vox"] = raw["IC_v"][idx]
+    D["IC"]["dir"] = raw["IC_o"][idx]
+    D["IC"]["str"] = raw["IC_f"][idx]
+    D["IC"]["len"] = raw["IC_len"][idx]
+    D["IC"]["n"] = int(idx.size)
+    D["IC"]["nSTR"] = int(raw["TRK_norm"].size)
+    D["IC"]["nVOX"] = int(m.sum())
+    if normalize:
+        D["IC"]["len"] = (D["IC"]["len"] / raw["TRK_norm"][D["IC"]["str"]]).astype(np.float32)
+    idx = stable_argsort_u32(raw["EC_v"], nvox_grid)
+    D["EC"]["vox"], D["EC"]["dir"], D["EC"]["n"] = raw["EC_v"][idx], raw["EC_o"][idx], int(idx.size)
+    idx = stable_argsort_u32(raw["ISO_v"], nvox_grid)
+    D["ISO"]["vox"], D["ISO"]["n"] = raw["ISO_v"][idx], int(idx.size)
+    lin = np.nonzero(m.ravel(order="F"))[0]
+    lut = np.zeros(nvox_grid, dtype=np.uint32)
+    lut[lin] = np.arange(lin.size, dtype=np.uint32)
+    for c in ("IC", "EC", "ISO"):
+        D[c]["vox"] = lut[D[c]["vox"]]
+    D["MASK_ix"], D["MASK_iy"], D["MASK_iz"] = np.unravel_index(lin, dim, order="F")
+    D["IC"]["eval"] = np.ones(D["IC"]["nSTR"] * 1, dtype=np.uint32)
+    return D
+
+
+# ------------------------------------------------------------- workloads --
+WORKLOADS = {
+    "small": dict(dim=(32, 32, 20), pixdim=1.0, nF=10_000, mode=0, step_vox=0.4,
+                  shells=(1000, 2000), ndirs_per_shell=30, n_b0=0, mask=None),
+    "brain": dict(dim=(145, 174, 145), pixdim=1.25, nF=1_000_000, mode=1, step_vox=0.5,
+                  len_mm=(20.0, 200.0), shells=(1000, 2000, 3000), ndirs_per_shell=90, n_b0=18,
+                  mask=("ellipsoid", (52.0, 62.0, 45.0))),
+    "brain200k": dict(dim=(145, 174, 145), pixdim=1.25, nF=200_000, mode=1, step_vox=0.5,
+                      len_mm=(20.0, 200.0), shells=(1000, 2000, 3000), ndirs_per_shell=90, n_b0=18,
+                      mask=("ellipsoid", (52.0, 62.0, 45.0))),
+}
+
+
+def make_problem(name="small", seed=1001, nF=None, with_ec=True, n_iso=2, normalize=True, **over):
+    """Dictionary + normalised SZB kernels for a named workload.
+
+    Returns (DICTIONARY, KERNELS, raw)."""
+    cfg = dict(WORKLOADS[name])
+    cfg.update(over)
+    if nF is not None:
+        cfg["nF"] = nF
+    mask = None
+    if cfg.get("mask") is not None:
+        kind, axes = cfg["mask"]
+        mask = ellipsoid_mask(cfg["dim"], axes)
+    raw = tractogram(seed, cfg["nF"], cfg["dim"], cfg["pixdim"], mask, cfg["mode"],
+                     cfg.get("len_mm", (20.0, 200.0)), cfg["step_vox"])
+    add_peaks_and_iso(raw)
+    D = to_dictionary(raw, normalize=normalize)
+    bvals, bvecs = scheme(cfg["shells"], cfg["ndirs_per_shell"], cfg["n_b0"])
+    K = szb_kernels(bvals, bvecs, raw["dirs"], D_PAR, D_PERP if with_ec else None, D_ISOS[:n_iso])
+    if normalize:
+        K = normalize_kernels(K)
+    if not with_ec:
+        D["EC"] = {"vox": np.zeros(0, np.uint32), "dir": np.zeros(0, np.uint16), "n": 0}
+    K["bvals"], K["bvecs"] = bvals, bvecs
+    return D, K, raw
+
+
+def x_true(nCOLS, seed, density=0.2, lo=0.1, hi=1.0):
+    """Non-negative ground truth with `density` nonzeros ~ U(lo, hi)."""
+    rng = np.random.default_rng(seed)
+    x = np.zeros(nCOLS)
+    nz = rng.random(nCOLS) < density
+    x[nz] = rng.uniform(lo, hi, int(nz.sum()))
+    return x
+
+
+def rician(signal, snr, seed):
+    """|signal + n1 + i n2| with sigma = mean(signal) / snr."""
+    rng = np.random.default_rng(seed)
+    sigma = float(np.mean(signal)) / snr
+    return np.sqrt((signal + sigma * rng.standard_normal(signal.size)) ** 2 +
+                   (sigma * rng.standard_normal(signal.size)) ** 2)
