@@ -3,26 +3,41 @@
 //
 // Reference algorithm: setup_operator.py:48-915 (generated COMMIT_A/COMMIT_At
 // and their LUT-free variants), driven from operator.pyx:114-184.  The design
-// here is GPU-first and shares none of the reference's structure:
+// here is GPU-first and shares none of the reference's structure.
 //
-//   * the plan builds, once, a voxel-major CSR layout of the segments (the
-//     order load_dictionary produces, core.pyx:509-513) and a fibre-major
-//     permutation of it, and keeps both resident in HBM;
-//   * A x: one wavefront per voxel, the nS samples of the voxel spread over
-//     the 64 lanes (NJ = ceil(nS/64) FP64 accumulators per lane); segment
-//     records are read cooperatively (one per lane) and broadcast with
-//     v_readlane, so the LUT row reads of a segment are fully coalesced;
-//   * A^T y: two atomic-free passes.  Pass 1 (voxel-major) stages the voxel's
-//     y row in LDS and computes, one segment per lane, the FP64 dot product of
-//     its LUT row with it; pass 2 (fibre-major) sums each fibre's products in
-//     ascending voxel order.  EC and ISO columns are owned by a single voxel and
-//     are written directly by pass 1.
+// Device layout (built once per plan, resident in HBM)
+//   * every compartment (IC segments, EC peaks, ISO voxels) as a voxel-major
+//     CSR: entry ranges per voxel, entries in the order load_dictionary leaves
+//     them (stable by voxel, core.pyx:509-513); entry = orientation (u16),
+//     x column (u32), length (f32, IC only) — 10 bytes per IC segment;
+//   * a fibre-major permutation of the IC entries (ascending voxel inside each
+//     fibre) for the adjoint's second pass;
+//   * the LUTs widened to 64*NJ samples per row with zeros, NJ = ceil(nS/64),
+//     so a row is NJ unpredicated coalesced loads.
+//
+// Kernels: one wavefront per voxel, the voxel's samples spread over the 64
+// lanes (NJ per lane).  Work-groups are remapped so that each XCD (own L2)
+// processes one contiguous voxel range: a spatial slab, whose fibres, LUT rows
+// and x entries then share that XCD's L2.
+//   A x   : NJ FP64 accumulators per lane.  Entry records are read one per
+//           lane and walked in order through a 3-stage software pipeline:
+//           while the row of entry t is applied, the rows of t+1 and t+2 are in
+//           flight.  Zero coefficients are dropped with a ballot.
+//   A^T y : pass 1 keeps the voxel's y row in registers; each entry's per-lane
+//           partial dot product goes to an LDS ring and every 8 entries are
+//           reduced together (transposed read + 3 shuffles).  IC products go
+//           to a scratch vector in entry order; EC and ISO columns belong to a
+//           single voxel and are written straight to x.  Pass 2 sums every
+//           fibre's products in ascending voxel order (the order the
+//           reference's owning thread uses) — no atomics, bitwise repeatable.
 //
 // Entry points are declared in include/commit_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -56,8 +71,37 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 constexpr int kMaxNJ = 8;            // nS <= 512 samples per voxel
 constexpr int kMaxRF = 20;           // the reference's own limit per compartment
+constexpr int kXcds = 8;             // MI355X: 8 XCDs, work-groups dealt round-robin
 
 // ------------------------------------------------------------------ device --
+struct Comp {                       // one compartment, voxel-major CSR
+    const uint32_t *ptr;            // [nVOX+1]
+    const uint16_t *o;              // [m] orientation (null: ISO, a single row)
+    const uint32_t *col;            // [m] x column inside one RF block
+    const float *len;               // [m] segment length (IC only)
+    uint32_t m;
+};
+
+struct DevDict {
+    Comp ic, ec, iso;
+    const uint32_t *eval;           // [nF*nIC]
+    const uint32_t *fib_ptr;        // [nF+1]
+    const uint32_t *fib_perm;       // [n] IC entry index, fibre-major, ascending voxel
+    const float *wmr, *wmh, *isol;  // padded LUTs: rows of nSp floats
+    int nS, nSp, ndirs, nF, n, nE, nI, nVOX, nIC, nEC, nISO;
+};
+
+// Voxel of this wavefront.  Blocks are dealt to the XCDs round-robin; block b
+// is remapped so that XCD (b % 8) walks a contiguous range of voxels.  The
+// result is made scalar: the compiler cannot prove threadIdx.x >> 6 uniform.
+__device__ __forceinline__ int voxel_id()
+{
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+    const int lb = x * q + min(x, r) + b / kXcds;
+    return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (int)(threadIdx.x >> 6));
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
     const long long b = __double_as_longlong(v);
@@ -66,206 +110,240 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-struct DevDict {
-    // IC, voxel-major CSR
-    const uint32_t *ic_ptr;   // [nVOX+1]
-    const uint32_t *ic_f;     // [n]
-    const uint16_t *ic_o;     // [n]
-    const float *ic_l;        // [n]
-    const uint32_t *eval;     // [nF*nIC]
-    // fibre-major view of the same segments
-    const uint32_t *fib_ptr;  // [nF+1]
-    const uint32_t *fib_perm; // [n] voxel-major position of each fibre-major segment
-    // EC, voxel-major CSR
-    const uint32_t *ec_ptr;   // [nVOX+1]
-    const uint32_t *ec_e;     // [nE] original peak index (x column)
-    const uint16_t *ec_o;     // [nE]
-    // ISO, voxel-major CSR
-    const uint32_t *iso_ptr;  // [nVOX+1]
-    const uint32_t *iso_k;    // [nV] original ISO index (x column)
-    // LUTs (float32, row-major [rf][dir][s])
-    const float *wmr, *wmh, *iso;
-    int nS, ndirs, nF, n, nE, nI, nVOX;
-    int nIC, nEC, nISO;
-};
-
-// Accumulate, for every sample owned by this lane, sum_t c_t * row_t[s] over the
-// (wave-uniform) records t < cnt.  c and o live one record per lane.
-template <int NJ>
-__device__ __forceinline__ void accumulate_rows(double (&acc)[NJ], double c, int o, int cnt,
-                                                const float *__restrict__ lut, int nS, int lane,
-                                                bool same_row)
+template <typename T>
+__device__ __forceinline__ T stream_load(const T *p)
 {
-    for (int t = 0; t < cnt; t++) {
-        const double ct = readlane_f64(c, t);
-        if (ct != 0.0) {
-            const int ot = same_row ? 0 : __builtin_amdgcn_readlane(o, t);
-            const float *row = lut + (size_t)ot * nS;
+    return __builtin_nontemporal_load(p);    // read once per product: keep L2 for LUT rows and x
+}
+
+// NJ coalesced loads of one padded LUT row (row points at this lane's sample 0)
+template <int NJ>
+__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row)
+{
 #pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                const int s = lane + kWave * j;
-                if (s < nS) acc[j] = fma(ct, (double)row[s], acc[j]);
+    for (int j = 0; j < NJ; j++) raw[j] = row[kWave * j];
+}
+
+// next set bit of m (consumed), or -1
+__device__ __forceinline__ int take(unsigned long long &m)
+{
+    if (!m) return -1;
+    const int t = __builtin_ctzll(m);
+    m &= m - 1;
+    return t;
+}
+
+// ---------------------------------------------------------------- y = A x ---
+template <int NJ>
+__device__ __forceinline__ void fma_row(double (&acc)[NJ], double c, const float (&raw)[NJ])
+{
+#pragma unroll
+    for (int j = 0; j < NJ; j++) acc[j] = fma(c, (double)raw[j], acc[j]);
+}
+
+// Accumulate the entries [b, e) of one compartment into acc.
+//   xb: x (or x*eval) block of RF 0, xs: stride between RF blocks;
+//   lut: padded RF 0 table, lut_rf: stride between RF tables.
+template <int NJ>
+__device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32_t b, uint32_t e,
+                                       const double *__restrict__ xb, size_t xs, int nrf,
+                                       const float *__restrict__ lut, size_t lut_rf, int nSp, int lane)
+{
+    for (int i = 0; i < nrf; i++) {
+        const float *L = lut + (size_t)i * lut_rf + lane;
+        for (uint32_t base = b; base < e; base += kWave) {
+            const uint32_t k = base + lane;
+            double c = 0.0;
+            uint32_t o = 0;
+            if (k < e) {
+                if (cp.o) o = stream_load(cp.o + k);
+                c = xb[(size_t)stream_load(cp.col + k) + (size_t)i * xs];
+                // IC coefficient l * x * eval                      (setup_operator.py:63-96)
+                if (cp.len) c *= (double)stream_load(cp.len + k);
+            }
+            // zero coefficients are skipped                      (setup_operator.py:88)
+            unsigned long long m = __ballot(c != 0.0);
+            float ra[NJ], rb[NJ], rc[NJ];
+            int ta = take(m), tb = -1, tc = -1;
+            if (ta >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp);
+            if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp);
+            // three rows rotate through the pipeline; entries keep their order
+            while (true) {
+                if (ta < 0) break;
+                if ((tc = take(m)) >= 0) fetch_row<NJ>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp);
+                fma_row<NJ>(acc, readlane_f64(c, ta), ra);
+                if (tb < 0) break;
+                if ((ta = take(m)) >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp);
+                fma_row<NJ>(acc, readlane_f64(c, tb), rb);
+                if (tc < 0) break;
+                if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp);
+                fma_row<NJ>(acc, readlane_f64(c, tc), rc);
             }
         }
     }
 }
 
-// y = A x, FP64.  One wavefront per voxel; lanes own samples lane + 64 j.
+// xic: the IC block of x, or x*eval when eval is not all ones.
 template <int NJ>
 __global__ __launch_bounds__(kBlock) void k_A_fp64(DevDict d, const double *__restrict__ x,
-                                                    double *__restrict__ y)
+                                                    const double *__restrict__ xic, double *__restrict__ y)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int v = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int v = voxel_id();
     if (v >= d.nVOX) return;
     double acc[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) acc[j] = 0.0;
-    const size_t lut_stride = (size_t)d.ndirs * d.nS;
-
-    // intra-cellular: c = l * x[f + i nF] * eval[f + i nF]    (setup_operator.py:62-103)
-    {
-        const uint32_t b = d.ic_ptr[v], e = d.ic_ptr[v + 1];
-        for (uint32_t base = b; base < e; base += kWave) {
-            const uint32_t k = base + lane;
-            const int cnt = (int)min((uint32_t)kWave, e - base);
-            uint32_t f = 0;
-            int o = 0;
-            double l = 0.0;
-            if (k < e) {
-                f = d.ic_f[k];
-                o = d.ic_o[k];
-                l = (double)d.ic_l[k];
-            }
-            for (int i = 0; i < d.nIC; i++) {
-                double c = 0.0;
-                if (k < e) {
-                    const size_t col = f + (size_t)i * d.nF;
-                    c = l * (x[col] * (double)d.eval[col]);
-                }
-                accumulate_rows<NJ>(acc, c, o, cnt, d.wmr + i * lut_stride, d.nS, lane, false);
-            }
-        }
-    }
-    // extra-cellular: c = x[nIC nF + i nE + e]                  (setup_operator.py:111-165)
-    if (d.nEC > 0) {
-        const uint32_t b = d.ec_ptr[v], e = d.ec_ptr[v + 1];
-        const double *xe = x + (size_t)d.nIC * d.nF;
-        for (uint32_t base = b; base < e; base += kWave) {
-            const uint32_t k = base + lane;
-            const int cnt = (int)min((uint32_t)kWave, e - base);
-            uint32_t eo = 0;
-            int o = 0;
-            if (k < e) {
-                eo = d.ec_e[k];
-                o = d.ec_o[k];
-            }
-            for (int i = 0; i < d.nEC; i++) {
-                const double c = (k < e) ? xe[eo + (size_t)i * d.nE] : 0.0;
-                accumulate_rows<NJ>(acc, c, o, cnt, d.wmh + i * lut_stride, d.nS, lane, false);
-            }
-        }
-    }
-    // isotropic: c = x[nIC nF + nEC nE + i nV + k]               (setup_operator.py:167-218)
-    if (d.nISO > 0) {
-        const uint32_t b = d.iso_ptr[v], e = d.iso_ptr[v + 1];
-        const double *xi = x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE;
-        for (uint32_t base = b; base < e; base += kWave) {
-            const uint32_t k = base + lane;
-            const int cnt = (int)min((uint32_t)kWave, e - base);
-            const uint32_t ko = (k < e) ? d.iso_k[k] : 0u;
-            for (int i = 0; i < d.nISO; i++) {
-                const double c = (k < e) ? xi[ko + (size_t)i * d.nI] : 0.0;
-                accumulate_rows<NJ>(acc, c, 0, cnt, d.iso + (size_t)i * d.nS, d.nS, lane, true);
-            }
-        }
-    }
+    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
+    a_comp<NJ>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], xic, (size_t)d.nF, d.nIC, d.wmr, lut_rf, d.nSp, lane);
+    if (d.nEC > 0)
+        a_comp<NJ>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
+                   d.wmh, lut_rf, d.nSp, lane);
+    if (d.nISO > 0)
+        a_comp<NJ>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
+                   (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane);
     double *yv = y + (size_t)v * d.nS;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int s = lane + kWave * j;
-        if (s < d.nS) yv[s] = acc[j];
+        if (s < d.nS) __builtin_nontemporal_store(acc[j], yv + s);
     }
 }
 
-// Dot product of a float32 LUT row with the voxel's y row staged in LDS.
-__device__ __forceinline__ double row_dot(const float *__restrict__ row, const double *yr, int nS,
-                                          bool vec4)
+// x*eval for the IC block (only when eval is not all ones)
+__global__ void k_x_eval_fp64(const double *__restrict__ x, const uint32_t *__restrict__ ev,
+                              double *__restrict__ out, size_t n)
 {
-    double acc = 0.0;
-    if (vec4) {
-        const float4 *r4 = reinterpret_cast<const float4 *>(row);
-        const double2 *y2 = reinterpret_cast<const double2 *>(yr);
-        for (int q = 0; q < (nS >> 2); q++) {
-            const float4 a = r4[q];
-            const double2 b0 = y2[2 * q], b1 = y2[2 * q + 1];
-            acc = fma((double)a.x, b0.x, acc);
-            acc = fma((double)a.y, b0.y, acc);
-            acc = fma((double)a.z, b1.x, acc);
-            acc = fma((double)a.w, b1.y, acc);
-        }
-    } else {
-        for (int s = 0; s < nS; s++) acc = fma((double)row[s], yr[s], acc);
-    }
-    return acc;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = x[i] * (double)ev[i];
 }
 
-// A^T y pass 1, FP64.  One wavefront per voxel; the voxel's y row is staged in
-// LDS and each lane takes one segment (peak, ISO entry) of the voxel.
-__global__ __launch_bounds__(kBlock) void k_At_voxel_fp64(DevDict d, const double *__restrict__ y,
-                                                          double *__restrict__ z,
-                                                          double *__restrict__ x, int nSp)
-{
-    extern __shared__ double ylds[];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x >> 6;
-    const int v = blockIdx.x * kWavesPerBlock + w;
-    const bool valid = v < d.nVOX;
-    double *yr = ylds + (size_t)w * nSp;
-    if (valid)
-        for (int s = lane; s < d.nS; s += kWave) yr[s] = y[(size_t)v * d.nS + s];
-    __syncthreads();
-    if (!valid) return;
-    const size_t lut_stride = (size_t)d.ndirs * d.nS;
-    const bool vec4 = (d.nS & 3) == 0;
+// ------------------------------------------------------------ x = A^T y -----
+// Per-wave LDS ring of per-lane partial dot products, kRing entries x 64 lanes;
+// rows padded so that the transposed reads of a flush are bank-conflict free.
+constexpr int kRing = 8;
+constexpr int kRingStride = 66;     // doubles per row (64 lanes + pad, 16-byte aligned)
 
-    // intra-cellular: z[i n + k] = l * (LUT_IC_i[o] . y[v]) * eval   (setup_operator.py:412-428)
-    {
-        const uint32_t b = d.ic_ptr[v], e = d.ic_ptr[v + 1];
-        for (uint32_t k = b + lane; k < e; k += kWave) {
-            const uint32_t f = d.ic_f[k];
-            const int o = d.ic_o[k];
-            const double l = (double)d.ic_l[k];
-            for (int i = 0; i < d.nIC; i++) {
-                const double dot = row_dot(d.wmr + i * lut_stride + (size_t)o * d.nS, yr, d.nS, vec4);
-                z[(size_t)i * d.n + k] = l * dot * (double)d.eval[f + (size_t)i * d.nF];
+template <int NJ>
+__device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (&yv)[NJ])
+{
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) a = fma((double)raw[j], yv[j], a);
+    return a;
+}
+
+// Reduce ring entries [tb, tb+8) over the 64 lanes; store the ones < cnt:
+//   IC     : out[i*m + base + t] = l * dot   (eval is applied per fibre in pass 2)
+//   EC/ISO : out[col + i*xs]     = dot
+__device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int lane, double l,
+                                       uint32_t colk, bool ic, uint32_t m, double *__restrict__ out,
+                                       size_t xs, int i, uint32_t base)
+{
+    const int te = tb + (lane >> 3);
+    const double2 *row = reinterpret_cast<const double2 *>(ring + (size_t)(te % kRing) * kRingStride + 8 * (lane & 7));
+    double q = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const double2 v = row[u];
+        q += v.x;
+        q += v.y;
+    }
+    q += __shfl_xor(q, 1);
+    q += __shfl_xor(q, 2);
+    q += __shfl_xor(q, 4);
+    const int src = te < kWave ? te : 0;
+    const double le = __shfl(l, src);
+    const uint32_t ce = (uint32_t)__shfl((int)colk, src);
+    if ((lane & 7) == 0 && te < cnt) {
+        if (ic)
+            out[(size_t)i * m + base + te] = le * q;            // (setup_operator.py:412-428)
+        else
+            out[(size_t)ce + (size_t)i * xs] = q;               // (setup_operator.py:444-563)
+    }
+}
+
+// One compartment of the voxel: per entry a partial dot product into the ring,
+// flushed every 8 entries.  Rows are pipelined as in a_comp.
+template <int NJ>
+__device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, uint32_t b, uint32_t e,
+                                        double *__restrict__ out, size_t xs, int nrf,
+                                        const float *__restrict__ lut, size_t lut_rf, int nSp, int lane,
+                                        double *ring)
+{
+    const bool ic = cp.len != nullptr;
+    for (int i = 0; i < nrf; i++) {
+        const float *L = lut + (size_t)i * lut_rf + lane;
+        for (uint32_t base = b; base < e; base += kWave) {
+            const uint32_t k = base + lane;
+            const int cnt = (int)min((uint32_t)kWave, e - base);
+            uint32_t o = 0, colk = 0;
+            double l = 0.0;
+            if (k < e) {
+                if (cp.o) o = stream_load(cp.o + k);
+                colk = stream_load(cp.col + k);
+                if (ic) l = (double)stream_load(cp.len + k);
+            }
+            float ra[NJ], rb[NJ], rc[NJ];
+            int t = 0;
+            fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, 0) * nSp);
+            if (cnt > 1) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, 1) * nSp);
+#define AT_STAGE(CUR, NEXT2)                                                                              \
+    {                                                                                                     \
+        if (t + 2 < cnt) fetch_row<NJ>(NEXT2, L + (size_t)__builtin_amdgcn_readlane((int)o, t + 2) * nSp); \
+        ring[(size_t)(t % kRing) * kRingStride + lane] = dot_row<NJ>(CUR, yv);                            \
+        if ((t & 7) == 7) {                                                                               \
+            __builtin_amdgcn_wave_barrier();                                                              \
+            flush8(ring, t - 7, cnt, lane, l, colk, ic, cp.m, out, xs, i, base);                          \
+            __builtin_amdgcn_wave_barrier();                                                              \
+        }                                                                                                 \
+        if (++t >= cnt) break;                                                                            \
+    }
+            while (true) {
+                AT_STAGE(ra, rc)
+                AT_STAGE(rb, ra)
+                AT_STAGE(rc, rb)
+            }
+#undef AT_STAGE
+            if (cnt & 7) {                        // last partial group of 8
+                __builtin_amdgcn_wave_barrier();
+                flush8(ring, cnt & ~7, cnt, lane, l, colk, ic, cp.m, out, xs, i, base);
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
-    // extra-cellular: x[nIC nF + i nE + e] = LUT_EC_i[o_e] . y[v]       (setup_operator.py:444-504)
-    if (d.nEC > 0) {
-        const uint32_t b = d.ec_ptr[v], e = d.ec_ptr[v + 1];
-        double *xe = x + (size_t)d.nIC * d.nF;
-        for (uint32_t k = b + lane; k < e; k += kWave) {
-            const uint32_t eo = d.ec_e[k];
-            const int o = d.ec_o[k];
-            for (int i = 0; i < d.nEC; i++)
-                xe[eo + (size_t)i * d.nE] = row_dot(d.wmh + i * lut_stride + (size_t)o * d.nS, yr, d.nS, vec4);
-        }
-    }
-    // isotropic: x[nIC nF + nEC nE + i nV + k] = LUT_ISO_i . y[v]       (setup_operator.py:506-563)
-    if (d.nISO > 0) {
-        const uint32_t b = d.iso_ptr[v], e = d.iso_ptr[v + 1];
-        double *xi = x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE;
-        for (uint32_t k = b + lane; k < e; k += kWave) {
-            const uint32_t ko = d.iso_k[k];
-            for (int i = 0; i < d.nISO; i++)
-                xi[ko + (size_t)i * d.nI] = row_dot(d.iso + (size_t)i * d.nS, yr, d.nS, vec4);
-        }
-    }
 }
 
-// A^T y pass 2: x[f + i nF] = sum over the fibre's segments, ascending voxel order.
+template <int NJ>
+__global__ __launch_bounds__(kBlock) void k_At_fp64(DevDict d, const double *__restrict__ y,
+                                                    double *__restrict__ z, double *__restrict__ x)
+{
+    __shared__ __align__(16) double rings[kWavesPerBlock][kRing * kRingStride];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int v = voxel_id();
+    if (v >= d.nVOX) return;
+    double *ring = rings[w];
+    double yv[NJ];
+    const double *yr = y + (size_t)v * d.nS;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int s = lane + kWave * j;
+        yv[j] = (s < d.nS) ? stream_load(yr + s) : 0.0;
+    }
+    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
+    at_comp<NJ>(yv, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], z, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
+    if (d.nEC > 0)
+        at_comp<NJ>(yv, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
+                    d.wmh, lut_rf, d.nSp, lane, ring);
+    if (d.nISO > 0)
+        at_comp<NJ>(yv, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
+                    (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane, ring);
+}
+
+// A^T y pass 2: x[f + i nF] = eval * sum of the fibre's products, ascending voxel
+// order.  eval is 0 or 1, so applying it once per fibre equals the reference's
+// per-segment product (setup_operator.py:428).
 __global__ __launch_bounds__(kBlock) void k_At_fibre_fp64(DevDict d, const double *__restrict__ z,
                                                           double *__restrict__ x)
 {
@@ -275,9 +353,18 @@ __global__ __launch_bounds__(kBlock) void k_At_fibre_fp64(DevDict d, const doubl
         const int i = (int)(g / d.nF);
         const uint32_t f = (uint32_t)(g - (size_t)i * d.nF);
         const double *zi = z + (size_t)i * d.n;
+        const uint32_t kb = d.fib_ptr[f], ke = d.fib_ptr[f + 1];
         double acc = 0.0;
-        for (uint32_t k = d.fib_ptr[f]; k < d.fib_ptr[f + 1]; k++) acc += zi[d.fib_perm[k]];
-        x[g] = acc;
+        uint32_t k = kb;
+        for (; k + 8 <= ke; k += 8) {      // 8 gathers in flight, summed in order
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = zi[d.fib_perm[k + u]];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc += v[u];
+        }
+        for (; k < ke; k++) acc += zi[d.fib_perm[k]];
+        x[g] = acc * (double)d.eval[g];
     }
 }
 
@@ -306,6 +393,14 @@ bool counting_sort(const uint32_t *keys, size_t n, uint32_t nkeys, std::vector<u
     return true;
 }
 
+// LUT with rows widened from nS to nSp samples (zeros)
+std::vector<float> pad_rows(const float *src, size_t rows, int nS, int nSp)
+{
+    std::vector<float> out(rows * (size_t)nSp, 0.0f);
+    for (size_t r = 0; r < rows; r++) std::memcpy(&out[r * nSp], src + r * nS, (size_t)nS * sizeof(float));
+    return out;
+}
+
 }  // namespace
 
 struct commit_hip_plan {
@@ -315,22 +410,32 @@ struct commit_hip_plan {
     int nS_ref = 0;            // nS as the reference reports it (rows = nVOX * nS_ref)
     DevDict d{};
     std::vector<void *> allocs;
-    double *zbuf = nullptr;    // [nIC][n] per-segment products of A^T pass 1
+    double *zbuf = nullptr;    // [nIC][n] per-entry products of A^T pass 1
+    double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
+    bool eval_ones = true;
     int64_t bytes = 0;
 };
 
 namespace {
 
+int dev_alloc(commit_hip_plan *p, size_t bytes, void **out)
+{
+    *out = nullptr;
+    if (bytes == 0) return 0;
+    if (hipMalloc(out, bytes) != hipSuccess)
+        return fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    p->allocs.push_back(*out);
+    p->bytes += (int64_t)bytes;
+    return 0;
+}
+
 template <typename T>
 int upload(commit_hip_plan *p, const T *host, size_t count, const T **dst)
 {
     *dst = nullptr;
-    if (count == 0) return 0;
     void *ptr = nullptr;
-    if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess)
-        return fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
-    p->allocs.push_back(ptr);
-    p->bytes += (int64_t)(count * sizeof(T));
+    int rc = dev_alloc(p, count * sizeof(T), &ptr);
+    if (rc || !ptr) return rc;
     HIP_TRY(hipMemcpy(ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
     *dst = static_cast<const T *>(ptr);
     return 0;
@@ -342,15 +447,24 @@ void release(commit_hip_plan *p)
     p->allocs.clear();
 }
 
+unsigned vox_blocks(const DevDict &d) { return (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock); }
+
 int launch_A(commit_hip_plan *p, const double *x, double *y, hipStream_t st)
 {
     const DevDict &d = p->d;
-    const int nj = (d.nS + kWave - 1) / kWave;
-    const unsigned grid = (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock);
+    const unsigned grid = vox_blocks(d);
     if (grid == 0) return 0;
-    switch (nj) {
+    const double *xic = x;
+    if (!p->eval_ones && d.nF > 0 && d.nIC > 0) {
+        const size_t n = (size_t)d.nF * d.nIC;
+        hipLaunchKernelGGL(k_x_eval_fp64, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
+                           x, d.eval, p->xic, n);
+        HIP_TRY(hipGetLastError());
+        xic = p->xic;
+    }
+    switch (d.nSp / kWave) {
 #define CASE(NJ) \
-    case NJ: hipLaunchKernelGGL(k_A_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, x, y); break;
+    case NJ: hipLaunchKernelGGL(k_A_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, x, xic, y); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
         default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", d.nS, kMaxNJ * kWave);
@@ -362,19 +476,52 @@ int launch_A(commit_hip_plan *p, const double *x, double *y, hipStream_t st)
 int launch_At(commit_hip_plan *p, const double *y, double *x, hipStream_t st)
 {
     const DevDict &d = p->d;
-    const int nSp = (d.nS + 1) & ~1;   // keep each wave's row 16-byte aligned
-    const unsigned grid = (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock);
+    const unsigned grid = vox_blocks(d);
     if (grid > 0) {
-        const size_t shm = (size_t)kWavesPerBlock * nSp * sizeof(double);
-        hipLaunchKernelGGL(k_At_voxel_fp64, dim3(grid), dim3(kBlock), shm, st, d, y, p->zbuf, x, nSp);
+        switch (d.nSp / kWave) {
+#define CASE(NJ) \
+    case NJ: hipLaunchKernelGGL(k_At_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, y, p->zbuf, x); break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", d.nS, kMaxNJ * kWave);
+        }
         HIP_TRY(hipGetLastError());
     }
     const size_t total = (size_t)d.nF * d.nIC;
     if (total > 0) {
-        unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 65536);
+        const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
         hipLaunchKernelGGL(k_At_fibre_fp64, dim3(g2), dim3(kBlock), 0, st, d, p->zbuf, x);
         HIP_TRY(hipGetLastError());
     }
+    return 0;
+}
+
+// Voxel-major CSR of one compartment: entries stably sorted by voxel.
+// col_in null: the column is the entry's input position.
+int build_comp(commit_hip_plan *p, const uint32_t *vox, size_t m, uint32_t nVOX, const uint16_t *o,
+               const uint32_t *col_in, const float *len, Comp &c, std::vector<uint32_t> *perm_out)
+{
+    std::vector<uint32_t> perm, ptr;
+    if (!counting_sort(vox, m, nVOX, perm, ptr)) return fail(COMMIT_HIP_EINVAL, "voxel index >= nVOX=%u", nVOX);
+    std::vector<uint32_t> col(m);
+    for (size_t k = 0; k < m; k++) col[k] = col_in ? col_in[perm[k]] : perm[k];
+    int rc;
+    c.m = (uint32_t)m;
+    if ((rc = upload(p, ptr.data(), ptr.size(), &c.ptr))) return rc;
+    if ((rc = upload(p, col.data(), m, &c.col))) return rc;
+    c.o = nullptr;
+    if (o) {
+        std::vector<uint16_t> oo(m);
+        for (size_t k = 0; k < m; k++) oo[k] = o[perm[k]];
+        if ((rc = upload(p, oo.data(), m, &c.o))) return rc;
+    }
+    c.len = nullptr;
+    if (len) {
+        std::vector<float> ll(m);
+        for (size_t k = 0; k < m; k++) ll[k] = len[perm[k]];
+        if ((rc = upload(p, ll.data(), m, &c.len))) return rc;
+    }
+    if (perm_out) perm_out->swap(perm);
     return 0;
 }
 
@@ -384,7 +531,7 @@ extern "C" {
 
 const char *commit_hip_last_error(void) { return g_err.c_str(); }
 
-const char *commit_hip_version(void) { return "commit_hip 0.1 (gfx950)"; }
+const char *commit_hip_version(void) { return "commit_hip 0.3 (gfx950)"; }
 
 int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, int n, int nE, int nV,
                            int nVOX, const uint32_t *ICf, const uint32_t *ICeval, const uint32_t *ICv,
@@ -406,7 +553,7 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
         return fail(COMMIT_HIP_EUNSUPPORTED, "LUT-free operator needs nIC=1, nEC=0, nISO<=1");
     if (!nolut && nS > kMaxNJ * kWave)
         return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", nS, kMaxNJ * kWave);
-    if ((n > 0 && nIC == 0) || (n > 0 && (!ICf || !ICv || !ICo || !ICl || !ICeval)))
+    if ((n > 0 && nIC == 0) || (n > 0 && (!ICf || !ICv || !ICo || !ICl)) || (nF > 0 && nIC > 0 && !ICeval))
         return fail(COMMIT_HIP_EINVAL, "IC arrays missing");
     if (nEC > 0 && nE > 0 && (!ECv || !ECo)) return fail(COMMIT_HIP_EINVAL, "EC arrays missing");
     if (nISO > 0 && nV > 0 && !ISOv) return fail(COMMIT_HIP_EINVAL, "ISO arrays missing");
@@ -419,6 +566,8 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
     if (nEC > 0)
         for (int e = 0; e < nE; e++)
             if ((int)ECo[e] >= ndirs) return fail(COMMIT_HIP_EINVAL, "ECo[%d]=%u >= ndirs=%d", e, ECo[e], ndirs);
+    for (size_t q = 0; q < (size_t)nF * nIC; q++)
+        if (ICeval[q] > 1u) return fail(COMMIT_HIP_EINVAL, "ICeval[%zu]=%u is not 0 or 1", q, ICeval[q]);
 
     HIP_TRY(hipSetDevice(device));
     commit_hip_plan *p = new commit_hip_plan();
@@ -428,6 +577,7 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
     p->nS_ref = nS;
     DevDict &d = p->d;
     d.nS = nolut ? 1 : nS;
+    d.nSp = ((d.nS + kWave - 1) / kWave) * kWave;
     d.ndirs = nolut ? 1 : ndirs;
     d.nF = nF;
     d.n = n;
@@ -439,74 +589,56 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
     d.nISO = (int)nISO;
     int rc = 0;
     {
-        // IC: voxel-major CSR (stable), then the fibre-major permutation of it
-        std::vector<uint32_t> perm, ptr;
-        if (!counting_sort(ICv, (size_t)n, (uint32_t)nVOX, perm, ptr)) {
-            rc = fail(COMMIT_HIP_EINVAL, "ICv holds a voxel index >= nVOX=%d", nVOX);
-            goto done;
+        std::vector<uint32_t> perm;
+        std::vector<uint16_t> zero_o;
+        const uint16_t *o = ICo;
+        if (nolut) {
+            zero_o.assign((size_t)n, 0);
+            o = zero_o.data();
         }
-        std::vector<uint32_t> f(n);
-        std::vector<uint16_t> o(n);
-        std::vector<float> l(n);
-        for (int k = 0; k < n; k++) {
-            f[k] = ICf[perm[k]];
-            o[k] = nolut ? 0 : ICo[perm[k]];
-            l[k] = ICl[perm[k]];
-        }
-        if ((rc = upload(p, ptr.data(), ptr.size(), &d.ic_ptr))) goto done;
-        if ((rc = upload(p, f.data(), (size_t)n, &d.ic_f))) goto done;
-        if ((rc = upload(p, o.data(), (size_t)n, &d.ic_o))) goto done;
-        if ((rc = upload(p, l.data(), (size_t)n, &d.ic_l))) goto done;
-        std::vector<uint32_t> fperm, fptr;
+        if ((rc = build_comp(p, ICv, (size_t)n, (uint32_t)nVOX, o, ICf, ICl, d.ic, &perm))) goto done;
+        // fibre-major permutation of the voxel-sorted entries
+        std::vector<uint32_t> f(n), fperm, fptr;
+        for (int k = 0; k < n; k++) f[k] = ICf[perm[k]];
         counting_sort(f.data(), (size_t)n, (uint32_t)nF, fperm, fptr);
         if ((rc = upload(p, fptr.data(), fptr.size(), &d.fib_ptr))) goto done;
         if ((rc = upload(p, fperm.data(), (size_t)n, &d.fib_perm))) goto done;
         if ((rc = upload(p, ICeval, (size_t)nF * nIC, &d.eval))) goto done;
+        for (size_t q = 0; q < (size_t)nF * nIC; q++)
+            if (ICeval[q] != 1u) { p->eval_ones = false; break; }
     }
-    {
-        // EC: voxel-major CSR; ec_e keeps the original peak index (x column)
-        std::vector<uint32_t> perm, ptr;
-        const size_t ne = (size_t)d.nE;
-        if (!counting_sort(ne ? ECv : nullptr, ne, (uint32_t)nVOX, perm, ptr)) {
-            rc = fail(COMMIT_HIP_EINVAL, "ECv holds a voxel index >= nVOX=%d", nVOX);
-            goto done;
-        }
-        std::vector<uint16_t> o(ne);
-        for (size_t k = 0; k < ne; k++) o[k] = ECo[perm[k]];
-        if ((rc = upload(p, ptr.data(), ptr.size(), &d.ec_ptr))) goto done;
-        if ((rc = upload(p, perm.data(), ne, &d.ec_e))) goto done;
-        if ((rc = upload(p, o.data(), ne, &d.ec_o))) goto done;
-    }
-    {
-        std::vector<uint32_t> perm, ptr;
-        const size_t ni = (size_t)d.nI;
-        if (!counting_sort(ni ? ISOv : nullptr, ni, (uint32_t)nVOX, perm, ptr)) {
-            rc = fail(COMMIT_HIP_EINVAL, "ISOv holds a voxel index >= nVOX=%d", nVOX);
-            goto done;
-        }
-        if ((rc = upload(p, ptr.data(), ptr.size(), &d.iso_ptr))) goto done;
-        if ((rc = upload(p, perm.data(), ni, &d.iso_k))) goto done;
-    }
+    if (nEC > 0 && (rc = build_comp(p, ECv, (size_t)d.nE, (uint32_t)nVOX, ECo, nullptr, nullptr, d.ec, nullptr)))
+        goto done;
+    if (nISO > 0 && (rc = build_comp(p, ISOv, (size_t)d.nI, (uint32_t)nVOX, nullptr, nullptr, nullptr, d.iso, nullptr)))
+        goto done;
     if (nolut) {
-        const float one = 1.0f;
-        if ((rc = upload(p, &one, 1, &d.wmr))) goto done;
-        if (nISO) { if ((rc = upload(p, &one, 1, &d.iso))) goto done; }
+        std::vector<float> row((size_t)d.nSp, 0.0f);
+        row[0] = 1.0f;
+        if ((rc = upload(p, row.data(), row.size(), &d.wmr))) goto done;
+        if (nISO && (rc = upload(p, row.data(), row.size(), &d.isol))) goto done;
     } else {
-        const size_t lut = (size_t)ndirs * nS;
-        if (nIC && (rc = upload(p, wmrSFP, lut * nIC, &d.wmr))) goto done;
-        if (nEC && (rc = upload(p, wmhSFP, lut * nEC, &d.wmh))) goto done;
-        if (nISO && (rc = upload(p, isoSFP, (size_t)nS * nISO, &d.iso))) goto done;
+        if (nIC) {
+            const std::vector<float> t = pad_rows(wmrSFP, (size_t)nIC * ndirs, nS, d.nSp);
+            if ((rc = upload(p, t.data(), t.size(), &d.wmr))) goto done;
+        }
+        if (nEC) {
+            const std::vector<float> t = pad_rows(wmhSFP, (size_t)nEC * ndirs, nS, d.nSp);
+            if ((rc = upload(p, t.data(), t.size(), &d.wmh))) goto done;
+        }
+        if (nISO) {
+            const std::vector<float> t = pad_rows(isoSFP, (size_t)nISO, nS, d.nSp);
+            if ((rc = upload(p, t.data(), t.size(), &d.isol))) goto done;
+        }
     }
     if (n > 0 && nIC > 0) {
         void *z = nullptr;
-        const size_t zb = (size_t)n * nIC * sizeof(double);
-        if (hipMalloc(&z, zb) != hipSuccess) {
-            rc = fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed", zb);
-            goto done;
-        }
-        p->allocs.push_back(z);
-        p->bytes += (int64_t)zb;
+        if ((rc = dev_alloc(p, (size_t)n * nIC * sizeof(double), &z))) goto done;
         p->zbuf = static_cast<double *>(z);
+    }
+    if (nF > 0 && nIC > 0) {
+        void *xi = nullptr;
+        if ((rc = dev_alloc(p, (size_t)nF * nIC * sizeof(double), &xi))) goto done;
+        p->xic = static_cast<double *>(xi);
     }
 done:
     if (rc) {
@@ -532,9 +664,14 @@ int commit_hip_plan_set_eval(commit_hip_plan *plan, const uint32_t *ICeval)
     if (!plan || !ICeval) return fail(COMMIT_HIP_EINVAL, "NULL argument");
     const size_t cnt = (size_t)plan->d.nF * plan->d.nIC;
     if (cnt == 0) return 0;
+    for (size_t q = 0; q < cnt; q++)
+        if (ICeval[q] > 1u) return fail(COMMIT_HIP_EINVAL, "ICeval[%zu]=%u is not 0 or 1", q, ICeval[q]);
     HIP_TRY(hipSetDevice(plan->device));
     HIP_TRY(hipMemcpy(const_cast<uint32_t *>(plan->d.eval), ICeval, cnt * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
+    plan->eval_ones = true;
+    for (size_t q = 0; q < cnt; q++)
+        if (ICeval[q] != 1u) { plan->eval_ones = false; break; }
     return 0;
 }
 
