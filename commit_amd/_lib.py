@@ -28,11 +28,34 @@ SIGNATURES = {
     "commit_hip_plan_device_bytes": (ctypes.c_int64, [_P]),
     "commit_hip_A": (_I, [_P, _P, _P, _P]),
     "commit_hip_At": (_I, [_P, _P, _P, _P]),
+    "commit_hip_A_residual": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "commit_hip_At_weighted": (_I, [_P, _P, _P, _P, _P]),
+    "commit_hip_prox": (_I, [_P, ctypes.c_int64, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P]),
+    "commit_hip_fista_sums": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
+                                   _P, _P, _P]),
+    "commit_hip_axpby": (_I, [ctypes.c_int64, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P]),
     "commit_hip_last_error": (ctypes.c_char_p, []),
     "commit_hip_version": (ctypes.c_char_p, []),
 }
 
 _lib = None
+
+REG_NONE, REG_LASSO, REG_GROUP_LASSO, REG_SPARSE_GROUP_LASSO = 0, 1, 2, 3
+SUMS_BLOCKS = 256
+
+
+class ProxParams(ctypes.Structure):
+    """commit_hip_prox_params (include/commit_hip.h)."""
+    _fields_ = [("start", ctypes.c_int64 * 3), ("size", ctypes.c_int64 * 3), ("reg", ctypes.c_int * 3),
+                ("nn", ctypes.c_int * 3), ("weighted", ctypes.c_int * 3), ("lam", ctypes.c_double * 3),
+                ("lam_group", ctypes.c_double), ("w", ctypes.c_void_p), ("ngroups", ctypes.c_int64),
+                ("group_ptr", ctypes.c_void_p), ("group_idx", ctypes.c_void_p), ("group_w", ctypes.c_void_p),
+                ("groups_overlap", ctypes.c_int)]
+
+
+def sums_work(ngroups):
+    """Doubles of workspace commit_hip_fista_sums needs (COMMIT_HIP_SUMS_WORK)."""
+    return 8 * SUMS_BLOCKS + int(ngroups)
 
 
 class CommitHipError(RuntimeError):

@@ -43,9 +43,9 @@
 #include <vector>
 
 #include "../../include/commit_hip.h"
+#include "common.h"
 
-namespace {
-
+namespace commit_internal {
 thread_local std::string g_err;
 
 int fail(int code, const char *fmt, ...)
@@ -58,17 +58,12 @@ int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
+}  // namespace commit_internal
 
-#define HIP_TRY(expr)                                                                           \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            return fail(COMMIT_HIP_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));    \
-    } while (0)
+using commit_internal::fail;
 
-constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
-constexpr int kBlock = kWave * kWavesPerBlock;
+namespace {
+
 constexpr int kMaxNJ = 8;            // nS <= 512 samples per voxel
 constexpr int kMaxRF = 20;           // the reference's own limit per compartment
 constexpr int kXcds = 8;             // MI355X: 8 XCDs, work-groups dealt round-robin
@@ -183,10 +178,19 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
     }
 }
 
+// Output stage of A: y = A x, or the FISTA residual r = sqrtw .* (A x - yobs)
+// with a per-voxel partial of ||r||^2 (reduced later in voxel order).
+struct AOut {
+    double *y;
+    const double *yobs;   // null: plain product
+    const double *sqrtw;  // null: unweighted
+    double *rpart;        // [nVOX] partial sums of r^2, or null
+};
+
 // xic: the IC block of x, or x*eval when eval is not all ones.
 template <int NJ>
 __global__ __launch_bounds__(kBlock) void k_A_fp64(DevDict d, const double *__restrict__ x,
-                                                    const double *__restrict__ xic, double *__restrict__ y)
+                                                    const double *__restrict__ xic, AOut out)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int v = voxel_id();
@@ -202,11 +206,23 @@ __global__ __launch_bounds__(kBlock) void k_A_fp64(DevDict d, const double *__re
     if (d.nISO > 0)
         a_comp<NJ>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
                    (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane);
-    double *yv = y + (size_t)v * d.nS;
+    const size_t row = (size_t)v * d.nS;
+    double sq = 0.0;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int s = lane + kWave * j;
-        if (s < d.nS) __builtin_nontemporal_store(acc[j], yv + s);
+        if (s < d.nS) {
+            double r = acc[j];
+            if (out.yobs) r -= stream_load(out.yobs + row + s);      // (solvers.py:308)
+            if (out.sqrtw) r *= stream_load(out.sqrtw + row + s);    // (solvers.py:306)
+            __builtin_nontemporal_store(r, out.y + row + s);
+            sq = fma(r, r, sq);
+        }
+    }
+    if (out.rpart) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
+        if (lane == 0) out.rpart[v] = sq;
     }
 }
 
@@ -316,6 +332,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
 
 template <int NJ>
 __global__ __launch_bounds__(kBlock) void k_At_fp64(DevDict d, const double *__restrict__ y,
+                                                    const double *__restrict__ sqrtw,
                                                     double *__restrict__ z, double *__restrict__ x)
 {
     __shared__ __align__(16) double rings[kWavesPerBlock][kRing * kRingStride];
@@ -330,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_At_fp64(DevDict d, const double *__r
     for (int j = 0; j < NJ; j++) {
         const int s = lane + kWave * j;
         yv[j] = (s < d.nS) ? stream_load(yr + s) : 0.0;
+        if (sqrtw && s < d.nS) yv[j] *= stream_load(sqrtw + (size_t)v * d.nS + s);   // (solvers.py:267)
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
     at_comp<NJ>(yv, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], z, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
@@ -374,6 +392,7 @@ __global__ void k_zero_fp64(double *p, size_t n)
         p[i] = 0.0;
 }
 
+
 // ---------------------------------------------------------------- host side --
 
 // Stable counting sort of keys in [0, nkeys): perm lists input positions in key
@@ -412,6 +431,7 @@ struct commit_hip_plan {
     std::vector<void *> allocs;
     double *zbuf = nullptr;    // [nIC][n] per-entry products of A^T pass 1
     double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
+    double *rpart = nullptr;   // [nVOX + kSumBlocks] residual partials
     bool eval_ones = true;
     int64_t bytes = 0;
 };
@@ -449,7 +469,7 @@ void release(commit_hip_plan *p)
 
 unsigned vox_blocks(const DevDict &d) { return (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock); }
 
-int launch_A(commit_hip_plan *p, const double *x, double *y, hipStream_t st)
+int launch_A(commit_hip_plan *p, const double *x, AOut out, hipStream_t st)
 {
     const DevDict &d = p->d;
     const unsigned grid = vox_blocks(d);
@@ -464,7 +484,7 @@ int launch_A(commit_hip_plan *p, const double *x, double *y, hipStream_t st)
     }
     switch (d.nSp / kWave) {
 #define CASE(NJ) \
-    case NJ: hipLaunchKernelGGL(k_A_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, x, xic, y); break;
+    case NJ: hipLaunchKernelGGL(k_A_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, x, xic, out); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
         default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", d.nS, kMaxNJ * kWave);
@@ -473,14 +493,14 @@ int launch_A(commit_hip_plan *p, const double *x, double *y, hipStream_t st)
     return 0;
 }
 
-int launch_At(commit_hip_plan *p, const double *y, double *x, hipStream_t st)
+int launch_At(commit_hip_plan *p, const double *y, const double *sqrtw, double *x, hipStream_t st)
 {
     const DevDict &d = p->d;
     const unsigned grid = vox_blocks(d);
     if (grid > 0) {
         switch (d.nSp / kWave) {
 #define CASE(NJ) \
-    case NJ: hipLaunchKernelGGL(k_At_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, y, p->zbuf, x); break;
+    case NJ: hipLaunchKernelGGL(k_At_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, y, sqrtw, p->zbuf, x); break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
             default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", d.nS, kMaxNJ * kWave);
@@ -529,7 +549,7 @@ int build_comp(commit_hip_plan *p, const uint32_t *vox, size_t m, uint32_t nVOX,
 
 extern "C" {
 
-const char *commit_hip_last_error(void) { return g_err.c_str(); }
+const char *commit_hip_last_error(void) { return commit_internal::g_err.c_str(); }
 
 const char *commit_hip_version(void) { return "commit_hip 0.3 (gfx950)"; }
 
@@ -635,6 +655,11 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
         if ((rc = dev_alloc(p, (size_t)n * nIC * sizeof(double), &z))) goto done;
         p->zbuf = static_cast<double *>(z);
     }
+    {
+        void *rp = nullptr;
+        if ((rc = dev_alloc(p, ((size_t)nVOX + kSumBlocks) * sizeof(double), &rp))) goto done;
+        p->rpart = static_cast<double *>(rp);
+    }
     if (nF > 0 && nIC > 0) {
         void *xi = nullptr;
         if ((rc = dev_alloc(p, (size_t)nF * nIC * sizeof(double), &xi))) goto done;
@@ -692,7 +717,7 @@ int commit_hip_A(commit_hip_plan *plan, const void *x, void *y, void *stream)
     HIP_TRY(hipSetDevice(plan->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
     double *yd = static_cast<double *>(y);
-    int rc = launch_A(plan, static_cast<const double *>(x), yd, st);
+    int rc = launch_A(plan, static_cast<const double *>(x), AOut{yd, nullptr, nullptr, nullptr}, st);
     if (rc) return rc;
     if (plan->nolut && plan->nS_ref > 1) {
         // LUT-free rows are indexed by voxel only (setup_operator.py:726-729);
@@ -706,10 +731,36 @@ int commit_hip_A(commit_hip_plan *plan, const void *x, void *y, void *stream)
 
 int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream)
 {
+    return commit_hip_At_weighted(plan, y, nullptr, x, stream);
+}
+
+int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, void *stream)
+{
     if (!plan || !x || !y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (plan->nolut && plan->nS_ref > 1 && sqrtw)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "row weights on a LUT-free operator with nS > 1");
     HIP_TRY(hipSetDevice(plan->device));
-    return launch_At(plan, static_cast<const double *>(y), static_cast<double *>(x),
-                     static_cast<hipStream_t>(stream));
+    return launch_At(plan, static_cast<const double *>(y), static_cast<const double *>(sqrtw),
+                     static_cast<double *>(x), static_cast<hipStream_t>(stream));
+}
+
+int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,
+                          void *res, double *rsq, void *stream)
+{
+    if (!plan || !x || !res || !rsq) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (plan->nolut && plan->nS_ref > 1)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "residual of a LUT-free operator with nS > 1");
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t nv = (size_t)plan->d.nVOX;
+    AOut out{static_cast<double *>(res), static_cast<const double *>(yobs), static_cast<const double *>(sqrtw),
+             plan->rpart};
+    int rc = launch_A(plan, static_cast<const double *>(x), out, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum_stage1, dim3(kSumBlocks), dim3(kBlock), 0, st, plan->rpart, nv, plan->rpart + nv);
+    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(kBlock), 0, st, plan->rpart + nv, kSumBlocks, rsq);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 }  // extern "C"

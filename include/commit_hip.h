@@ -90,6 +90,79 @@ int commit_hip_A(commit_hip_plan *plan, const void *x, void *y, void *stream);
  */
 int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream);
 
+/* ------------------------------------------------------------------------
+ * FISTA building blocks (solvers.py:244-391, proximals.pyx:13-186).
+ * All vectors are DEVICE float64 arrays; reductions are deterministic
+ * (fixed work split and summation order, no atomics).
+ * ---------------------------------------------------------------------- */
+
+/*
+ * res = sqrtw .* (A x - yobs) and *rsq = ||res||^2 (rsq: one device double).
+ * yobs may be NULL (res = A x, *rsq = ||A x||^2, the Lipschitz quotient of
+ * solvers.py:282-286); sqrtw may be NULL (no confidence weights,
+ * solvers.py:238-239).  Fuses the reference's A.dot(x) - y and norm
+ * (solvers.py:265-270, 308-311).
+ */
+int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,
+                          void *res, double *rsq, void *stream);
+
+/* x = A^T (sqrtw .* y); sqrtw may be NULL (solvers.py:266-267, 365-366). */
+int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, void *stream);
+
+/* Regularisation of one fit, the device image of solvers.init_regularisation
+ * (solvers.py:16-206).  Compartments: 0 = IC, 1 = EC, 2 = ISO. */
+#define COMMIT_HIP_REG_NONE 0
+#define COMMIT_HIP_REG_LASSO 1
+#define COMMIT_HIP_REG_GROUP_LASSO 2
+#define COMMIT_HIP_REG_SPARSE_GROUP_LASSO 3
+typedef struct {
+    int64_t start[3], size[3];  /* compartment ranges in x */
+    int reg[3];                 /* COMMIT_HIP_REG_*; group types only for IC */
+    int nn[3];                  /* non-negativity per compartment */
+    int weighted[3];            /* lasso (or the l1 part of sparse GL) uses w */
+    double lam[3];              /* lasso lambda (lambda_1 for sparse GL) */
+    double lam_group;           /* group-lasso lambda (IC) */
+    const double *w;            /* [n] per-coefficient weights, absolute index */
+    int64_t ngroups;
+    const int32_t *group_ptr;   /* [ngroups+1] offsets into group_idx */
+    const int32_t *group_idx;   /* member indices into x */
+    const double *group_w;      /* [ngroups] */
+    int groups_overlap;         /* 1: apply groups one after another, in order */
+} commit_hip_prox_params;
+
+/*
+ * x = prox(xin - step * grad, scale) with the reference's composition
+ * proxIC(proxEC(proxISO(.))) (solvers.py:200-201): soft-thresholding, group
+ * shrinkage (proximals.pyx:103-134, groups in the given order), then
+ * non-negativity (proximals.pyx:13-22).  grad may be NULL (x = prox(xin));
+ * xin may equal x.
+ */
+int commit_hip_prox(const commit_hip_prox_params *p, int64_t n, const double *xin, const double *grad,
+                    double step, double scale, double *x, void *stream);
+
+/* doubles of workspace commit_hip_fista_sums needs, for ngroups groups */
+#define COMMIT_HIP_SUMS_WORK(ngroups) (8 * 256 + (ngroups))
+
+/*
+ * Deterministic sums for one FISTA step (device out[8]; device workspace
+ * `work` of COMMIT_HIP_SUMS_WORK(ngroups) doubles):
+ *   out[0] = sum (x - xhat) * grad      out[1] = sum (x - xhat)^2
+ *   out[2] = sum (x - prev)^2           out[3] = sum x^2
+ *   out[4..6] = sum |w_i x_i| over l1_range[2t]..l1_range[2t+1] (w_i = 1
+ *               unless l1_weighted[t])
+ *   out[7] = sum_g group_w[g] * ||x_g||
+ * xhat/grad or prev may be NULL (their sums are 0); ngroups may be 0.
+ */
+int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const double *grad,
+                          const double *prev, const int64_t *l1_range, const int *l1_weighted,
+                          const double *w, int64_t ngroups, const int32_t *group_ptr,
+                          const int32_t *group_idx, const double *group_w, double *out, double *work,
+                          void *stream);
+
+/* out = a * x + b * y (FISTA extrapolation, solvers.py:360). */
+int commit_hip_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
+                     void *stream);
+
 /* Thread-local message describing the last failure of this thread. */
 const char *commit_hip_last_error(void);
 

@@ -451,29 +451,29 @@ def init_regularisation(rp):
         lam = rp["lambdaIC"]
         if dIC is not None and "coeff_weights_kept" in dIC:
             omegaIC = lambda x: lam * np.linalg.norm(w_all[sIC:zIC] * x[sIC:zIC], 1)
-            base = lambda x, sc: w_soft_thresholding(x, w_all, sc * lam, sIC, zIC)
+            baseIC = lambda x, sc: w_soft_thresholding(x, w_all, sc * lam, sIC, zIC)
         else:
             omegaIC = lambda x: lam * np.linalg.norm(x[sIC:zIC], 1)
-            base = lambda x, sc: soft_thresholding(x, sc * lam, sIC, zIC)
-        proxIC = (lambda x, sc: non_negativity(base(x, sc), sIC, zIC)) if rp.get("nnIC") else base
+            baseIC = lambda x, sc: soft_thresholding(x, sc * lam, sIC, zIC)
+        proxIC = (lambda x, sc: non_negativity(baseIC(x, sc), sIC, zIC)) if rp.get("nnIC") else baseIC
     elif reg in ("group_lasso", "sparse_group_lasso"):
         gidx, gsz = _flatten_groups(dIC["group_idx_kept"])
         gw = dIC["group_weights"]
         if reg == "group_lasso":
             lam_g = rp["lambdaIC"]
             omegaIC = lambda x: omega_group_lasso(x, gidx, gsz, gw, lam_g)
-            base = lambda x, sc: prox_group_lasso(x, gidx, gsz, gw, sc * lam_g)
+            baseIC = lambda x, sc: prox_group_lasso(x, gidx, gsz, gw, sc * lam_g)
         else:
             lam1, lam_g = rp["lambdaIC"][0], rp["lambdaIC"][1]
             if "coeff_weights_kept" in dIC:
                 omegaIC = lambda x: omega_w_sparse_group_lasso(x, w_all, gidx, gsz, gw, lam1, lam_g)
-                base = lambda x, sc: prox_group_lasso(w_soft_thresholding(x, w_all, sc * lam1, sIC, zIC),
+                baseIC = lambda x, sc: prox_group_lasso(w_soft_thresholding(x, w_all, sc * lam1, sIC, zIC),
                                                       gidx, gsz, gw, sc * lam_g)
             else:
                 omegaIC = lambda x: omega_sparse_group_lasso(x, gidx, gsz, gw, lam1, lam_g)
-                base = lambda x, sc: prox_group_lasso(soft_thresholding(x, sc * lam1, sIC, zIC),
+                baseIC = lambda x, sc: prox_group_lasso(soft_thresholding(x, sc * lam1, sIC, zIC),
                                                       gidx, gsz, gw, sc * lam_g)
-        proxIC = (lambda x, sc: non_negativity(base(x, sc), sIC, zIC)) if rp.get("nnIC") else base
+        proxIC = (lambda x, sc: non_negativity(baseIC(x, sc), sIC, zIC)) if rp.get("nnIC") else baseIC
     else:
         raise ValueError("Regularisation not in the list")
 
@@ -484,8 +484,8 @@ def init_regularisation(rp):
     elif reg == "lasso":
         lamE = rp.get("lambdaEC")
         omegaEC = lambda x: lamE * np.linalg.norm(x[sEC:sEC + zEC], 1)
-        base = lambda x, sc: soft_thresholding(x, sc * lamE, sEC, zEC)
-        proxEC = (lambda x, sc: non_negativity(base(x, sc), sEC, zEC)) if rp.get("nnEC") else base
+        baseEC = lambda x, sc: soft_thresholding(x, sc * lamE, sEC, zEC)
+        proxEC = (lambda x, sc: non_negativity(baseEC(x, sc), sEC, zEC)) if rp.get("nnEC") else baseEC
     else:
         raise ValueError("Regularisation not in the list")
 
@@ -497,11 +497,11 @@ def init_regularisation(rp):
         lamI = rp.get("lambdaISO")
         if dISO is not None and "coeff_weights" in dISO:
             omegaISO = lambda x: lamI * np.linalg.norm(w_all[sISO:zISO] * x[sISO:zISO], 1)
-            base = lambda x, sc: w_soft_thresholding(x, w_all, sc * lamI, sISO, zISO)
+            baseISO = lambda x, sc: w_soft_thresholding(x, w_all, sc * lamI, sISO, zISO)
         else:
             omegaISO = lambda x: lamI * np.linalg.norm(x[sISO:sISO + zISO], 1)
-            base = lambda x, sc: soft_thresholding(x, sc * lamI, sISO, zISO)
-        proxISO = (lambda x, sc: non_negativity(base(x, sc), sISO, zISO)) if rp.get("nnISO") else base
+            baseISO = lambda x, sc: soft_thresholding(x, sc * lamI, sISO, zISO)
+        proxISO = (lambda x, sc: non_negativity(baseISO(x, sc), sISO, zISO)) if rp.get("nnISO") else baseISO
     else:
         raise ValueError("Regularisation not in the list")
 

@@ -293,3 +293,21 @@ def test_kernel_normalisation_matches_oracle():
         assert np.array_equal(a[key], b[key])
         assert np.allclose(a[key + "_norm"], b[key + "_norm"], rtol=0, atol=0)
     assert np.isclose(np.linalg.norm(a["wmr"][0, 0]), 1.0, rtol=1e-6)
+
+
+def test_composite_prox_three_compartments():
+    """prox = proxIC(proxEC(proxISO(x))) (solvers.py:200-201): each compartment
+    thresholded with its own lambda, then clipped, against the closed form."""
+    n1, n2, n3 = 7, 5, 6
+    rp = {"startIC": 0, "sizeIC": n1, "startEC": n1, "sizeEC": n2, "startISO": n1 + n2, "sizeISO": n3,
+          "regIC": "lasso", "regEC": "lasso", "regISO": "lasso", "nnIC": True, "nnEC": False, "nnISO": True,
+          "lambdaIC": 0.3, "lambdaEC": 0.2, "lambdaISO": 0.1}
+    reg = oracle.init_regularisation(rp)
+    x = np.random.default_rng(3).standard_normal(n1 + n2 + n3)
+    sc = 0.5
+    st = lambda v, t: np.sign(v) * np.maximum(np.abs(v) - t, 0.0)
+    exp = np.concatenate([np.maximum(st(x[:n1], 0.3 * sc), 0), st(x[n1:n1 + n2], 0.2 * sc),
+                          np.maximum(st(x[n1 + n2:], 0.1 * sc), 0)])
+    assert np.allclose(reg["prox"](x.copy(), sc), exp, rtol=0, atol=1e-15)
+    om = 0.3 * np.abs(x[:n1]).sum() + 0.2 * np.abs(x[n1:n1 + n2]).sum() + 0.1 * np.abs(x[n1 + n2:]).sum()
+    assert reg["omega"](x) == pytest.approx(om, rel=1e-15)
