@@ -294,8 +294,9 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0):
     qfval = prev_obj
     a_grad = residual(grad, with_y=False)
     g_sq = reg.sums(grad)[3]
-    L = (sqrt(a_grad) / sqrt(g_sq)) ** 2
-    step_size = 1.9 / L
+    with np.errstate(divide="ignore", invalid="ignore"):   # numpy semantics of solvers.py:283-286
+        L = (np.sqrt(np.float64(a_grad)) / np.sqrt(np.float64(g_sq))) ** 2
+        step_size = np.float64(1.9) / L
     if verbose == 4:
         print()
         print("      |  1/2||Ax-y||^2      Omega      |  Cost function    Abs error      Rel error    |      Abs x          Rel x    ")

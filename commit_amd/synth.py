@@ -328,3 +328,40 @@ def rician(signal, snr, seed):
     sigma = float(np.mean(signal)) / snr
     return np.sqrt((signal + sigma * rng.standard_normal(signal.size)) ** 2 +
                    (sigma * rng.standard_normal(signal.size)) ** 2)
+
+
+def write_dictionary(raw, path, n_threads=1):
+    """Write raw dictionary arrays in the trk2dictionary on-disk layout
+    (trk2dictionary.pyx:551-650): dictionary_*.dict raw little-endian files,
+    dictionary_tdi.nii.gz / dictionary_mask.nii.gz and dictionary_info.pickle."""
+    import os
+    import pickle
+
+    from . import nifti
+
+    os.makedirs(path, exist_ok=True)
+    files = {"IC_f": np.uint32, "IC_v": np.uint32, "IC_o": np.uint16, "IC_len": np.float32,
+             "TRK_norm": np.float32, "TRK_len": np.float32, "TRK_lenTot": np.float32, "TRK_kept": np.uint8,
+             "EC_v": np.uint32, "EC_o": np.uint16, "ISO_v": np.uint32}
+    for key, dt in files.items():
+        np.ascontiguousarray(raw[key], dtype=dt).astype(np.dtype(dt).newbyteorder("<")).tofile(
+            os.path.join(path, f"dictionary_{key}.dict"))
+    dim = tuple(raw["dim"])
+    tdi = np.bincount(raw["IC_v"], weights=raw["IC_len"].astype(np.float64), minlength=int(np.prod(dim)))
+    tdi = np.reshape(tdi, dim, order="F").astype(np.float32)
+    pix = (raw.get("pixdim", 1.0),) * 3
+    aff = np.diag([pix[0], pix[1], pix[2], 1.0])
+    nifti.save(nifti.Nifti1Image(tdi, aff, pix), os.path.join(path, "dictionary_tdi.nii.gz"))
+    nifti.save(nifti.Nifti1Image((tdi > 0).astype(np.float32), aff, pix), os.path.join(path, "dictionary_mask.nii.gz"))
+    info = {"ndirs": int(raw["dirs"].shape[0]), "n_threads": int(n_threads), "iso_mask": False,
+            "blur_gauss_extent": 0, "blur_core_extent": 0, "n_count": int(raw["TRK_kept"].size)}
+    with open(os.path.join(path, "dictionary_info.pickle"), "wb") as f:
+        pickle.dump(info, f, protocol=2)
+
+
+def write_scheme(path, bvals, bvecs):
+    """Text scheme 'VERSION: BVECTOR' + rows 'gx gy gz b'."""
+    with open(path, "w") as f:
+        f.write("VERSION: BVECTOR\n")
+        for g, b in zip(np.asarray(bvecs), np.asarray(bvals)):
+            f.write(f"{g[0]:.9f} {g[1]:.9f} {g[2]:.9f} {b:.6f}\n")
