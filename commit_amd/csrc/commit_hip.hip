@@ -493,11 +493,11 @@ int launch_A(commit_hip_plan *p, const double *x, AOut out, hipStream_t st)
     return 0;
 }
 
-int launch_At(commit_hip_plan *p, const double *y, const double *sqrtw, double *x, hipStream_t st)
+int launch_At(commit_hip_plan *p, const double *y, const double *sqrtw, double *x, hipStream_t st, int pass = 3)
 {
     const DevDict &d = p->d;
     const unsigned grid = vox_blocks(d);
-    if (grid > 0) {
+    if (grid > 0 && (pass & 1)) {
         switch (d.nSp / kWave) {
 #define CASE(NJ) \
     case NJ: hipLaunchKernelGGL(k_At_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, y, sqrtw, p->zbuf, x); break;
@@ -508,7 +508,7 @@ int launch_At(commit_hip_plan *p, const double *y, const double *sqrtw, double *
         HIP_TRY(hipGetLastError());
     }
     const size_t total = (size_t)d.nF * d.nIC;
-    if (total > 0) {
+    if (total > 0 && (pass & 2)) {
         const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
         hipLaunchKernelGGL(k_At_fibre_fp64, dim3(g2), dim3(kBlock), 0, st, d, p->zbuf, x);
         HIP_TRY(hipGetLastError());
@@ -742,6 +742,17 @@ int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqr
     HIP_TRY(hipSetDevice(plan->device));
     return launch_At(plan, static_cast<const double *>(y), static_cast<const double *>(sqrtw),
                      static_cast<double *>(x), static_cast<hipStream_t>(stream));
+}
+
+int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, int pass, void *stream)
+{
+    if (!plan || !x || !y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (pass < 1 || pass > 3) return fail(COMMIT_HIP_EINVAL, "pass must be 1, 2 or 3");
+    if (plan->nolut && plan->nS_ref > 1 && sqrtw)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "row weights on a LUT-free operator with nS > 1");
+    HIP_TRY(hipSetDevice(plan->device));
+    return launch_At(plan, static_cast<const double *>(y), static_cast<const double *>(sqrtw),
+                     static_cast<double *>(x), static_cast<hipStream_t>(stream), pass);
 }
 
 int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,

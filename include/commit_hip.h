@@ -109,6 +109,15 @@ int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs
 /* x = A^T (sqrtw .* y); sqrtw may be NULL (solvers.py:266-267, 365-366). */
 int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, void *stream);
 
+/*
+ * One pass of A^T y, for timing and profiling each kernel on its own (no
+ * counterpart in the reference): pass 1 = voxel-major products (and the EC/ISO
+ * columns of x), pass 2 = per-fibre sums into the IC columns of x; pass 3 = both
+ * (commit_hip_At_weighted).  Pass 2 reads the scratch pass 1 left in the plan.
+ */
+int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, int pass,
+                       void *stream);
+
 /* Regularisation of one fit, the device image of solvers.init_regularisation
  * (solvers.py:16-206).  Compartments: 0 = IC, 1 = EC, 2 = ISO. */
 #define COMMIT_HIP_REG_NONE 0
