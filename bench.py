@@ -49,7 +49,13 @@ def main():
     ap.add_argument("--workload", default="brain")
     ap.add_argument("--fibres", type=int, default=None)
     ap.add_argument("--cpu-pairs", type=int, default=2, help="timed pairs of the CPU baseline (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 256 MiB device copies (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the operator runs on a single MI355X; see the sharding notes)", file=sys.stderr)
@@ -171,6 +177,20 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration():
+    """Three device-to-device copies of 256 MiB each: a kernel whose HBM
+    traffic is known exactly (256 MiB read + 256 MiB written per launch)."""
+    import torch
+    n = (256 << 20) // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: 3 copies of {n * 8} bytes", flush=True)
 
 
 def measured_traffic(kernel):
