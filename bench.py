@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--workload", default="brain")
     ap.add_argument("--fibres", type=int, default=None)
     ap.add_argument("--cpu-pairs", type=int, default=2, help="timed pairs of the CPU baseline (0: skip)")
+    ap.add_argument("--rhs", type=int, default=16, help="right-hand sides of the batched leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
@@ -143,6 +144,10 @@ def main():
     xs = torch.from_numpy(synth.x_true(A.shape[1], seed + 1)).to(dev)
     ms_A_sparse = time_op(lambda: A.matvec(xs, out=y), reps)
 
+    batched = None
+    if args.rhs > 0:
+        batched = batched_leg(A, At, args.rhs, reps, time_op, ms_A + ms_At, seed, dev)
+
     cpu = None
     if args.cpu_pairs > 0:
         cpu = cpu_baseline(D, K, x.cpu().numpy(), args.cpu_pairs)
@@ -169,6 +174,7 @@ def main():
             "ms_A": round(ms_A, 4), "ms_At": round(ms_At, 4), "ms_At_pass1": round(ms_At1, 4),
             "ms_At_pass2": round(ms_At2, 4), "ms_A_sparse_x": round(ms_A_sparse, 4),
             "algorithmic_bytes_per_product": int(bytes_pass),
+            "batched": batched,
         },
         "roofline": {
             "bound": "hbm", "kernel": dom_name, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
@@ -177,6 +183,26 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def batched_leg(A, At, R, reps, time_op, ms_plain_pair, seed, dev):
+    """R right-hand sides at once (float32 X [ncols, R], Y [nrows, R]) through
+    the matrix-core products; per-RHS pair time against the plain pair."""
+    import torch
+    g = torch.Generator(device=dev).manual_seed(seed + 7)
+    X = torch.rand((A.shape[1], R), dtype=torch.float32, device=dev, generator=g)
+    Y = torch.empty((A.shape[0], R), dtype=torch.float32, device=dev)
+    Xo = torch.empty_like(X)
+    ms_Am = time_op(lambda: A.matmat(X, out=Y), reps)
+    ms_Atm = time_op(lambda: At.matmat(Y, out=Xo), reps)
+    per_rhs = (ms_Am + ms_Atm) / R
+    out = {"nrhs": R, "dtype": "f32 storage, f32 matrix-core products, f64 per-fibre sums",
+           "ms_A": round(ms_Am, 4), "ms_At": round(ms_Atm, 4), "ms_pair_per_rhs": round(per_rhs, 4),
+           "pairs_per_s_per_rhs": round(1e3 / per_rhs, 2),
+           "speedup_vs_plain_per_rhs": round(ms_plain_pair / per_rhs, 3)}
+    print(f"[bench] batched R={R}: A {ms_Am:.3f} ms, A^T {ms_Atm:.3f} ms -> {per_rhs:.3f} ms per RHS pair "
+          f"({out['speedup_vs_plain_per_rhs']:.2f}x the plain pair)", file=sys.stderr, flush=True)
+    return out
 
 
 def copy_calibration():

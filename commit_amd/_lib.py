@@ -31,6 +31,8 @@ SIGNATURES = {
     "commit_hip_A_residual": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_At_weighted": (_I, [_P, _P, _P, _P, _P]),
     "commit_hip_At_pass": (_I, [_P, _P, _P, _P, _I, _P]),
+    "commit_hip_A_multi": (_I, [_P, _P, _P, _I, _I, _P]),
+    "commit_hip_At_multi": (_I, [_P, _P, _P, _P, _I, _I, _P]),
     "commit_hip_prox": (_I, [_P, ctypes.c_int64, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P]),
     "commit_hip_fista_sums": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
                                    _P, _P, _P]),

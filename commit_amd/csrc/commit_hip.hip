@@ -44,6 +44,7 @@
 
 #include "../../include/commit_hip.h"
 #include "common.h"
+#include "plan.h"
 
 namespace commit_internal {
 thread_local std::string g_err;
@@ -66,36 +67,11 @@ namespace {
 
 constexpr int kMaxNJ = 8;            // nS <= 512 samples per voxel
 constexpr int kMaxRF = 20;           // the reference's own limit per compartment
-constexpr int kXcds = 8;             // MI355X: 8 XCDs, work-groups dealt round-robin
+using commit_internal::Comp;
+using commit_internal::DevDict;
 
-// ------------------------------------------------------------------ device --
-struct Comp {                       // one compartment, voxel-major CSR
-    const uint32_t *ptr;            // [nVOX+1]
-    const uint16_t *o;              // [m] orientation (null: ISO, a single row)
-    const uint32_t *col;            // [m] x column inside one RF block
-    const float *len;               // [m] segment length (IC only)
-    uint32_t m;
-};
-
-struct DevDict {
-    Comp ic, ec, iso;
-    const uint32_t *eval;           // [nF*nIC]
-    const uint32_t *fib_ptr;        // [nF+1]
-    const uint32_t *fib_perm;       // [n] IC entry index, fibre-major, ascending voxel
-    const float *wmr, *wmh, *isol;  // padded LUTs: rows of nSp floats
-    int nS, nSp, ndirs, nF, n, nE, nI, nVOX, nIC, nEC, nISO;
-};
-
-// Voxel of this wavefront.  Blocks are dealt to the XCDs round-robin; block b
-// is remapped so that XCD (b % 8) walks a contiguous range of voxels.  The
-// result is made scalar: the compiler cannot prove threadIdx.x >> 6 uniform.
-__device__ __forceinline__ int voxel_id()
-{
-    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
-    const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
-    const int lb = x * q + min(x, r) + b / kXcds;
-    return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (int)(threadIdx.x >> 6));
-}
+// Voxel of this wavefront (one wavefront per voxel).
+__device__ __forceinline__ int voxel_id() { return commit_internal::wave_index(); }
 
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
@@ -422,23 +398,7 @@ std::vector<float> pad_rows(const float *src, size_t rows, int nS, int nSp)
 
 }  // namespace
 
-struct commit_hip_plan {
-    int device = 0;
-    int precision = COMMIT_HIP_FP64;
-    int nolut = 0;
-    int nS_ref = 0;            // nS as the reference reports it (rows = nVOX * nS_ref)
-    DevDict d{};
-    std::vector<void *> allocs;
-    double *zbuf = nullptr;    // [nIC][n] per-entry products of A^T pass 1
-    double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
-    double *rpart = nullptr;   // [nVOX + kSumBlocks] residual partials
-    bool eval_ones = true;
-    int64_t bytes = 0;
-};
-
-namespace {
-
-int dev_alloc(commit_hip_plan *p, size_t bytes, void **out)
+int commit_internal::dev_alloc(commit_hip_plan *p, size_t bytes, void **out)
 {
     *out = nullptr;
     if (bytes == 0) return 0;
@@ -448,6 +408,10 @@ int dev_alloc(commit_hip_plan *p, size_t bytes, void **out)
     p->bytes += (int64_t)bytes;
     return 0;
 }
+
+namespace {
+
+using commit_internal::dev_alloc;
 
 template <typename T>
 int upload(commit_hip_plan *p, const T *host, size_t count, const T **dst)

@@ -1,0 +1,378 @@
+// Multi-right-hand-side products on the matrix cores: Y = A X and X = A^T Y
+// for R columns at once (lambda sweeps, bootstrap signals), FP32 storage.
+//
+// Reference: the single-vector products COMMIT_A / COMMIT_At
+// (setup_operator.py:48-365, 368-710); the reference has no batched form and
+// runs one A.dot per right-hand side (solvers.py:266-270).
+//
+// Per voxel v with entries e (orientation o_e, coefficient c_e[r]) the
+// products are two small dense GEMMs, mapped onto v_mfma_f32_16x16x4f32
+// (D[16x16] += A[16x4] * B[4x16]; lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and receives D[4(l/16)+k][l%16], k = 0..3):
+//   A X   : Y_v[s][r]  = sum_e LUT[o_e][s] * c_e[r]
+//           M = samples (NT tiles of 16), N = 16 RHS, K = entries (4 per step).
+//           Tile m, row i holds sample i*NT + m, so a lane reads its NT
+//           samples of a LUT row as one contiguous vector.
+//   A^T Y : Z[e][r]    = sum_s LUT[o_e][s] * Y_v[s][r]
+//           M = entries (16 per tile), N = 16 RHS, K = samples; lane group g
+//           takes samples [g*4NT, g*4NT+4NT), again contiguous in the row.
+//           IC products go to a per-entry buffer that a second pass sums per
+//           fibre in ascending voxel order (deterministic, no atomics); EC and
+//           ISO columns belong to one voxel and are written directly.
+// One wavefront per (voxel, group of 16 RHS); the LUT row of an entry is read
+// once for 16 RHS instead of once per RHS.  Accumulation is FP32 inside the
+// MFMA over one voxel (A) or one LUT row (A^T); the per-fibre sums of A^T are
+// FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/commit_hip.h"
+#include "common.h"
+#include "plan.h"
+
+using commit_internal::fail;
+
+namespace {
+
+using commit_internal::Comp;
+using commit_internal::DevDict;
+using commit_internal::wave_index;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxNT = 20;          // nS <= 320 for the multi-RHS kernels
+constexpr int kTileR = 16;          // RHS per wavefront (MFMA N)
+
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// NT consecutive floats at p (p is 4*NT-byte aligned when NT is even)
+template <int NT>
+__device__ __forceinline__ void load_run(float (&a)[NT], const float *__restrict__ p)
+{
+    if constexpr (NT % 4 == 0) {
+#pragma unroll
+        for (int m = 0; m < NT / 4; m++) {
+            const float4 v = reinterpret_cast<const float4 *>(p)[m];
+            a[4 * m] = v.x, a[4 * m + 1] = v.y, a[4 * m + 2] = v.z, a[4 * m + 3] = v.w;
+        }
+    } else if constexpr (NT % 2 == 0) {
+#pragma unroll
+        for (int m = 0; m < NT / 2; m++) {
+            const float2 v = reinterpret_cast<const float2 *>(p)[m];
+            a[2 * m] = v.x, a[2 * m + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NT; m++) a[m] = p[m];
+    }
+}
+
+// ------------------------------------------------------------- Y = A X ------
+// One K-step: entries 4t..4t+3 of the current chunk of 64 (lane group g owns
+// entry 4t+g).  a: this lane's NT samples of the entry's LUT row; b: the
+// entry's coefficient for this lane's RHS.
+struct StepIn {
+    const uint32_t *eval;       // IC: eval (null: all ones)
+    const float *X;             // X + rhs (column 0 of the compartment's RF 0 block)
+    const float *L;             // LUT of the RF, + this lane's sample offset
+    size_t xs;                  // column stride between RF blocks
+    int R, nSp, nF, i;
+    bool rhs_ok;
+};
+
+template <int NT>
+__device__ __forceinline__ void a_step(float (&a)[NT], float &b, const StepIn &in, int t, int g, int cnt,
+                                       uint32_t o, uint32_t col, float len)
+{
+    const int ent = 4 * t + g;
+    const uint32_t oe = (uint32_t)__shfl((int)o, ent);
+    const uint32_t ce = (uint32_t)__shfl((int)col, ent);
+    const float le = __shfl(len, ent);
+    float c = 0.0f;
+    if (ent < cnt && in.rhs_ok) {
+        const size_t cc = (size_t)ce + (size_t)in.i * in.xs;
+        c = le * in.X[cc * (size_t)in.R];                       // (setup_operator.py:63-96)
+        if (in.eval && in.eval[ce + (size_t)in.i * in.nF] == 0u) c = 0.0f;
+    }
+    b = c;
+    load_run<NT>(a, in.L + (size_t)(ent < cnt ? oe : 0u) * in.nSp);
+}
+
+template <int NT>
+__device__ __forceinline__ void a_mma(f4 (&acc)[NT], const float (&a)[NT], float b)
+{
+#pragma unroll
+    for (int m = 0; m < NT; m++) acc[m] = mfma4(a[m], b, acc[m]);
+}
+
+template <int NT>
+__device__ __forceinline__ void mm_comp_A(f4 (&acc)[NT], const Comp &cp, uint32_t b, uint32_t e, StepIn in,
+                                          int nrf, const float *__restrict__ lut, size_t lut_rf, int lane)
+{
+    const int g = lane >> 4, q = lane & 15;
+    for (int i = 0; i < nrf; i++) {
+        in.i = i;
+        in.L = lut + (size_t)i * lut_rf + (size_t)q * NT;
+        for (uint32_t base = b; base < e; base += kWave) {
+            const uint32_t k = base + lane;
+            const int cnt = (int)min((uint32_t)kWave, e - base);
+            uint32_t o = 0, col = 0;
+            float len = 1.0f;
+            if (k < e) {
+                if (cp.o) o = __builtin_nontemporal_load(cp.o + k);
+                col = __builtin_nontemporal_load(cp.col + k);
+                if (cp.len) len = __builtin_nontemporal_load(cp.len + k);
+            }
+            const int nsteps = (cnt + 3) >> 2;
+            float a0[NT], a1[NT], b0, b1;
+            a_step<NT>(a0, b0, in, 0, g, cnt, o, col, len);
+            for (int t = 0;; t += 2) {
+                if (t + 1 < nsteps) a_step<NT>(a1, b1, in, t + 1, g, cnt, o, col, len);
+                a_mma<NT>(acc, a0, b0);
+                if (t + 1 >= nsteps) break;
+                if (t + 2 < nsteps) a_step<NT>(a0, b0, in, t + 2, g, cnt, o, col, len);
+                a_mma<NT>(acc, a1, b1);
+                if (t + 2 >= nsteps) break;
+            }
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restrict__ X, float *__restrict__ Y,
+                                                 int R, int nrg, const uint32_t *__restrict__ eval)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_index();
+    const int v = w / nrg, rg = w - v * nrg;
+    if (v >= d.nVOX) return;
+    const int g = lane >> 4, q = lane & 15;
+    const int rhs = rg * kTileR + q;
+    f4 acc[NT];
+#pragma unroll
+    for (int m = 0; m < NT; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
+    StepIn in{eval, X + rhs, nullptr, (size_t)d.nF, R, d.nSp, d.nF, 0, rhs < R};
+    mm_comp_A<NT>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], in, d.nIC, d.wmr, lut_rf, lane);
+    if (d.nEC > 0) {
+        StepIn ie{nullptr, X + (size_t)d.nIC * d.nF * R + rhs, nullptr, (size_t)d.nE, R, d.nSp, d.nF, 0, rhs < R};
+        mm_comp_A<NT>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], ie, d.nEC, d.wmh, lut_rf, lane);
+    }
+    if (d.nISO > 0) {
+        StepIn is{nullptr, X + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R + rhs, nullptr, (size_t)d.nI, R,
+                  0, d.nF, 0, rhs < R};
+        mm_comp_A<NT>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], is, d.nISO, d.isol, (size_t)d.nSp, lane);
+    }
+    if (rhs >= R) return;
+    float *yv = Y + (size_t)v * d.nS * R + rhs;
+#pragma unroll
+    for (int m = 0; m < NT; m++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int s = (4 * g + k) * NT + m;
+            if (s < d.nS) __builtin_nontemporal_store(acc[m][k], yv + (size_t)s * R);
+        }
+    }
+}
+
+// ------------------------------------------------------------ X = A^T Y -----
+// Entries [b, e) of one compartment, 16 per M-tile.  yb: this lane's B
+// operand, Y_v[g*4NT + t][rhs] for t < 4NT.
+//   IC     : z[(i*n + entry)*16 + q] = len * dot
+//   EC/ISO : X[(col + i*xs)*R + rhs]  = dot
+template <int NT>
+__device__ __forceinline__ void mm_comp_At(const float (&yb)[4 * NT], const Comp &cp, uint32_t b, uint32_t e,
+                                           int nrf, const float *__restrict__ lut, size_t lut_rf, int nSp,
+                                           int lane, float *__restrict__ z, uint32_t n, float *__restrict__ Xc,
+                                           size_t xs, int R, bool rhs_ok)
+{
+    const int g = lane >> 4, q = lane & 15;
+    const bool ic = cp.len != nullptr;
+    for (int i = 0; i < nrf; i++) {
+        const float *L = lut + (size_t)i * lut_rf + (size_t)g * 4 * NT;
+        for (uint32_t e0 = b; e0 < e; e0 += 16) {
+            const uint32_t ent = e0 + q;                  // A operand: row q of the tile
+            const uint32_t oe = (ent < e && cp.o) ? cp.o[ent] : 0u;
+            const float *row = L + (size_t)oe * nSp;
+            f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int tb = 0; tb < NT; tb++) {
+                const float4 a = reinterpret_cast<const float4 *>(row)[tb];
+                acc = mfma4(a.x, yb[4 * tb + 0], acc);
+                acc = mfma4(a.y, yb[4 * tb + 1], acc);
+                acc = mfma4(a.z, yb[4 * tb + 2], acc);
+                acc = mfma4(a.w, yb[4 * tb + 3], acc);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {                // D row 4g+k = entry e0+4g+k, column q
+                const uint32_t eo = e0 + 4 * g + k;
+                if (eo < e) {
+                    if (ic) {
+                        const float l = cp.len[eo];       // (setup_operator.py:412-428)
+                        z[((size_t)i * n + eo) * kTileR + q] = l * acc[k];
+                    } else if (rhs_ok) {
+                        Xc[((size_t)cp.col[eo] + (size_t)i * xs) * R] = acc[k];   // (setup_operator.py:444-563)
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__restrict__ Y,
+                                                  const float *__restrict__ sqrtw, float *__restrict__ z,
+                                                  float *__restrict__ X, int R, int rg)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int v = wave_index();
+    if (v >= d.nVOX) return;
+    const int g = lane >> 4, q = lane & 15;
+    const int rhs = rg * kTileR + q;
+    const bool rhs_ok = rhs < R;
+    float yb[4 * NT];
+    const size_t row0 = (size_t)v * d.nS;
+#pragma unroll
+    for (int t = 0; t < 4 * NT; t++) {
+        const int s = g * 4 * NT + t;
+        float yv = 0.0f;
+        if (s < d.nS && rhs_ok) {
+            yv = Y[(row0 + s) * R + rhs];
+            if (sqrtw) yv *= sqrtw[row0 + s];
+        }
+        yb[t] = yv;
+    }
+    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
+    float *Xr = X + rhs;
+    mm_comp_At<NT>(yb, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], d.nIC, d.wmr, lut_rf, d.nSp, lane, z, (uint32_t)d.n,
+                   Xr, 0, R, rhs_ok);
+    if (d.nEC > 0)
+        mm_comp_At<NT>(yb, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], d.nEC, d.wmh, lut_rf, d.nSp, lane, z,
+                       (uint32_t)d.n, Xr + (size_t)d.nIC * d.nF * R, (size_t)d.nE, R, rhs_ok);
+    if (d.nISO > 0)
+        mm_comp_At<NT>(yb, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], d.nISO, d.isol, (size_t)d.nSp, 0, lane, z,
+                       (uint32_t)d.n, Xr + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R, (size_t)d.nI, R,
+                       rhs_ok);
+}
+
+// Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
+// voxel order) of z, in FP64.  16 lanes per (fibre, RF): one 64-byte row of z
+// per entry.
+__global__ __launch_bounds__(kBlock) void k_At_fibre_mm(DevDict d, const float *__restrict__ z,
+                                                        float *__restrict__ X, int R, int rg)
+{
+    const size_t total = (size_t)d.nF * d.nIC * kTileR;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int q = (int)(t & (kTileR - 1));
+        const size_t gcol = t / kTileR;                  // f + i*nF
+        const int i = (int)(gcol / d.nF);
+        const uint32_t f = (uint32_t)(gcol - (size_t)i * d.nF);
+        const float *zi = z + (size_t)i * d.n * kTileR + q;
+        const uint32_t kb = d.fib_ptr[f], ke = d.fib_ptr[f + 1];
+        double acc = 0.0;
+        uint32_t k = kb;
+        for (; k + 8 <= ke; k += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = zi[(size_t)d.fib_perm[k + u] * kTileR];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc += (double)v[u];
+        }
+        for (; k < ke; k++) acc += (double)zi[(size_t)d.fib_perm[k] * kTileR];
+        const int rhs = rg * kTileR + q;
+        if (rhs < R) X[gcol * R + rhs] = d.eval[gcol] ? (float)acc : 0.0f;
+    }
+}
+
+int nt_of(const DevDict &d) { return (d.nS + 15) / 16; }
+
+int check_multi(commit_hip_plan *p, int nrhs, int dtype)
+{
+    if (dtype != COMMIT_HIP_FP32) return fail(COMMIT_HIP_EUNSUPPORTED, "multi-RHS products are built for FP32 only");
+    if (nrhs < 1) return fail(COMMIT_HIP_EINVAL, "nrhs=%d", nrhs);
+    if (p->nolut) return fail(COMMIT_HIP_EUNSUPPORTED, "multi-RHS products of a LUT-free operator");
+    if (nt_of(p->d) > kMaxNT)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "multi-RHS products need nS <= %d (nS=%d)", kMaxNT * 16, p->d.nS);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, int dtype, void *stream)
+{
+    if (!plan || !X || !Y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    int rc = check_multi(plan, nrhs, dtype);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(plan->device));
+    const DevDict &d = plan->d;
+    const int nrg = (nrhs + kTileR - 1) / kTileR;
+    const size_t waves = (size_t)d.nVOX * nrg;
+    if (waves == 0) return 0;
+    if (waves > (size_t)INT32_MAX - kWavesPerBlock) return fail(COMMIT_HIP_EUNSUPPORTED, "too many voxels x RHS groups");
+    const unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    const uint32_t *ev = plan->eval_ones ? nullptr : d.eval;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const float *x = static_cast<const float *>(X);
+    float *y = static_cast<float *>(Y);
+    switch (nt_of(d)) {
+#define CASE(NT) \
+    case NT: hipLaunchKernelGGL(k_A_mm<NT>, dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+        CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
+#undef CASE
+        default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw, void *X, int nrhs, int dtype,
+                        void *stream)
+{
+    if (!plan || !X || !Y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    int rc = check_multi(plan, nrhs, dtype);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(plan->device));
+    const DevDict &d = plan->d;
+    if (!plan->zmulti && d.n > 0 && d.nIC > 0) {
+        void *zp = nullptr;
+        if ((rc = commit_internal::dev_alloc(plan, (size_t)d.n * d.nIC * kTileR * sizeof(float), &zp))) return rc;
+        plan->zmulti = static_cast<float *>(zp);
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const float *y = static_cast<const float *>(Y);
+    const float *w = static_cast<const float *>(sqrtw);
+    float *x = static_cast<float *>(X);
+    const int nrg = (nrhs + kTileR - 1) / kTileR;
+    const unsigned grid = (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock);
+    const size_t total = (size_t)d.nF * d.nIC * kTileR;
+    const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
+    for (int rg = 0; rg < nrg; rg++) {
+        if (grid > 0) {
+            switch (nt_of(d)) {
+#define CASE(NT)                                                                                            \
+    case NT:                                                                                                \
+        hipLaunchKernelGGL(k_At_mm<NT>, dim3(grid), dim3(kBlock), 0, st, d, y, w, plan->zmulti, x, nrhs, rg); \
+        break;
+                CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+                CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
+#undef CASE
+                default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        if (total > 0) {
+            hipLaunchKernelGGL(k_At_fibre_mm, dim3(g2), dim3(kBlock), 0, st, d, plan->zmulti, x, nrhs, rg);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

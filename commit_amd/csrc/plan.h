@@ -1,0 +1,65 @@
+// Device-resident dictionary and plan, shared by the translation units of
+// libcommit_hip.so.  Built by commit_hip_plan_create (commit_hip.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace commit_internal {
+
+constexpr int kXcds = 8;             // MI355X: 8 XCDs, work-groups dealt round-robin
+
+struct Comp {                       // one compartment, voxel-major CSR
+    const uint32_t *ptr;            // [nVOX+1]
+    const uint16_t *o;              // [m] orientation (null: ISO, a single row)
+    const uint32_t *col;            // [m] x column inside one RF block
+    const float *len;               // [m] segment length (IC only)
+    uint32_t m;
+};
+
+struct DevDict {
+    Comp ic, ec, iso;
+    const uint32_t *eval;           // [nF*nIC]
+    const uint32_t *fib_ptr;        // [nF+1]
+    const uint32_t *fib_perm;       // [n] IC entry index, fibre-major, ascending voxel
+    const float *wmr, *wmh, *isol;  // padded LUTs: rows of nSp floats
+    int nS, nSp, ndirs, nF, n, nE, nI, nVOX, nIC, nEC, nISO;
+};
+
+// Global wavefront index.  Blocks are dealt to the XCDs round-robin; block b
+// is remapped so that XCD (b % 8) walks a contiguous range of the work (a
+// spatial slab of voxels, whose fibres, LUT rows and x entries then share that
+// XCD's L2).  The result is made scalar: the compiler cannot prove
+// threadIdx.x >> 6 uniform.
+__device__ __forceinline__ int wave_index()
+{
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+    const int lb = x * q + min(x, r) + b / kXcds;
+    return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (int)(threadIdx.x >> 6));
+}
+
+}  // namespace commit_internal
+
+struct commit_hip_plan {
+    int device = 0;
+    int precision = 0;
+    int nolut = 0;
+    int nS_ref = 0;            // nS as the reference reports it (rows = nVOX * nS_ref)
+    commit_internal::DevDict d{};
+    std::vector<void *> allocs;
+    double *zbuf = nullptr;    // [nIC][n] per-entry products of A^T pass 1
+    double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
+    double *rpart = nullptr;   // [nVOX + kSumBlocks] residual partials
+    float *zmulti = nullptr;   // [nIC][n][16] per-entry products of the multi-RHS A^T pass 1
+    bool eval_ones = true;
+    int64_t bytes = 0;
+};
+
+namespace commit_internal {
+// hipMalloc tracked by the plan (freed by commit_hip_plan_destroy)
+int dev_alloc(commit_hip_plan *p, size_t bytes, void **out);
+}  // namespace commit_internal

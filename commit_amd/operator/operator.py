@@ -85,6 +85,17 @@ class _Plan:
                 ctypes.c_void_p(stream))
         _lib.check(rc, "commit_hip_At" if adjoint else "commit_hip_A")
 
+    def run_multi(self, adjoint, v_in, v_out, nrhs, sqrtw, stream):
+        lib = self._lib
+        if adjoint:
+            rc = lib.commit_hip_At_multi(self.handle, ctypes.c_void_p(v_in.data_ptr()),
+                                         ctypes.c_void_p(sqrtw.data_ptr() if sqrtw is not None else None),
+                                         ctypes.c_void_p(v_out.data_ptr()), nrhs, _lib.FP32, ctypes.c_void_p(stream))
+        else:
+            rc = lib.commit_hip_A_multi(self.handle, ctypes.c_void_p(v_in.data_ptr()),
+                                        ctypes.c_void_p(v_out.data_ptr()), nrhs, _lib.FP32, ctypes.c_void_p(stream))
+        _lib.check(rc, "commit_hip_At_multi" if adjoint else "commit_hip_A_multi")
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -146,6 +157,30 @@ class LinearOperator:
             out = torch.empty(self.shape[0], dtype=self._plan.dtype, device=v_in.device)
         stream = torch.cuda.current_stream(v_in.device).cuda_stream
         self._plan.run(self.adjoint, v_in, out, stream)
+        return out
+
+    def matmat(self, V, out=None, sqrtw=None):
+        """Multi-RHS device product on the matrix cores: V is a contiguous
+        float32 CUDA tensor of shape [shape[1], R] (R right-hand sides, the RHS
+        index fastest); returns [shape[0], R] float32.  For the adjoint,
+        ``sqrtw`` (float32 [nROWS]) weights the rows of V first.  No
+        counterpart in the reference, which calls dot() once per vector."""
+        if V.dim() != 2 or V.shape[0] != self.shape[1]:
+            raise RuntimeError("A.matmat(): dimensions do not match")
+        if V.dtype != torch.float32 or not V.is_cuda or not V.is_contiguous():
+            raise RuntimeError("A.matmat(): expected a contiguous CUDA float32 tensor")
+        R = int(V.shape[1])
+        if out is None:
+            out = torch.empty((self.shape[0], R), dtype=torch.float32, device=V.device)
+        elif out.shape != (self.shape[0], R) or out.dtype != torch.float32 or not out.is_contiguous():
+            raise RuntimeError("A.matmat(): bad output tensor")
+        if sqrtw is not None:
+            if not self.adjoint:
+                raise RuntimeError("A.matmat(): row weights apply to the adjoint only")
+            if sqrtw.numel() != self.nROWS or sqrtw.dtype != torch.float32 or not sqrtw.is_contiguous():
+                raise RuntimeError("A.matmat(): sqrtw must be contiguous float32 [nROWS]")
+        stream = torch.cuda.current_stream(V.device).cuda_stream
+        self._plan.run_multi(self.adjoint, V, out, R, sqrtw, stream)
         return out
 
     def dot(self, v_in):

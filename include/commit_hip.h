@@ -118,6 +118,21 @@ int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqr
 int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, int pass,
                        void *stream);
 
+/*
+ * Multi-right-hand-side products (no counterpart in the reference, which runs
+ * one COMMIT_A / COMMIT_At per vector, solvers.py:266-270): R = nrhs columns
+ * at once, row-major device matrices with the RHS index fastest:
+ *   commit_hip_A_multi  : Y[nrows][R] = A X[ncols][R]
+ *   commit_hip_At_multi : X[ncols][R] = A^T (sqrtw .* Y)   (sqrtw [nrows] or NULL)
+ * dtype must be COMMIT_HIP_FP32 (float32 X, Y, sqrtw; FP32 matrix-core
+ * products, FP64 per-fibre sums in A^T).  Needs nS <= 320 and a LUT-based
+ * plan.  Outputs are fully overwritten.  A^T allocates its per-entry scratch
+ * (16 floats per IC segment and RF) on first use and keeps it in the plan.
+ */
+int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, int dtype, void *stream);
+int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw, void *X, int nrhs, int dtype,
+                        void *stream);
+
 /* Regularisation of one fit, the device image of solvers.init_regularisation
  * (solvers.py:16-206).  Compartments: 0 = IC, 1 = EC, 2 = ISO. */
 #define COMMIT_HIP_REG_NONE 0
