@@ -72,9 +72,12 @@ __device__ __forceinline__ void load_run(float (&a)[NT], const float *__restrict
 }
 
 // ------------------------------------------------------------- Y = A X ------
-// One K-step: entries 4t..4t+3 of the current chunk of 64 (lane group g owns
-// entry 4t+g).  a: this lane's NT samples of the entry's LUT row; b: the
-// entry's coefficient for this lane's RHS.
+// Entries are taken in chunks of 64 (one per lane: orientation, column,
+// length).  K-step t of a chunk covers entries 4t..4t+3; lane group g owns
+// entry 4t+g.  At the start of a chunk every lane gathers the coefficients of
+// all its 16 steps (one latency for the chunk) and the next chunk's entry
+// records are requested; the LUT rows then stream two steps ahead of the
+// matrix-core work.
 struct StepIn {
     const uint32_t *eval;       // IC: eval (null: all ones)
     const float *X;             // X + rhs (column 0 of the compartment's RF 0 block)
@@ -85,141 +88,223 @@ struct StepIn {
 };
 
 template <int NT>
-__device__ __forceinline__ void a_step(float (&a)[NT], float &b, const StepIn &in, int t, int g, int cnt,
-                                       uint32_t o, uint32_t col, float len)
-{
-    const int ent = 4 * t + g;
-    const uint32_t oe = (uint32_t)__shfl((int)o, ent);
-    const uint32_t ce = (uint32_t)__shfl((int)col, ent);
-    const float le = __shfl(len, ent);
-    float c = 0.0f;
-    if (ent < cnt && in.rhs_ok) {
-        const size_t cc = (size_t)ce + (size_t)in.i * in.xs;
-        c = le * in.X[cc * (size_t)in.R];                       // (setup_operator.py:63-96)
-        if (in.eval && in.eval[ce + (size_t)in.i * in.nF] == 0u) c = 0.0f;
-    }
-    b = c;
-    load_run<NT>(a, in.L + (size_t)(ent < cnt ? oe : 0u) * in.nSp);
-}
-
-template <int NT>
 __device__ __forceinline__ void a_mma(f4 (&acc)[NT], const float (&a)[NT], float b)
 {
 #pragma unroll
     for (int m = 0; m < NT; m++) acc[m] = mfma4(a[m], b, acc[m]);
 }
 
-template <int NT>
-__device__ __forceinline__ void mm_comp_A(f4 (&acc)[NT], const Comp &cp, uint32_t b, uint32_t e, StepIn in,
-                                          int nrf, const float *__restrict__ lut, size_t lut_rf, int lane)
+__device__ __forceinline__ void entry_rec(const Comp &cp, uint32_t k, uint32_t e, uint32_t &o, uint32_t &col,
+                                          float &len)
+{
+    o = 0, col = 0, len = 1.0f;
+    if (k < e) {
+        if (cp.o) o = __builtin_nontemporal_load(cp.o + k);
+        col = __builtin_nontemporal_load(cp.col + k);
+        if (cp.len) len = __builtin_nontemporal_load(cp.len + k);
+    }
+}
+
+__device__ __forceinline__ const float *row_of(const StepIn &in, uint32_t oc, int ent, int cnt)
+{
+    const uint32_t oe = (uint32_t)__shfl((int)oc, ent);
+    return in.L + (size_t)(ent < cnt ? oe : 0u) * in.nSp;
+}
+
+// NH: the sample tiles this wavefront owns (of NT = NH * MS); soff: offset of
+// its first sample inside each lane's run of NT.
+template <int NH, int NT>
+__device__ __forceinline__ void mm_comp_A(f4 (&acc)[NH], const Comp &cp, uint32_t b, uint32_t e, StepIn in,
+                                          int nrf, const float *__restrict__ lut, size_t lut_rf, int lane, int soff)
 {
     const int g = lane >> 4, q = lane & 15;
     for (int i = 0; i < nrf; i++) {
         in.i = i;
-        in.L = lut + (size_t)i * lut_rf + (size_t)q * NT;
+        in.L = lut + (size_t)i * lut_rf + (size_t)q * NT + soff;
+        uint32_t o, col;
+        float len;
+        entry_rec(cp, b + lane, e, o, col, len);
         for (uint32_t base = b; base < e; base += kWave) {
-            const uint32_t k = base + lane;
             const int cnt = (int)min((uint32_t)kWave, e - base);
-            uint32_t o = 0, col = 0;
-            float len = 1.0f;
-            if (k < e) {
-                if (cp.o) o = __builtin_nontemporal_load(cp.o + k);
-                col = __builtin_nontemporal_load(cp.col + k);
-                if (cp.len) len = __builtin_nontemporal_load(cp.len + k);
-            }
             const int nsteps = (cnt + 3) >> 2;
-            float a0[NT], a1[NT], b0, b1;
-            a_step<NT>(a0, b0, in, 0, g, cnt, o, col, len);
-            for (int t = 0;; t += 2) {
-                if (t + 1 < nsteps) a_step<NT>(a1, b1, in, t + 1, g, cnt, o, col, len);
-                a_mma<NT>(acc, a0, b0);
-                if (t + 1 >= nsteps) break;
-                if (t + 2 < nsteps) a_step<NT>(a0, b0, in, t + 2, g, cnt, o, col, len);
-                a_mma<NT>(acc, a1, b1);
-                if (t + 2 >= nsteps) break;
+            // coefficients l * x * eval of this lane's entries     (setup_operator.py:63-96)
+            float xv[16];
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const int ent = 4 * t + g;
+                const uint32_t ce = (uint32_t)__shfl((int)col, ent);
+                const float le = __shfl(len, ent);
+                float c = 0.0f;
+                if (ent < cnt && in.rhs_ok) {
+                    const size_t cc = (size_t)ce + (size_t)i * in.xs;
+                    c = le * in.X[cc * (size_t)in.R];
+                    if (in.eval && in.eval[ce + (size_t)i * in.nF] == 0u) c = 0.0f;
+                }
+                xv[t] = c;
             }
+            const uint32_t oc = o;
+            if (base + kWave < e) entry_rec(cp, base + kWave + lane, e, o, col, len);
+            float a0[NH], a1[NH];
+            // (the shuffle runs on every lane; entries past the chunk read row 0)
+#define ROW(t) row_of(in, oc, 4 * (t) + g, cnt)
+            load_run<NH>(a0, ROW(0));
+            if (nsteps > 1) load_run<NH>(a1, ROW(1));
+#pragma unroll
+            for (int t = 0; t < 16; t += 2) {
+                a_mma<NH>(acc, a0, xv[t]);
+                if (t + 1 >= nsteps) break;
+                if (t + 2 < nsteps) load_run<NH>(a0, ROW(t + 2));
+                a_mma<NH>(acc, a1, xv[t + 1]);
+                if (t + 2 >= nsteps) break;
+                if (t + 3 < nsteps) load_run<NH>(a1, ROW(t + 3));
+            }
+#undef ROW
         }
     }
 }
 
-template <int NT>
+// One wavefront per (voxel, RHS group, sample half h < MS): tiles
+// [h*NH, h*NH + NH) of the NT = NH * MS, i.e. samples i*NT + h*NH + m.
+template <int NH, int MS>
 __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restrict__ X, float *__restrict__ Y,
                                                  int R, int nrg, const uint32_t *__restrict__ eval)
 {
+    constexpr int NT = NH * MS;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_index();
-    const int v = w / nrg, rg = w - v * nrg;
+    const int v = w / (nrg * MS), rem = w - v * (nrg * MS);
+    const int rg = rem / MS, h = rem - rg * MS;
     if (v >= d.nVOX) return;
     const int g = lane >> 4, q = lane & 15;
     const int rhs = rg * kTileR + q;
-    f4 acc[NT];
+    const int soff = h * NH;
+    f4 acc[NH];
 #pragma unroll
-    for (int m = 0; m < NT; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int m = 0; m < NH; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
     StepIn in{eval, X + rhs, nullptr, (size_t)d.nF, R, d.nSp, d.nF, 0, rhs < R};
-    mm_comp_A<NT>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], in, d.nIC, d.wmr, lut_rf, lane);
+    mm_comp_A<NH, NT>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], in, d.nIC, d.wmr, lut_rf, lane, soff);
     if (d.nEC > 0) {
         StepIn ie{nullptr, X + (size_t)d.nIC * d.nF * R + rhs, nullptr, (size_t)d.nE, R, d.nSp, d.nF, 0, rhs < R};
-        mm_comp_A<NT>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], ie, d.nEC, d.wmh, lut_rf, lane);
+        mm_comp_A<NH, NT>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], ie, d.nEC, d.wmh, lut_rf, lane, soff);
     }
     if (d.nISO > 0) {
         StepIn is{nullptr, X + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R + rhs, nullptr, (size_t)d.nI, R,
                   0, d.nF, 0, rhs < R};
-        mm_comp_A<NT>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], is, d.nISO, d.isol, (size_t)d.nSp, lane);
+        mm_comp_A<NH, NT>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], is, d.nISO, d.isol, (size_t)d.nSp, lane, soff);
     }
     if (rhs >= R) return;
     float *yv = Y + (size_t)v * d.nS * R + rhs;
 #pragma unroll
-    for (int m = 0; m < NT; m++) {
+    for (int m = 0; m < NH; m++) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int s = (4 * g + k) * NT + m;
+            const int s = (4 * g + k) * NT + soff + m;
             if (s < d.nS) __builtin_nontemporal_store(acc[m][k], yv + (size_t)s * R);
         }
     }
 }
 
 // ------------------------------------------------------------ X = A^T Y -----
-// Entries [b, e) of one compartment, 16 per M-tile.  yb: this lane's B
-// operand, Y_v[g*4NT + t][rhs] for t < 4NT.
+// Entries [b, e) of one compartment, 16 per M-tile.  The 16 LUT rows of a tile
+// (4NT 16-byte chunks each) go straight from memory into a wavefront-private
+// LDS tile (global_load_lds_dwordx4: lane l of a load writes 16 bytes at
+// base + 16 l), chunk-major: slot cc*16 + r holds chunk cc of row r, so each
+// load covers 64 contiguous bytes of 16 rows and the 16 lanes of one operand
+// read hit 16 consecutive slots (no bank conflicts).  A tile's operands are
+// read into registers, the next tile is requested into the same LDS, and the
+// matrix-core work of the current tile runs under that load.
+// yb: this lane's B operand, Y_v[g*4NT + t][rhs] for t < 4NT.
 //   IC     : z[(i*n + entry)*16 + q] = len * dot
 //   EC/ISO : X[(col + i*xs)*R + rhs]  = dot
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int NT>
+__device__ __forceinline__ void at_request(const Comp &cp, uint32_t e0, uint32_t e, const float *__restrict__ Lb,
+                                           int nSp, int lane, float *T)
+{
+    const uint32_t eq = e0 + (lane & 15);                        // row lane%16 of the tile
+    const uint32_t oe = (eq < e && cp.o) ? (uint32_t)cp.o[eq] : 0u;
+    const float *src = Lb + (size_t)oe * nSp + 4 * (lane >> 4);
+#pragma unroll
+    for (int u = 0; u < NT; u++)                                  // chunks 4u .. 4u+3 of every row
+        __builtin_amdgcn_global_load_lds(src + 16 * u, (lds_void *)(T + 256 * u), 16, 0, 0);
+}
+
+// Per-lane record of the 4 D rows (entries e0+4g+k) a tile produces: the
+// length (IC) or the x column (EC/ISO), fetched with the tile's LUT rows.
+__device__ __forceinline__ void at_aux(const Comp &cp, uint32_t e0, uint32_t e, int g, uint32_t (&aux)[4])
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t eo = e0 + 4 * g + k;
+        aux[k] = eo < e ? (cp.len ? __float_as_uint(cp.len[eo]) : cp.col[eo]) : 0u;
+    }
+}
+
+// Write the D tile of entries e0.. (deferred by one tile, so that the wait for
+// the next LUT tile does not also wait for these stores).
+__device__ __forceinline__ void at_store(const f4 &acc, const uint32_t (&aux)[4], bool ic, uint32_t e0, uint32_t e,
+                                         int g, int q, int i, float *__restrict__ z, uint32_t n,
+                                         float *__restrict__ Xc, size_t xs, int R, bool rhs_ok)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {                        // D row 4g+k = entry e0+4g+k, column q
+        const uint32_t eo = e0 + 4 * g + k;
+        if (eo < e) {
+            if (ic)
+                z[((size_t)i * n + eo) * kTileR + q] = __uint_as_float(aux[k]) * acc[k];   // (setup_operator.py:412-428)
+            else if (rhs_ok)
+                Xc[((size_t)aux[k] + (size_t)i * xs) * R] = acc[k];                       // (setup_operator.py:444-563)
+        }
+    }
+}
+
 template <int NT>
 __device__ __forceinline__ void mm_comp_At(const float (&yb)[4 * NT], const Comp &cp, uint32_t b, uint32_t e,
                                            int nrf, const float *__restrict__ lut, size_t lut_rf, int nSp,
                                            int lane, float *__restrict__ z, uint32_t n, float *__restrict__ Xc,
-                                           size_t xs, int R, bool rhs_ok)
+                                           size_t xs, int R, bool rhs_ok, float *T)
 {
     const int g = lane >> 4, q = lane & 15;
     const bool ic = cp.len != nullptr;
     for (int i = 0; i < nrf; i++) {
-        const float *L = lut + (size_t)i * lut_rf + (size_t)g * 4 * NT;
+        const float *Lb = lut + (size_t)i * lut_rf;
+        uint32_t aux[4] = {0u, 0u, 0u, 0u}, auxn[4] = {0u, 0u, 0u, 0u}, auxp[4] = {0u, 0u, 0u, 0u};
+        if (b < e) {
+            at_request<NT>(cp, b, e, Lb, nSp, lane, T);
+            at_aux(cp, b, e, g, aux);
+        }
+        f4 pend = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t pe0 = e;                                 // e: nothing pending
         for (uint32_t e0 = b; e0 < e; e0 += 16) {
-            const uint32_t ent = e0 + q;                  // A operand: row q of the tile
-            const uint32_t oe = (ent < e && cp.o) ? cp.o[ent] : 0u;
-            const float *row = L + (size_t)oe * nSp;
-            f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");     // tile e0 is in LDS
+            __builtin_amdgcn_wave_barrier();
+            float4 a[NT];
+#pragma unroll
+            for (int tb = 0; tb < NT; tb++)                      // chunk g*NT + tb of row q
+                a[tb] = *reinterpret_cast<const float4 *>(T + 4 * ((g * NT + tb) * 16 + q));
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // operands in registers
+            __builtin_amdgcn_wave_barrier();
+            if (pe0 < e) at_store(pend, auxp, ic, pe0, e, g, q, i, z, n, Xc, xs, R, rhs_ok);
+            if (e0 + 16 < e) {
+                at_request<NT>(cp, e0 + 16, e, Lb, nSp, lane, T);
+                at_aux(cp, e0 + 16, e, g, auxn);
+            }
+            f4 acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
 #pragma unroll
             for (int tb = 0; tb < NT; tb++) {
-                const float4 a = reinterpret_cast<const float4 *>(row)[tb];
-                acc = mfma4(a.x, yb[4 * tb + 0], acc);
-                acc = mfma4(a.y, yb[4 * tb + 1], acc);
-                acc = mfma4(a.z, yb[4 * tb + 2], acc);
-                acc = mfma4(a.w, yb[4 * tb + 3], acc);
+                acc0 = mfma4(a[tb].x, yb[4 * tb + 0], acc0);
+                acc1 = mfma4(a[tb].y, yb[4 * tb + 1], acc1);
+                acc0 = mfma4(a[tb].z, yb[4 * tb + 2], acc0);
+                acc1 = mfma4(a[tb].w, yb[4 * tb + 3], acc1);
             }
+            pend = acc0 + acc1;
+            pe0 = e0;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {                // D row 4g+k = entry e0+4g+k, column q
-                const uint32_t eo = e0 + 4 * g + k;
-                if (eo < e) {
-                    if (ic) {
-                        const float l = cp.len[eo];       // (setup_operator.py:412-428)
-                        z[((size_t)i * n + eo) * kTileR + q] = l * acc[k];
-                    } else if (rhs_ok) {
-                        Xc[((size_t)cp.col[eo] + (size_t)i * xs) * R] = acc[k];   // (setup_operator.py:444-563)
-                    }
-                }
-            }
+            for (int k = 0; k < 4; k++) auxp[k] = aux[k], aux[k] = auxn[k];
         }
+        if (pe0 < e) at_store(pend, auxp, ic, pe0, e, g, q, i, z, n, Xc, xs, R, rhs_ok);
     }
 }
 
@@ -228,9 +313,12 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
                                                   const float *__restrict__ sqrtw, float *__restrict__ z,
                                                   float *__restrict__ X, int R, int rg)
 {
+    __shared__ __align__(16) float tiles[kWavesPerBlock][256 * NT];   // 16 rows x 16NT floats per wave
     const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int v = wave_index();
     if (v >= d.nVOX) return;
+    float *T = tiles[w];
     const int g = lane >> 4, q = lane & 15;
     const int rhs = rg * kTileR + q;
     const bool rhs_ok = rhs < R;
@@ -249,14 +337,14 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
     float *Xr = X + rhs;
     mm_comp_At<NT>(yb, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], d.nIC, d.wmr, lut_rf, d.nSp, lane, z, (uint32_t)d.n,
-                   Xr, 0, R, rhs_ok);
+                   Xr, 0, R, rhs_ok, T);
     if (d.nEC > 0)
         mm_comp_At<NT>(yb, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], d.nEC, d.wmh, lut_rf, d.nSp, lane, z,
-                       (uint32_t)d.n, Xr + (size_t)d.nIC * d.nF * R, (size_t)d.nE, R, rhs_ok);
+                       (uint32_t)d.n, Xr + (size_t)d.nIC * d.nF * R, (size_t)d.nE, R, rhs_ok, T);
     if (d.nISO > 0)
         mm_comp_At<NT>(yb, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], d.nISO, d.isol, (size_t)d.nSp, 0, lane, z,
                        (uint32_t)d.n, Xr + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R, (size_t)d.nI, R,
-                       rhs_ok);
+                       rhs_ok, T);
 }
 
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
@@ -312,7 +400,7 @@ int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, 
     HIP_TRY(hipSetDevice(plan->device));
     const DevDict &d = plan->d;
     const int nrg = (nrhs + kTileR - 1) / kTileR;
-    const size_t waves = (size_t)d.nVOX * nrg;
+    const size_t waves = (size_t)d.nVOX * nrg * 2;
     if (waves == 0) return 0;
     if (waves > (size_t)INT32_MAX - kWavesPerBlock) return fail(COMMIT_HIP_EUNSUPPORTED, "too many voxels x RHS groups");
     const unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -320,11 +408,14 @@ int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     const float *x = static_cast<const float *>(X);
     float *y = static_cast<float *>(Y);
-    switch (nt_of(d)) {
-#define CASE(NT) \
-    case NT: hipLaunchKernelGGL(k_A_mm<NT>, dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev); break;
+    // two wavefronts per (voxel, RHS group), each owning half of the sample
+    // tiles (NT rounded up to even; the padded LUT rows hold zeros there)
+    switch ((nt_of(d) + 1) / 2) {
+#define CASE(NH)                                                                                                \
+    case NH:                                                                                                    \
+        hipLaunchKernelGGL((k_A_mm<NH, 2>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev);         \
+        break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-        CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
 #undef CASE
         default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
     }
