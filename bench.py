@@ -144,6 +144,15 @@ def main():
     xs = torch.from_numpy(synth.x_true(A.shape[1], seed + 1)).to(dev)
     ms_A_sparse = time_op(lambda: A.matvec(xs, out=y), reps)
 
+    # FP32 storage mode (float32 x and y, FP64 accumulation) on the same plan
+    from commit_amd import _lib as L
+    A.set_precision(L.FP32)
+    x32, y32, xo32 = x.float(), torch.empty(A.shape[0], dtype=torch.float32, device=dev), xo.float()
+    ms_A32 = time_op(lambda: A.matvec(x32, out=y32), reps)
+    ms_At32 = time_op(lambda: At.matvec(y32, out=xo32), reps)
+    A.set_precision(L.FP64)
+    print(f"[bench] fp32 storage: A x {ms_A32:.3f} ms; A^T y {ms_At32:.3f} ms", file=sys.stderr, flush=True)
+
     batched = None
     if args.rhs > 0:
         batched = batched_leg(A, At, args.rhs, reps, time_op, ms_A + ms_At, seed, dev)
@@ -174,6 +183,9 @@ def main():
             "ms_A": round(ms_A, 4), "ms_At": round(ms_At, 4), "ms_At_pass1": round(ms_At1, 4),
             "ms_At_pass2": round(ms_At2, 4), "ms_A_sparse_x": round(ms_A_sparse, 4),
             "algorithmic_bytes_per_product": int(bytes_pass),
+            "fp32_storage": {"ms_A": round(ms_A32, 4), "ms_At": round(ms_At32, 4),
+                             "pairs_per_s": round(1e3 / (ms_A32 + ms_At32), 2),
+                             "algorithmic_bytes_per_product": int(algorithmic_bytes(D, K, 4, 4))},
             "batched": batched,
         },
         "roofline": {

@@ -26,6 +26,8 @@ SIGNATURES = {
     "commit_hip_plan_set_eval": (_I, [_P, _P]),
     "commit_hip_plan_shape": (_I, [_P, _I64P, _I64P]),
     "commit_hip_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "commit_hip_plan_set_precision": (_I, [_P, _I]),
+    "commit_hip_plan_precision": (_I, [_P]),
     "commit_hip_A": (_I, [_P, _P, _P, _P]),
     "commit_hip_At": (_I, [_P, _P, _P, _P]),
     "commit_hip_A_residual": (_I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -37,6 +39,10 @@ SIGNATURES = {
     "commit_hip_fista_sums": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
                                    _P, _P, _P]),
     "commit_hip_axpby": (_I, [ctypes.c_int64, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P]),
+    "commit_hip_prox_f32": (_I, [_P, ctypes.c_int64, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P]),
+    "commit_hip_fista_sums_f32": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
+                                       _P, _P, _P]),
+    "commit_hip_axpby_f32": (_I, [ctypes.c_int64, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P]),
     "commit_hip_last_error": (ctypes.c_char_p, []),
     "commit_hip_version": (ctypes.c_char_p, []),
 }

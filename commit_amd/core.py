@@ -30,7 +30,7 @@ from os.path import exists, isfile, join as pjoin
 
 import numpy as np
 
-from . import nifti, solvers, synth
+from . import _lib, nifti, solvers, synth
 from .operator import operator
 
 VERSION = "commit_amd 0.1"
@@ -177,6 +177,10 @@ class Evaluation:
         self.set_config("doNormalizeMaps", False)
         self.set_config("sortDictionaryWith", "mergesort")
         self.set_config("sortThreadsWith", "mergesort")
+        # vector storage of the device operator and FISTA: "fp64" (the
+        # reference's float64) or "fp32" (float32 vectors, FP64 accumulation,
+        # FP64 stopping norms and final residual); no counterpart in the reference
+        self.set_config("precision", "fp64")
 
     # ------------------------------------------------------------- config --
     def set_verbose(self, verbose):
@@ -367,7 +371,11 @@ class Evaluation:
         self.DICTIONARY["IC"]["eval"] = np.ones(int(self.DICTIONARY["IC"]["nSTR"] * self.KERNELS["wmr"].shape[0]),
                                                 dtype=np.uint32)
         nolut = getattr(self.model, "nolut", False) if self.model is not None else False
-        self.A = operator.LinearOperator(self.DICTIONARY, self.KERNELS, self.THREADS, nolut)
+        prec = str(self.get_config("precision") or "fp64").lower()
+        if prec not in ("fp64", "fp32"):
+            raise ValueError('CONFIG["precision"] must be "fp64" or "fp32"')
+        self.A = operator.LinearOperator(self.DICTIONARY, self.KERNELS, self.THREADS, nolut,
+                                         precision=_lib.FP32 if prec == "fp32" else _lib.FP64)
 
     def get_y(self):
         """Measured signal at the mask voxels, voxel-major (core.pyx:751-766)."""

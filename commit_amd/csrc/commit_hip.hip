@@ -115,9 +115,9 @@ __device__ __forceinline__ void fma_row(double (&acc)[NJ], double c, const float
 // Accumulate the entries [b, e) of one compartment into acc.
 //   xb: x (or x*eval) block of RF 0, xs: stride between RF blocks;
 //   lut: padded RF 0 table, lut_rf: stride between RF tables.
-template <int NJ>
+template <int NJ, typename T>
 __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32_t b, uint32_t e,
-                                       const double *__restrict__ xb, size_t xs, int nrf,
+                                       const T *__restrict__ xb, size_t xs, int nrf,
                                        const float *__restrict__ lut, size_t lut_rf, int nSp, int lane)
 {
     for (int i = 0; i < nrf; i++) {
@@ -128,7 +128,7 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
             uint32_t o = 0;
             if (k < e) {
                 if (cp.o) o = stream_load(cp.o + k);
-                c = xb[(size_t)stream_load(cp.col + k) + (size_t)i * xs];
+                c = (double)xb[(size_t)stream_load(cp.col + k) + (size_t)i * xs];
                 // IC coefficient l * x * eval                      (setup_operator.py:63-96)
                 if (cp.len) c *= (double)stream_load(cp.len + k);
             }
@@ -156,17 +156,20 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
 
 // Output stage of A: y = A x, or the FISTA residual r = sqrtw .* (A x - yobs)
 // with a per-voxel partial of ||r||^2 (reduced later in voxel order).
+template <typename T>
 struct AOut {
-    double *y;
-    const double *yobs;   // null: plain product
-    const double *sqrtw;  // null: unweighted
+    T *y;
+    const T *yobs;        // null: plain product
+    const T *sqrtw;       // null: unweighted
     double *rpart;        // [nVOX] partial sums of r^2, or null
 };
 
 // xic: the IC block of x, or x*eval when eval is not all ones.
-template <int NJ>
-__global__ __launch_bounds__(kBlock) void k_A_fp64(DevDict d, const double *__restrict__ x,
-                                                    const double *__restrict__ xic, AOut out)
+// T: storage type of x and y (double, or float for the FP32 mode); the
+// accumulation is FP64 in both (exact products of the FP32 LUT entries).
+template <int NJ, typename T>
+__global__ __launch_bounds__(kBlock) void k_A(DevDict d, const T *__restrict__ x, const T *__restrict__ xic,
+                                              AOut<T> out)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int v = voxel_id();
@@ -189,9 +192,9 @@ __global__ __launch_bounds__(kBlock) void k_A_fp64(DevDict d, const double *__re
         const int s = lane + kWave * j;
         if (s < d.nS) {
             double r = acc[j];
-            if (out.yobs) r -= stream_load(out.yobs + row + s);      // (solvers.py:308)
-            if (out.sqrtw) r *= stream_load(out.sqrtw + row + s);    // (solvers.py:306)
-            __builtin_nontemporal_store(r, out.y + row + s);
+            if (out.yobs) r -= (double)stream_load(out.yobs + row + s);      // (solvers.py:308)
+            if (out.sqrtw) r *= (double)stream_load(out.sqrtw + row + s);    // (solvers.py:306)
+            __builtin_nontemporal_store((T)r, out.y + row + s);
             sq = fma(r, r, sq);
         }
     }
@@ -203,11 +206,11 @@ __global__ __launch_bounds__(kBlock) void k_A_fp64(DevDict d, const double *__re
 }
 
 // x*eval for the IC block (only when eval is not all ones)
-__global__ void k_x_eval_fp64(const double *__restrict__ x, const uint32_t *__restrict__ ev,
-                              double *__restrict__ out, size_t n)
+template <typename T>
+__global__ void k_x_eval(const T *__restrict__ x, const uint32_t *__restrict__ ev, T *__restrict__ out, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = x[i] * (double)ev[i];
+        out[i] = ev[i] ? x[i] : (T)0;
 }
 
 // ------------------------------------------------------------ x = A^T y -----
@@ -228,8 +231,9 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (
 // Reduce ring entries [tb, tb+8) over the 64 lanes; store the ones < cnt:
 //   IC     : out[i*m + base + t] = l * dot   (eval is applied per fibre in pass 2)
 //   EC/ISO : out[col + i*xs]     = dot
+template <typename T>
 __device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int lane, double l,
-                                       uint32_t colk, bool ic, uint32_t m, double *__restrict__ out,
+                                       uint32_t colk, bool ic, uint32_t m, double *__restrict__ z, T *__restrict__ out,
                                        size_t xs, int i, uint32_t base)
 {
     const int te = tb + (lane >> 3);
@@ -249,17 +253,17 @@ __device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int 
     const uint32_t ce = (uint32_t)__shfl((int)colk, src);
     if ((lane & 7) == 0 && te < cnt) {
         if (ic)
-            out[(size_t)i * m + base + te] = le * q;            // (setup_operator.py:412-428)
+            z[(size_t)i * m + base + te] = le * q;              // (setup_operator.py:412-428)
         else
-            out[(size_t)ce + (size_t)i * xs] = q;               // (setup_operator.py:444-563)
+            out[(size_t)ce + (size_t)i * xs] = (T)q;            // (setup_operator.py:444-563)
     }
 }
 
 // One compartment of the voxel: per entry a partial dot product into the ring,
 // flushed every 8 entries.  Rows are pipelined as in a_comp.
-template <int NJ>
+template <int NJ, typename T>
 __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, uint32_t b, uint32_t e,
-                                        double *__restrict__ out, size_t xs, int nrf,
+                                        double *__restrict__ z, T *__restrict__ out, size_t xs, int nrf,
                                         const float *__restrict__ lut, size_t lut_rf, int nSp, int lane,
                                         double *ring)
 {
@@ -286,7 +290,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
         ring[(size_t)(t % kRing) * kRingStride + lane] = dot_row<NJ>(CUR, yv);                            \
         if ((t & 7) == 7) {                                                                               \
             __builtin_amdgcn_wave_barrier();                                                              \
-            flush8(ring, t - 7, cnt, lane, l, colk, ic, cp.m, out, xs, i, base);                          \
+            flush8(ring, t - 7, cnt, lane, l, colk, ic, cp.m, z, out, xs, i, base);                       \
             __builtin_amdgcn_wave_barrier();                                                              \
         }                                                                                                 \
         if (++t >= cnt) break;                                                                            \
@@ -299,17 +303,16 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
 #undef AT_STAGE
             if (cnt & 7) {                        // last partial group of 8
                 __builtin_amdgcn_wave_barrier();
-                flush8(ring, cnt & ~7, cnt, lane, l, colk, ic, cp.m, out, xs, i, base);
+                flush8(ring, cnt & ~7, cnt, lane, l, colk, ic, cp.m, z, out, xs, i, base);
                 __builtin_amdgcn_wave_barrier();
             }
         }
     }
 }
 
-template <int NJ>
-__global__ __launch_bounds__(kBlock) void k_At_fp64(DevDict d, const double *__restrict__ y,
-                                                    const double *__restrict__ sqrtw,
-                                                    double *__restrict__ z, double *__restrict__ x)
+template <int NJ, typename T>
+__global__ __launch_bounds__(kBlock) void k_At(DevDict d, const T *__restrict__ y, const T *__restrict__ sqrtw,
+                                               double *__restrict__ z, T *__restrict__ x)
 {
     __shared__ __align__(16) double rings[kWavesPerBlock][kRing * kRingStride];
     const int lane = threadIdx.x & (kWave - 1);
@@ -318,28 +321,28 @@ __global__ __launch_bounds__(kBlock) void k_At_fp64(DevDict d, const double *__r
     if (v >= d.nVOX) return;
     double *ring = rings[w];
     double yv[NJ];
-    const double *yr = y + (size_t)v * d.nS;
+    const T *yr = y + (size_t)v * d.nS;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int s = lane + kWave * j;
-        yv[j] = (s < d.nS) ? stream_load(yr + s) : 0.0;
-        if (sqrtw && s < d.nS) yv[j] *= stream_load(sqrtw + (size_t)v * d.nS + s);   // (solvers.py:267)
+        yv[j] = (s < d.nS) ? (double)stream_load(yr + s) : 0.0;
+        if (sqrtw && s < d.nS) yv[j] *= (double)stream_load(sqrtw + (size_t)v * d.nS + s);   // (solvers.py:267)
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    at_comp<NJ>(yv, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], z, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
+    at_comp<NJ>(yv, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], z, x, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
     if (d.nEC > 0)
-        at_comp<NJ>(yv, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
+        at_comp<NJ>(yv, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], z, x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
                     d.wmh, lut_rf, d.nSp, lane, ring);
     if (d.nISO > 0)
-        at_comp<NJ>(yv, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
+        at_comp<NJ>(yv, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], z, x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
                     (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane, ring);
 }
 
 // A^T y pass 2: x[f + i nF] = eval * sum of the fibre's products, ascending voxel
 // order.  eval is 0 or 1, so applying it once per fibre equals the reference's
 // per-segment product (setup_operator.py:428).
-__global__ __launch_bounds__(kBlock) void k_At_fibre_fp64(DevDict d, const double *__restrict__ z,
-                                                          double *__restrict__ x)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_At_fibre(DevDict d, const double *__restrict__ z, T *__restrict__ x)
 {
     const size_t total = (size_t)d.nF * d.nIC;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -358,14 +361,15 @@ __global__ __launch_bounds__(kBlock) void k_At_fibre_fp64(DevDict d, const doubl
             for (int u = 0; u < 8; u++) acc += v[u];
         }
         for (; k < ke; k++) acc += zi[d.fib_perm[k]];
-        x[g] = acc * (double)d.eval[g];
+        x[g] = (T)(acc * (double)d.eval[g]);
     }
 }
 
-__global__ void k_zero_fp64(double *p, size_t n)
+template <typename T>
+__global__ void k_zero(T *p, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0.0;
+        p[i] = (T)0;
 }
 
 
@@ -433,22 +437,24 @@ void release(commit_hip_plan *p)
 
 unsigned vox_blocks(const DevDict &d) { return (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock); }
 
-int launch_A(commit_hip_plan *p, const double *x, AOut out, hipStream_t st)
+template <typename T>
+int launch_A(commit_hip_plan *p, const T *x, AOut<T> out, hipStream_t st)
 {
     const DevDict &d = p->d;
     const unsigned grid = vox_blocks(d);
     if (grid == 0) return 0;
-    const double *xic = x;
+    const T *xic = x;
     if (!p->eval_ones && d.nF > 0 && d.nIC > 0) {
         const size_t n = (size_t)d.nF * d.nIC;
-        hipLaunchKernelGGL(k_x_eval_fp64, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
-                           x, d.eval, p->xic, n);
+        T *xe = reinterpret_cast<T *>(p->xic);
+        hipLaunchKernelGGL(k_x_eval<T>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
+                           x, d.eval, xe, n);
         HIP_TRY(hipGetLastError());
-        xic = p->xic;
+        xic = xe;
     }
     switch (d.nSp / kWave) {
 #define CASE(NJ) \
-    case NJ: hipLaunchKernelGGL(k_A_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, x, xic, out); break;
+    case NJ: hipLaunchKernelGGL((k_A<NJ, T>), dim3(grid), dim3(kBlock), 0, st, d, x, xic, out); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
         default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", d.nS, kMaxNJ * kWave);
@@ -457,14 +463,15 @@ int launch_A(commit_hip_plan *p, const double *x, AOut out, hipStream_t st)
     return 0;
 }
 
-int launch_At(commit_hip_plan *p, const double *y, const double *sqrtw, double *x, hipStream_t st, int pass = 3)
+template <typename T>
+int launch_At(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t st, int pass = 3)
 {
     const DevDict &d = p->d;
     const unsigned grid = vox_blocks(d);
     if (grid > 0 && (pass & 1)) {
         switch (d.nSp / kWave) {
 #define CASE(NJ) \
-    case NJ: hipLaunchKernelGGL(k_At_fp64<NJ>, dim3(grid), dim3(kBlock), 0, st, d, y, sqrtw, p->zbuf, x); break;
+    case NJ: hipLaunchKernelGGL((k_At<NJ, T>), dim3(grid), dim3(kBlock), 0, st, d, y, sqrtw, p->zbuf, x); break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
             default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds %d", d.nS, kMaxNJ * kWave);
@@ -474,7 +481,7 @@ int launch_At(commit_hip_plan *p, const double *y, const double *sqrtw, double *
     const size_t total = (size_t)d.nF * d.nIC;
     if (total > 0 && (pass & 2)) {
         const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
-        hipLaunchKernelGGL(k_At_fibre_fp64, dim3(g2), dim3(kBlock), 0, st, d, p->zbuf, x);
+        hipLaunchKernelGGL(k_At_fibre<T>, dim3(g2), dim3(kBlock), 0, st, d, p->zbuf, x);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -526,8 +533,8 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
 {
     if (!plan) return fail(COMMIT_HIP_EINVAL, "plan pointer is NULL");
     *plan = nullptr;
-    if (precision != COMMIT_HIP_FP64)
-        return fail(COMMIT_HIP_EUNSUPPORTED, "precision %d is not built in this library", precision);
+    if (precision != COMMIT_HIP_FP64 && precision != COMMIT_HIP_FP32)
+        return fail(COMMIT_HIP_EINVAL, "unknown precision %d", precision);
     if (nS < 1 || ndirs < 1 || nF < 0 || n < 0 || nE < 0 || nV < 0 || nVOX < 0)
         return fail(COMMIT_HIP_EINVAL, "negative or zero size (nS=%d ndirs=%d nF=%d n=%d nE=%d nV=%d nVOX=%d)",
                     nS, ndirs, nF, n, nE, nV, nVOX);
@@ -675,22 +682,68 @@ int commit_hip_plan_shape(const commit_hip_plan *plan, int64_t *nrows, int64_t *
 
 int64_t commit_hip_plan_device_bytes(const commit_hip_plan *plan) { return plan ? plan->bytes : -1; }
 
-int commit_hip_A(commit_hip_plan *plan, const void *x, void *y, void *stream)
+int commit_hip_plan_set_precision(commit_hip_plan *plan, int precision)
 {
-    if (!plan || !x || !y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
-    HIP_TRY(hipSetDevice(plan->device));
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    double *yd = static_cast<double *>(y);
-    int rc = launch_A(plan, static_cast<const double *>(x), AOut{yd, nullptr, nullptr, nullptr}, st);
+    if (!plan) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (precision != COMMIT_HIP_FP64 && precision != COMMIT_HIP_FP32)
+        return fail(COMMIT_HIP_EINVAL, "unknown precision %d", precision);
+    plan->precision = precision;
+    return 0;
+}
+
+int commit_hip_plan_precision(const commit_hip_plan *plan) { return plan ? plan->precision : -1; }
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int do_A(commit_hip_plan *plan, const void *x, void *y, hipStream_t st)
+{
+    T *yd = static_cast<T *>(y);
+    int rc = launch_A<T>(plan, static_cast<const T *>(x), AOut<T>{yd, nullptr, nullptr, nullptr}, st);
     if (rc) return rc;
     if (plan->nolut && plan->nS_ref > 1) {
         // LUT-free rows are indexed by voxel only (setup_operator.py:726-729);
         // the remaining rows of the reference-sized output stay zero.
         const size_t tail = (size_t)plan->d.nVOX * (plan->nS_ref - 1);
-        hipLaunchKernelGGL(k_zero_fp64, dim3(1024), dim3(256), 0, st, yd + plan->d.nVOX, tail);
+        hipLaunchKernelGGL(k_zero<T>, dim3(1024), dim3(256), 0, st, yd + plan->d.nVOX, tail);
         HIP_TRY(hipGetLastError());
     }
     return 0;
+}
+
+template <typename T>
+int do_At(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, hipStream_t st, int pass)
+{
+    return launch_At<T>(plan, static_cast<const T *>(y), static_cast<const T *>(sqrtw), static_cast<T *>(x), st,
+                        pass);
+}
+
+template <typename T>
+int do_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw, void *res,
+                  double *rsq, hipStream_t st)
+{
+    const size_t nv = (size_t)plan->d.nVOX;
+    AOut<T> out{static_cast<T *>(res), static_cast<const T *>(yobs), static_cast<const T *>(sqrtw), plan->rpart};
+    int rc = launch_A<T>(plan, static_cast<const T *>(x), out, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum_stage1, dim3(kSumBlocks), dim3(kBlock), 0, st, plan->rpart, nv, plan->rpart + nv);
+    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(kBlock), 0, st, plan->rpart + nv, kSumBlocks, rsq);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int commit_hip_A(commit_hip_plan *plan, const void *x, void *y, void *stream)
+{
+    if (!plan || !x || !y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return plan->precision == COMMIT_HIP_FP32 ? do_A<float>(plan, x, y, st) : do_A<double>(plan, x, y, st);
 }
 
 int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream)
@@ -700,12 +753,7 @@ int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream)
 
 int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, void *stream)
 {
-    if (!plan || !x || !y) return fail(COMMIT_HIP_EINVAL, "NULL argument");
-    if (plan->nolut && plan->nS_ref > 1 && sqrtw)
-        return fail(COMMIT_HIP_EUNSUPPORTED, "row weights on a LUT-free operator with nS > 1");
-    HIP_TRY(hipSetDevice(plan->device));
-    return launch_At(plan, static_cast<const double *>(y), static_cast<const double *>(sqrtw),
-                     static_cast<double *>(x), static_cast<hipStream_t>(stream));
+    return commit_hip_At_pass(plan, y, sqrtw, x, 3, stream);
 }
 
 int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, int pass, void *stream)
@@ -715,8 +763,9 @@ int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, 
     if (plan->nolut && plan->nS_ref > 1 && sqrtw)
         return fail(COMMIT_HIP_EUNSUPPORTED, "row weights on a LUT-free operator with nS > 1");
     HIP_TRY(hipSetDevice(plan->device));
-    return launch_At(plan, static_cast<const double *>(y), static_cast<const double *>(sqrtw),
-                     static_cast<double *>(x), static_cast<hipStream_t>(stream), pass);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return plan->precision == COMMIT_HIP_FP32 ? do_At<float>(plan, y, sqrtw, x, st, pass)
+                                              : do_At<double>(plan, y, sqrtw, x, st, pass);
 }
 
 int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,
@@ -727,15 +776,8 @@ int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs
         return fail(COMMIT_HIP_EUNSUPPORTED, "residual of a LUT-free operator with nS > 1");
     HIP_TRY(hipSetDevice(plan->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t nv = (size_t)plan->d.nVOX;
-    AOut out{static_cast<double *>(res), static_cast<const double *>(yobs), static_cast<const double *>(sqrtw),
-             plan->rpart};
-    int rc = launch_A(plan, static_cast<const double *>(x), out, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_sum_stage1, dim3(kSumBlocks), dim3(kBlock), 0, st, plan->rpart, nv, plan->rpart + nv);
-    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(kBlock), 0, st, plan->rpart + nv, kSumBlocks, rsq);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    return plan->precision == COMMIT_HIP_FP32 ? do_A_residual<float>(plan, x, yobs, sqrtw, res, rsq, st)
+                                              : do_A_residual<double>(plan, x, yobs, sqrtw, res, rsq, st);
 }
 
 }  // extern "C"

@@ -30,12 +30,15 @@ __device__ __forceinline__ double soft(double v, double t)
 // x = xin - step*grad, then soft-thresholding and (for compartments without a
 // group penalty) non-negativity, element by element.
 // xin may alias x (in-place prox): each element is read and written by one thread.
-__global__ __launch_bounds__(kBlock) void k_prox_elem(ProxArgs P, int64_t n, const double *xin,
-                                                      const double *__restrict__ grad, double step, double *x)
+// T: storage type of the vectors (double, or float for the FP32 mode); the
+// arithmetic is FP64 and the result is rounded once on store.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prox_elem(ProxArgs P, int64_t n, const T *xin,
+                                                      const T *__restrict__ grad, double step, T *x)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double v = xin[i];
-        if (grad) v = v - step * grad[i];                                 // (solvers.py:300)
+        double v = (double)xin[i];
+        if (grad) v = v - step * (double)grad[i];                         // (solvers.py:300)
         for (int c = 0; c < 3; c++) {
             if (i < P.start[c] || i >= P.end[c]) continue;
             const int r = P.reg[c];
@@ -45,16 +48,17 @@ __global__ __launch_bounds__(kBlock) void k_prox_elem(ProxArgs P, int64_t n, con
                 v = (v <= 0.0) ? 0.0 : v;                                  // (proximals.pyx:19-21)
             break;
         }
-        x[i] = v;
+        x[i] = (T)v;
     }
 }
 
 // Group shrinkage (proximals.pyx:103-134), one wavefront per group: for groups
 // that do not overlap the order does not matter.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_group_prox_par(int64_t ngroups, const int32_t *__restrict__ gptr,
                                                            const int32_t *__restrict__ gidx,
                                                            const double *__restrict__ gw, double lam,
-                                                           double *__restrict__ x)
+                                                           T *__restrict__ x)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t g = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)));
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_group_prox_par(int64_t ngroups, cons
     const int b = gptr[g], e = gptr[g + 1];
     double s = 0.0;
     for (int k = b + lane; k < e; k += kWave) {
-        const double v = x[gidx[k]];
+        const double v = (double)x[gidx[k]];
         s = fma(v, v, s);
     }
 #pragma unroll
@@ -72,15 +76,16 @@ __global__ __launch_bounds__(kBlock) void k_group_prox_par(int64_t ngroups, cons
     const double f = (nrm <= wl) ? 0.0 : (nrm - wl) / nrm;
     for (int k = b + lane; k < e; k += kWave) {
         const int32_t q = gidx[k];
-        x[q] = (f == 0.0) ? 0.0 : x[q] * f;
+        x[q] = (f == 0.0) ? (T)0 : (T)((double)x[q] * f);
     }
 }
 
 // Overlapping (hierarchical) groups: one block applies the groups in order.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_group_prox_seq(int64_t ngroups, const int32_t *__restrict__ gptr,
                                                            const int32_t *__restrict__ gidx,
                                                            const double *__restrict__ gw, double lam,
-                                                           double *__restrict__ x)
+                                                           T *__restrict__ x)
 {
     __shared__ double sh[kBlock / kWave];
     __shared__ double fac;
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_group_prox_seq(int64_t ngroups, cons
         const int b = gptr[g], e = gptr[g + 1];
         double s = 0.0;
         for (int k = b + (int)threadIdx.x; k < e; k += kBlock) {
-            const double v = x[gidx[k]];
+            const double v = (double)x[gidx[k]];
             s = fma(v, v, s);
         }
         s = block_sum_256(s, sh);
@@ -101,40 +106,44 @@ __global__ __launch_bounds__(kBlock) void k_group_prox_seq(int64_t ngroups, cons
         const double f = fac;
         for (int k = b + (int)threadIdx.x; k < e; k += kBlock) {
             const int32_t q = gidx[k];
-            x[q] = (f == 0.0) ? 0.0 : x[q] * f;
+            x[q] = (f == 0.0) ? (T)0 : (T)((double)x[q] * f);
         }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_nn_range(double *__restrict__ x, int64_t b, int64_t e)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_nn_range(T *__restrict__ x, int64_t b, int64_t e)
 {
     for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] = (x[i] <= 0.0) ? 0.0 : x[i];
+        x[i] = (x[i] <= (T)0) ? (T)0 : x[i];
 }
 
+template <typename T>
 struct SumArgs {
-    const double *x, *xhat, *grad, *prev, *w;
+    const T *x, *xhat, *grad, *prev;
+    const double *w;
     int64_t l1b[3], l1e[3];
     int l1w[3];
 };
 
 // Stage 1 of the seven elementwise sums; part[t * kSumBlocks + block].
-__global__ __launch_bounds__(kBlock) void k_sums_stage1(SumArgs a, int64_t n, double *__restrict__ part)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sums_stage1(SumArgs<T> a, int64_t n, double *__restrict__ part)
 {
     __shared__ double sh[kBlock / kWave];
     const int64_t chunk = (n + kSumBlocks - 1) / kSumBlocks;
     const int64_t b = (int64_t)blockIdx.x * chunk, e = b + chunk < n ? b + chunk : n;
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int64_t i = b + threadIdx.x; i < e; i += kBlock) {
-        const double xi = a.x[i];
+        const double xi = (double)a.x[i];
         if (a.xhat) {
-            const double d = xi - a.xhat[i];
-            s[0] = fma(d, a.grad[i], s[0]);
+            const double d = xi - (double)a.xhat[i];
+            s[0] = fma(d, (double)a.grad[i], s[0]);
             s[1] = fma(d, d, s[1]);
         }
         if (a.prev) {
-            const double d = xi - a.prev[i];
+            const double d = xi - (double)a.prev[i];
             s[2] = fma(d, d, s[2]);
         }
         s[3] = fma(xi, xi, s[3]);
@@ -150,9 +159,10 @@ __global__ __launch_bounds__(kBlock) void k_sums_stage1(SumArgs a, int64_t n, do
 }
 
 // gn[g] = group_w[g] * ||x_g||  (proximals.pyx:80-99), one wavefront per group
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_group_wnorm(int64_t ngroups, const int32_t *__restrict__ gptr,
                                                         const int32_t *__restrict__ gidx,
-                                                        const double *__restrict__ gw, const double *__restrict__ x,
+                                                        const double *__restrict__ gw, const T *__restrict__ x,
                                                         double *__restrict__ gn)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -160,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_group_wnorm(int64_t ngroups, const i
     if (g >= ngroups) return;
     double s = 0.0;
     for (int k = gptr[g] + lane; k < gptr[g + 1]; k += kWave) {
-        const double v = x[gidx[k]];
+        const double v = (double)x[gidx[k]];
         s = fma(v, v, s);
     }
 #pragma unroll
@@ -176,11 +186,12 @@ __global__ __launch_bounds__(kBlock) void k_sums_stage2(const double *__restrict
     if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a, const double *__restrict__ x, double b,
-                                                  const double *__restrict__ y, double *__restrict__ out)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a, const T *__restrict__ x, double b,
+                                                  const T *__restrict__ y, T *__restrict__ out)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = a * x[i] + b * y[i];
+        out[i] = (T)(a * (double)x[i] + b * (double)y[i]);
 }
 
 unsigned elem_grid(int64_t n)
@@ -199,10 +210,11 @@ unsigned wave_grid(int64_t nw)
 
 static_assert(kSumBlocks == kBlock, "stage 2 reduces one partial per thread");
 
-extern "C" {
+namespace {
 
-int commit_hip_prox(const commit_hip_prox_params *p, int64_t n, const double *xin, const double *grad,
-                    double step, double scale, double *x, void *stream)
+template <typename T>
+int prox_impl(const commit_hip_prox_params *p, int64_t n, const T *xin, const T *grad, double step, double scale,
+              T *x, void *stream)
 {
     if (!p || !xin || !x || n < 0) return fail(COMMIT_HIP_EINVAL, "NULL argument or negative size");
     if (n == 0) return 0;
@@ -223,7 +235,7 @@ int commit_hip_prox(const commit_hip_prox_params *p, int64_t n, const double *xi
         if (a.weighted[c] && !p->w) return fail(COMMIT_HIP_EINVAL, "weighted lasso without weights");
     }
     a.w = p->w;
-    hipLaunchKernelGGL(k_prox_elem, dim3(elem_grid(n)), dim3(kBlock), 0, st, a, n, xin, grad, step, x);
+    hipLaunchKernelGGL(k_prox_elem<T>, dim3(elem_grid(n)), dim3(kBlock), 0, st, a, n, xin, grad, step, x);
     HIP_TRY(hipGetLastError());
     const int r = p->reg[0];
     if (r == COMMIT_HIP_REG_GROUP_LASSO || r == COMMIT_HIP_REG_SPARSE_GROUP_LASSO) {
@@ -231,31 +243,30 @@ int commit_hip_prox(const commit_hip_prox_params *p, int64_t n, const double *xi
         if (lam != 0.0 && p->ngroups > 0) {          // proximals.pyx:115
             if (!p->group_ptr || !p->group_idx || !p->group_w) return fail(COMMIT_HIP_EINVAL, "group arrays missing");
             if (p->groups_overlap)
-                hipLaunchKernelGGL(k_group_prox_seq, dim3(1), dim3(kBlock), 0, st, p->ngroups, p->group_ptr,
+                hipLaunchKernelGGL(k_group_prox_seq<T>, dim3(1), dim3(kBlock), 0, st, p->ngroups, p->group_ptr,
                                    p->group_idx, p->group_w, lam, x);
             else
-                hipLaunchKernelGGL(k_group_prox_par, dim3(wave_grid(p->ngroups)), dim3(kBlock), 0, st, p->ngroups,
+                hipLaunchKernelGGL(k_group_prox_par<T>, dim3(wave_grid(p->ngroups)), dim3(kBlock), 0, st, p->ngroups,
                                    p->group_ptr, p->group_idx, p->group_w, lam, x);
             HIP_TRY(hipGetLastError());
         }
         if (p->nn[0] && p->size[0] > 0) {
-            hipLaunchKernelGGL(k_nn_range, dim3(elem_grid(p->size[0])), dim3(kBlock), 0, st, x, a.start[0], a.end[0]);
+            hipLaunchKernelGGL(k_nn_range<T>, dim3(elem_grid(p->size[0])), dim3(kBlock), 0, st, x, a.start[0], a.end[0]);
             HIP_TRY(hipGetLastError());
         }
     }
     return 0;
 }
 
-int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const double *grad,
-                          const double *prev, const int64_t *l1_range, const int *l1_weighted,
-                          const double *w, int64_t ngroups, const int32_t *group_ptr,
-                          const int32_t *group_idx, const double *group_w, double *out, double *work,
-                          void *stream)
+template <typename T>
+int sums_impl(int64_t n, const T *x, const T *xhat, const T *grad, const T *prev, const int64_t *l1_range,
+              const int *l1_weighted, const double *w, int64_t ngroups, const int32_t *group_ptr,
+              const int32_t *group_idx, const double *group_w, double *out, double *work, void *stream)
 {
     if (!x || !out || !work || n < 0 || ngroups < 0) return fail(COMMIT_HIP_EINVAL, "NULL argument");
     if ((xhat == nullptr) != (grad == nullptr)) return fail(COMMIT_HIP_EINVAL, "xhat and grad go together");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    SumArgs a{};
+    SumArgs<T> a{};
     a.x = x;
     a.xhat = xhat;
     a.grad = grad;
@@ -269,11 +280,11 @@ int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const 
     }
     double *part = work;
     double *gn = work + 8 * kSumBlocks;
-    hipLaunchKernelGGL(k_sums_stage1, dim3(kSumBlocks), dim3(kBlock), 0, st, a, n, part);
+    hipLaunchKernelGGL(k_sums_stage1<T>, dim3(kSumBlocks), dim3(kBlock), 0, st, a, n, part);
     HIP_TRY(hipGetLastError());
     if (ngroups > 0) {
         if (!group_ptr || !group_idx || !group_w) return fail(COMMIT_HIP_EINVAL, "group arrays missing");
-        hipLaunchKernelGGL(k_group_wnorm, dim3(wave_grid(ngroups)), dim3(kBlock), 0, st, ngroups, group_ptr,
+        hipLaunchKernelGGL(k_group_wnorm<T>, dim3(wave_grid(ngroups)), dim3(kBlock), 0, st, ngroups, group_ptr,
                            group_idx, group_w, x, gn);
         hipLaunchKernelGGL(k_sum_stage1, dim3(kSumBlocks), dim3(kBlock), 0, st, gn, (size_t)ngroups,
                            part + 7 * kSumBlocks);
@@ -286,14 +297,61 @@ int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const 
     return 0;
 }
 
-int commit_hip_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out, void *stream)
+template <typename T>
+int axpby_impl(int64_t n, double a, const T *x, double b, const T *y, T *out, void *stream)
 {
     if (!x || !y || !out || n < 0) return fail(COMMIT_HIP_EINVAL, "NULL argument");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_axpby, dim3(elem_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, a, x, b,
+    hipLaunchKernelGGL(k_axpby<T>, dim3(elem_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, a, x, b,
                        y, out);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int commit_hip_prox(const commit_hip_prox_params *p, int64_t n, const double *xin, const double *grad,
+                    double step, double scale, double *x, void *stream)
+{
+    return prox_impl<double>(p, n, xin, grad, step, scale, x, stream);
+}
+
+int commit_hip_prox_f32(const commit_hip_prox_params *p, int64_t n, const float *xin, const float *grad,
+                        double step, double scale, float *x, void *stream)
+{
+    return prox_impl<float>(p, n, xin, grad, step, scale, x, stream);
+}
+
+int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const double *grad,
+                          const double *prev, const int64_t *l1_range, const int *l1_weighted,
+                          const double *w, int64_t ngroups, const int32_t *group_ptr,
+                          const int32_t *group_idx, const double *group_w, double *out, double *work,
+                          void *stream)
+{
+    return sums_impl<double>(n, x, xhat, grad, prev, l1_range, l1_weighted, w, ngroups, group_ptr, group_idx,
+                             group_w, out, work, stream);
+}
+
+int commit_hip_fista_sums_f32(int64_t n, const float *x, const float *xhat, const float *grad,
+                              const float *prev, const int64_t *l1_range, const int *l1_weighted,
+                              const double *w, int64_t ngroups, const int32_t *group_ptr,
+                              const int32_t *group_idx, const double *group_w, double *out, double *work,
+                              void *stream)
+{
+    return sums_impl<float>(n, x, xhat, grad, prev, l1_range, l1_weighted, w, ngroups, group_ptr, group_idx,
+                            group_w, out, work, stream);
+}
+
+int commit_hip_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out, void *stream)
+{
+    return axpby_impl<double>(n, a, x, b, y, out, stream);
+}
+
+int commit_hip_axpby_f32(int64_t n, double a, const float *x, double b, const float *y, float *out, void *stream)
+{
+    return axpby_impl<float>(n, a, x, b, y, out, stream);
 }
 
 }  // extern "C"

@@ -72,6 +72,11 @@ class _Plan:
         self.device_bytes = int(lib.commit_hip_plan_device_bytes(handle))
         self._lib = lib
 
+    def set_precision(self, precision):
+        _lib.check(self._lib.commit_hip_plan_set_precision(self.handle, int(precision)), "commit_hip_plan_set_precision")
+        self.precision = int(precision)
+        self.dtype = torch.float64 if self.precision == _lib.FP64 else torch.float32
+
     def set_eval(self, ICeval):
         ev = np.ascontiguousarray(ICeval, dtype=np.uint32)
         if ev.size != self.nF * self.nIC:
@@ -141,6 +146,16 @@ class LinearOperator:
     @property
     def device(self):
         return torch.device("cuda", self._plan.device)
+
+    @property
+    def precision(self):
+        """Storage precision of the vectors (commit_amd._lib.FP64 or FP32)."""
+        return self._plan.precision
+
+    def set_precision(self, precision):
+        """Switch the vector storage type of the shared device plan (FP64: the
+        reference's float64 vectors; FP32: float32 vectors, FP64 accumulation)."""
+        self._plan.set_precision(precision)
 
     def set_eval(self, ICeval):
         """Update DICTIONARY['IC']['eval'] on the device (debias, core.pyx:1481-1487)."""

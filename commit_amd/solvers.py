@@ -127,11 +127,13 @@ class DeviceRegularisation:
         self.out = torch.empty(8, dtype=torch.float64, device=device)
 
     def sums(self, x, xhat=None, grad=None, prev=None):
-        """The eight deterministic sums of commit_hip_fista_sums, on the host."""
+        """The eight deterministic sums of commit_hip_fista_sums (FP64
+        accumulation for float64 and float32 vectors alike), on the host."""
         lib = _lib.load()
         P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
         p = self.params
-        rc = lib.commit_hip_fista_sums(self.n, P(x), P(xhat), P(grad), P(prev), self.l1_range, self.l1_weighted,
+        fn = lib.commit_hip_fista_sums_f32 if x.dtype == torch.float32 else lib.commit_hip_fista_sums
+        rc = fn(self.n, P(x), P(xhat), P(grad), P(prev), self.l1_range, self.l1_weighted,
                                        ctypes.c_void_p(self.w.data_ptr()), self.ngroups,
                                        ctypes.c_void_p(p.group_ptr or 0), ctypes.c_void_p(p.group_idx or 0),
                                        ctypes.c_void_p(p.group_w or 0), P(self.out), P(self.work), _stream(x))
@@ -155,7 +157,8 @@ class DeviceRegularisation:
         lib = _lib.load()
         P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
         src = x if xin is None else xin
-        rc = lib.commit_hip_prox(ctypes.byref(self.params), self.n, P(src), P(grad), float(step), float(scale),
+        fn = lib.commit_hip_prox_f32 if x.dtype == torch.float32 else lib.commit_hip_prox
+        rc = fn(ctypes.byref(self.params), self.n, P(src), P(grad), float(step), float(scale),
                                  P(x), _stream(x))
         _lib.check(rc, "commit_hip_prox")
         return x
@@ -216,8 +219,15 @@ def evaluate_model(y, A, x, regularisation=None):
 
 
 def solve(y, A, At, tol_fun=1e-4, tol_x=1e-6, max_iter=1000, verbose=True, x0=None, regularisation=None,
-          confidence_array=None):
-    """solvers.solve (solvers.py:218-241)."""
+          confidence_array=None, lipschitz_iters=0):
+    """solvers.solve (solvers.py:218-241).
+
+    The vectors are stored in the operator's precision (A.precision): float64
+    reproduces the reference; float32 keeps x, y and the FISTA vectors in
+    float32 while the stopping norms, the objective, the Lipschitz estimate
+    and the final residual stay FP64.  lipschitz_iters > 0 replaces the
+    reference's one-step estimate (solvers.py:282-286) by that many FP64
+    power iterations on A^T W A."""
     if regularisation is None:
         omega = lambda x: 0.0
         prox = lambda x, _: x
@@ -228,11 +238,12 @@ def solve(y, A, At, tol_fun=1e-4, tol_x=1e-6, max_iter=1000, verbose=True, x0=No
     if confidence_array is not None:
         confidence_array = np.sqrt(confidence_array)
     return fista(y, A, At, omega, prox, confidence_array, tol_fun, tol_x, max_iter, verbose, x0,
-                 _dev=None if regularisation is None else regularisation.get("_device"))
+                 _dev=None if regularisation is None else regularisation.get("_device"),
+                 lipschitz_iters=lipschitz_iters)
 
 
 def fista(y, A, At, omega, prox, sqrt_W=None, tol_fun=1e-4, tol_x=1e-6, max_iter=1000, verbose=False, x0=None,
-          _dev=None):
+          _dev=None, lipschitz_iters=0):
     """FISTA with backtracking (solvers.py:244-391; Beck & Teboulle).
 
     Returns (x, opt_details) with x a numpy float64 vector."""
@@ -244,22 +255,39 @@ def fista(y, A, At, omega, prox, sqrt_W=None, tol_fun=1e-4, tol_x=1e-6, max_iter
         rp = {"startIC": 0, "sizeIC": n, "startEC": n, "sizeEC": 0, "startISO": n, "sizeISO": 0,
               "regIC": None, "regEC": None, "regISO": None, "nnIC": False, "nnEC": False, "nnISO": False}
         _dev = DeviceRegularisation(rp, n, device)
-    return _fista_device(y, A, _dev, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0)
+    return _fista_device(y, A, _dev, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lipschitz_iters)
 
 
-def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0):
+class _Precision:
+    """Temporarily switch the shared device plan to another vector precision."""
+
+    def __init__(self, plan, precision):
+        self.plan, self.precision = plan, precision
+
+    def __enter__(self):
+        self.saved = self.plan.precision
+        self.plan.set_precision(self.precision)
+
+    def __exit__(self, *exc):
+        self.plan.set_precision(self.saved)
+
+
+def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lipschitz_iters=0):
     lib = _lib.load()
     plan = A._plan.handle
     device = A.device
     m, n = A.nROWS, A.nCOLS
-    f64 = dict(dtype=torch.float64, device=device)
+    dt = A._plan.dtype
+    fp32 = dt == torch.float32
+    vec = dict(dtype=dt, device=device)
     P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    axpby = lib.commit_hip_axpby_f32 if fp32 else lib.commit_hip_axpby
 
-    def dev_vec(v, size):
+    def dev_vec(v, size, dtype=dt):
         if isinstance(v, torch.Tensor):
-            t = v.to(device=device, dtype=torch.float64).contiguous()
+            t = v.to(device=device, dtype=dtype).contiguous()
         else:
-            t = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(device)
+            t = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(device=device, dtype=dtype)
         if t.numel() != size:
             raise RuntimeError("A.dot(): dimensions do not match")
         return t
@@ -267,16 +295,23 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0):
     yd = dev_vec(y, m)
     sw = None if sqrt_W is None else dev_vec(sqrt_W, m)
     xhat = dev_vec(np.zeros(n) if x0 is None else x0, n).clone()
-    res = torch.empty(m, **f64)
-    grad = torch.empty(n, **f64)
-    x = torch.empty(n, **f64)
-    tmp = torch.empty(n, **f64)
-    rsq = torch.empty(1, **f64)
+    res = torch.empty(m, **vec)
+    grad = torch.empty(n, **vec)
+    x = torch.empty(n, **vec)
+    tmp = torch.empty(n, **vec)
+    rsq = torch.empty(1, dtype=torch.float64, device=device)
     st = _stream(yd)
 
     def residual(v, with_y=True):
         _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yd) if with_y else None, P(sw), P(res), P(rsq), st),
                    "commit_hip_A_residual")
+        return float(rsq.item())
+
+    def residual64(v, yv, swv, out):
+        """||sqrtw (A v - y)||^2 with FP64 vectors on the same plan."""
+        with _Precision(A._plan, _lib.FP64):
+            _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yv), P(swv), P(out), P(rsq), st),
+                       "commit_hip_A_residual")
         return float(rsq.item())
 
     def gradient():
@@ -292,10 +327,20 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0):
     beta = 0.9
     prev_x = xhat.clone()
     qfval = prev_obj
-    a_grad = residual(grad, with_y=False)
-    g_sq = reg.sums(grad)[3]
-    with np.errstate(divide="ignore", invalid="ignore"):   # numpy semantics of solvers.py:283-286
-        L = (np.sqrt(np.float64(a_grad)) / np.sqrt(np.float64(g_sq))) ** 2
+    if lipschitz_iters > 0:
+        L = _lipschitz_power(A, lib, plan, sw, grad, int(lipschitz_iters), st)
+    else:
+        if fp32:                       # the estimate is taken in FP64
+            g64 = grad.double()
+            sw64 = None if sw is None else sw.double()
+            a_grad = residual64(g64, None, sw64, torch.empty(m, dtype=torch.float64, device=device))
+            g_sq = float(torch.dot(g64, g64).item())
+        else:
+            a_grad = residual(grad, with_y=False)
+            g_sq = reg.sums(grad)[3]
+        with np.errstate(divide="ignore", invalid="ignore"):   # numpy semantics of solvers.py:283-286
+            L = (np.sqrt(np.float64(a_grad)) / np.sqrt(np.float64(g_sq))) ** 2
+    with np.errstate(divide="ignore", invalid="ignore"):
         step_size = np.float64(1.9) / L
     if verbose == 4:
         print()
@@ -344,8 +389,8 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0):
             break
         # FISTA update (solvers.py:359-375): xhat = x + (told-1)/t * (x - prev_x)
         t = 0.5 * (1 + sqrt(1 + 4 * told ** 2))
-        _lib.check(lib.commit_hip_axpby(n, 1.0, P(x), -1.0, P(prev_x), P(tmp), st), "commit_hip_axpby")
-        _lib.check(lib.commit_hip_axpby(n, 1.0, P(x), (told - 1) / t, P(tmp), P(xhat), st), "commit_hip_axpby")
+        _lib.check(axpby(n, 1.0, P(x), -1.0, P(prev_x), P(tmp), st), "commit_hip_axpby")
+        _lib.check(axpby(n, 1.0, P(x), (told - 1) / t, P(tmp), P(xhat), st), "commit_hip_axpby")
         qf = residual(xhat)
         gradient()
         it += 1
@@ -358,4 +403,41 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0):
     details = {"residual": 0.5 * res_norm ** 2, "regterm": reg_term_x, "cost_function": curr_obj,
                "abs_cost": abs_obj, "rel_cost": rel_obj, "abs_x": abs_x, "rel_x": rel_x,
                "iterations": it, "stopping_criterion": criterion}
-    return x.cpu().numpy(), details
+    if fp32:
+        # final residual reported in FP64 (x as stored, y and weights in float64)
+        x64 = x.double()
+        y64 = dev_vec(y, m, torch.float64)
+        sw64 = None if sqrt_W is None else dev_vec(sqrt_W, m, torch.float64)
+        r64 = torch.empty(m, dtype=torch.float64, device=device)
+        rn64 = sqrt(residual64(x64, y64, sw64, r64))
+        details["residual"] = 0.5 * rn64 ** 2
+        details["cost_function"] = details["residual"] + reg_term_x
+        details["precision"] = "fp32"
+    return x.double().cpu().numpy(), details
+
+
+def _lipschitz_power(A, lib, plan, sw, g, iters, st):
+    """Largest eigenvalue of A^T W A by `iters` FP64 power iterations started
+    from g (the first gradient), with the operator in FP64."""
+    device = A.device
+    m, n = A.nROWS, A.nCOLS
+    P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    v = g.double().clone()
+    nv = float(torch.linalg.vector_norm(v).item())
+    if not nv > 0:
+        return np.float64(np.nan)
+    v /= nv
+    sw64 = None if sw is None else sw.double()
+    r = torch.empty(m, dtype=torch.float64, device=device)
+    rsq = torch.empty(1, dtype=torch.float64, device=device)
+    w = torch.empty(n, dtype=torch.float64, device=device)
+    lam = 0.0
+    with _Precision(A._plan, _lib.FP64):
+        for _ in range(iters):
+            _lib.check(lib.commit_hip_A_residual(plan, P(v), None, P(sw64), P(r), P(rsq), st), "commit_hip_A_residual")
+            _lib.check(lib.commit_hip_At_weighted(plan, P(r), P(sw64), P(w), st), "commit_hip_At_weighted")
+            lam = float(torch.linalg.vector_norm(w).item())
+            if not lam > 0:
+                break
+            v = w / lam
+    return np.float64(lam)

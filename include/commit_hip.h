@@ -32,7 +32,8 @@ extern "C" {
 
 /* precision of x, y and the accumulation */
 #define COMMIT_HIP_FP64 0         /* drop-in: x, y float64; FP64 accumulation */
-#define COMMIT_HIP_FP32 1         /* x, y float32; compensated FP32 accumulation */
+#define COMMIT_HIP_FP32 1         /* x, y (and FISTA vectors) float32; products of the float32 LUT,
+                                     lengths and vectors accumulated in FP64, rounded once on store */
 
 typedef struct commit_hip_plan commit_hip_plan;
 
@@ -74,6 +75,16 @@ int commit_hip_plan_shape(const commit_hip_plan *plan, int64_t *nrows, int64_t *
 int64_t commit_hip_plan_device_bytes(const commit_hip_plan *plan);
 
 /*
+ * Storage type of the vectors the single-vector entry points (commit_hip_A,
+ * _At, _At_weighted, _At_pass, _A_residual) read and write: COMMIT_HIP_FP64
+ * or COMMIT_HIP_FP32.  The dictionary is shared; switching is free, so a
+ * mixed-precision fit can take its Lipschitz estimate and final residual in
+ * FP64 on the same plan.  No counterpart in the reference (FP64 only).
+ */
+int commit_hip_plan_set_precision(commit_hip_plan *plan, int precision);
+int commit_hip_plan_precision(const commit_hip_plan *plan);
+
+/*
  * y = A x.  Replaces COMMIT_A (setup_operator.py:314-365) and, for a LUT-free
  * plan, COMMIT_A_nolut (setup_operator.py:779-813).  x: cols, y: rows, device
  * pointers of the plan's precision.  y is fully overwritten (no pre-zeroing
@@ -92,8 +103,10 @@ int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream);
 
 /* ------------------------------------------------------------------------
  * FISTA building blocks (solvers.py:244-391, proximals.pyx:13-186).
- * All vectors are DEVICE float64 arrays; reductions are deterministic
- * (fixed work split and summation order, no atomics).
+ * Vectors are DEVICE arrays of the plan's precision for the products;
+ * commit_hip_prox / _fista_sums / _axpby take float64 vectors and their _f32
+ * twins float32 ones (weights stay float64; every sum accumulates in FP64).
+ * Reductions are deterministic (fixed work split and order, no atomics).
  * ---------------------------------------------------------------------- */
 
 /*
@@ -186,6 +199,17 @@ int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const 
 /* out = a * x + b * y (FISTA extrapolation, solvers.py:360). */
 int commit_hip_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
                      void *stream);
+
+/* float32 twins of the three above (same arguments, x-like vectors float32) */
+int commit_hip_prox_f32(const commit_hip_prox_params *p, int64_t n, const float *xin, const float *grad,
+                        double step, double scale, float *x, void *stream);
+int commit_hip_fista_sums_f32(int64_t n, const float *x, const float *xhat, const float *grad,
+                              const float *prev, const int64_t *l1_range, const int *l1_weighted,
+                              const double *w, int64_t ngroups, const int32_t *group_ptr,
+                              const int32_t *group_idx, const double *group_w, double *out, double *work,
+                              void *stream);
+int commit_hip_axpby_f32(int64_t n, double a, const float *x, double b, const float *y, float *out,
+                         void *stream);
 
 /* Thread-local message describing the last failure of this thread. */
 const char *commit_hip_last_error(void);
