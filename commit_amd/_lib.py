@@ -43,6 +43,13 @@ SIGNATURES = {
     "commit_hip_fista_sums_f32": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
                                        _P, _P, _P]),
     "commit_hip_axpby_f32": (_I, [ctypes.c_int64, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P]),
+    "commit_hip_fista_multi_work": (ctypes.c_int64, [_P, ctypes.c_int64]),
+    "commit_hip_prox_multi": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "commit_hip_fista_multi_sums": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "commit_hip_multi_residual": (_I, [_P, _P, _P, _I, _P, _P, _P, _P, _P]),
+    "commit_hip_fista_multi_control": (_I, [_P, _P, ctypes.c_double, ctypes.c_double, _I, _P, _P, _P, _P]),
+    "commit_hip_fista_multi_update": (_I, [_P, _P, _P, _P, _P, _P]),
+    "commit_hip_fista_multi_qf": (_I, [_P, _P, _P, _P]),
     "commit_hip_last_error": (ctypes.c_char_p, []),
     "commit_hip_version": (ctypes.c_char_p, []),
 }

@@ -211,6 +211,71 @@ int commit_hip_fista_sums_f32(int64_t n, const float *x, const float *xhat, cons
 int commit_hip_axpby_f32(int64_t n, double a, const float *x, double b, const float *y, float *out,
                          void *stream);
 
+/* ------------------------------------------------------------------------
+ * Batched FISTA (no counterpart in the reference, which fits one problem per
+ * solve call, solvers.py:218-391): nprob x nrhs independent fits advanced
+ * together, nrhs columns of float32 [rows][nrhs] matrices (lambda sweeps,
+ * bootstrap signals) times nprob problems occupying disjoint unknowns and
+ * rows of one block-diagonal plan (tractograms of uneven size).  Every
+ * decision of the reference's loop (step acceptance and backtracking,
+ * stopping tests in order, momentum) is taken per problem on the device, so
+ * the kernels of one trial step can be captured in a HIP graph and replayed.
+ * All sums and scalars are FP64; reductions are deterministic.
+ * ---------------------------------------------------------------------- */
+
+/* State of one (problem, column) fit; device array indexed problem * nrhs + column. */
+typedef struct {
+    double step, told, prev_obj, qfval, coef;
+    double cost, regterm, residual, abs_cost, rel_cost, abs_x, rel_x;
+    double lam[3], lam_group;   /* the penalty of this fit (lambda * its weights' scale) */
+    int32_t iterations, done, criterion, advance, trials, pad_;
+} commit_hip_fista_col;
+
+/* criterion codes: 1 abs. objective, 2 rel. objective, 3 abs. unknowns, 4 rel. unknowns, 5 max iterations */
+
+typedef struct {
+    int64_t n, m;                 /* unknowns (cols of A) and rows */
+    int nrhs, nprob;
+    const int32_t *segx;          /* [n] problem of each unknown (device) */
+    const int64_t *xpieces;       /* [2*nxpieces] [begin, end) unknown ranges, each inside one problem */
+    const int32_t *xpseg;         /* [nxpieces] their problem */
+    int64_t nxpieces;
+    const int64_t *ypieces;       /* [2*nypieces] row ranges, each inside one problem */
+    const int32_t *ypseg;
+    int64_t nypieces;
+    const int32_t *gseg;          /* [ngroups] problem of each group (group penalties), or NULL */
+} commit_hip_multi_layout;
+
+/* doubles of workspace commit_hip_fista_multi_sums needs; the residual needs nypieces * nrhs */
+int64_t commit_hip_fista_multi_work(const commit_hip_multi_layout *L, int64_t ngroups);
+
+/* x = prox(xhat - step * grad, step) per fit (step and lambdas from col); finished fits keep x */
+int commit_hip_prox_multi(const commit_hip_prox_params *p, const commit_hip_multi_layout *L, const float *xhat,
+                          const float *grad, float *x, const commit_hip_fista_col *col, void *stream);
+
+/* out[(problem * nrhs + column) * 8 + k]: the eight sums of commit_hip_fista_sums per fit */
+int commit_hip_fista_multi_sums(const commit_hip_prox_params *p, const commit_hip_multi_layout *L, const float *x,
+                                const float *xhat, const float *grad, const float *prev, const int64_t *l1_range,
+                                const int *l1_weighted, double *out, double *work, void *stream);
+
+/* res = sqrtw .* (AX - y) (res may be NULL) and rsq[problem * nrhs + column] = ||res||^2;
+ * y is [m] shared by the columns, or [m][nrhs] when y_per_column */
+int commit_hip_multi_residual(const commit_hip_multi_layout *L, const float *AX, const float *y, int y_per_column,
+                              const float *sqrtw, float *res, double *rsq, double *work, void *stream);
+
+/* accept / backtrack / stop each fit after a trial (sums of the trial x, rsq of A x - y) */
+int commit_hip_fista_multi_control(const commit_hip_prox_params *p, const commit_hip_multi_layout *L, double tol_fun,
+                                   double tol_x, int max_iter, const double *sums, const double *rsq,
+                                   commit_hip_fista_col *col, void *stream);
+
+/* accepted fits: xhat = x + coef (x - prev), prev = x */
+int commit_hip_fista_multi_update(const commit_hip_multi_layout *L, const float *x, float *xhat, float *prev,
+                                  const commit_hip_fista_col *col, void *stream);
+
+/* accepted fits: qfval = ||res at the new xhat||^2 / 2 */
+int commit_hip_fista_multi_qf(const commit_hip_multi_layout *L, const double *rsq, commit_hip_fista_col *col,
+                              void *stream);
+
 /* Thread-local message describing the last failure of this thread. */
 const char *commit_hip_last_error(void);
 
