@@ -50,6 +50,8 @@ SIGNATURES = {
     "commit_hip_fista_multi_control": (_I, [_P, _P, ctypes.c_double, ctypes.c_double, _I, _P, _P, _P, _P]),
     "commit_hip_fista_multi_update": (_I, [_P, _P, _P, _P, _P, _P]),
     "commit_hip_fista_multi_qf": (_I, [_P, _P, _P, _P]),
+    "commit_hip_plan_set_lut": (_I, [_P, _P, _P, _P, _P]),
+    "commit_hip_lut_grad": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_last_error": (ctypes.c_char_p, []),
     "commit_hip_version": (ctypes.c_char_p, []),
 }

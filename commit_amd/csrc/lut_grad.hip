@@ -1,0 +1,203 @@
+// Gradient of y = A(LUT) x with respect to the response-function tables, and
+// replacement of the plan's tables from device memory: the two pieces a
+// differentiable operator needs to fit the compartment diffusivities (the
+// tables are functions of them) by gradient.  No counterpart in the
+// reference, whose tables are fixed inputs (setup_operator.py:93,151,299).
+//
+// With gy = dL/dy (rows v*nS + s):
+//   dL/dwmr[i][o][s] = sum over IC segments (f, v, o, l):  l * x[i*nF + f] * eval * gy[v*nS + s]
+//   dL/dwmh[i][o][s] = sum over EC peaks (v, o) of column e: x[EC i, e] * gy[v*nS + s]
+//   dL/diso[i][s]    = sum over ISO voxels v of column k:   x[ISO i, k] * gy[v*nS + s]
+// (the LUT entry multiplies exactly these terms in COMMIT_A, setup_operator.py:62-218).
+//
+// Scatter-reduction with privatised LDS tables: a work-group is one wavefront
+// that owns an FP64 table of 250 orientations x 64 samples (128 KB of LDS) for
+// one (RF, sample chunk, orientation part) and walks a slab of voxels; lane s
+// only ever touches column s, entries are applied in order, so no atomics are
+// needed and the result is deterministic.  The slab tables go to a partial
+// buffer and a second kernel adds the slabs in order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/commit_hip.h"
+#include "common.h"
+#include "plan.h"
+
+using commit_internal::fail;
+
+namespace {
+
+using commit_internal::Comp;
+using commit_internal::DevDict;
+
+constexpr int kOrPart = 250;        // orientations per LDS table
+constexpr int kSlabs = 48;          // voxel slabs (work-groups per table)
+
+// Pad [rows][nS] float tables into the plan's [rows][nSp] layout.
+__global__ void k_pad_lut(const float *__restrict__ src, float *__restrict__ dst, int64_t rows, int nS, int nSp)
+{
+    const int64_t total = rows * nSp;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / nSp;
+        const int s = (int)(t - r * nSp);
+        dst[t] = s < nS ? src[r * nS + s] : 0.0f;
+    }
+}
+
+// part[((slab * nparts + h) * nchunk + c) * kOrPart * 64 + (o - h*kOrPart) * 64 + lane]
+// for one RF of one compartment.  xb: x block of the RF (IC: + eval applied
+// through ev), lengths for IC.
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int ndirs, const T *__restrict__ xb,
+                                                         const uint32_t *__restrict__ ev, const T *__restrict__ gy,
+                                                         int nparts, int nchunk, double *__restrict__ part)
+{
+    __shared__ double tab[kOrPart * kWave];
+    const int lane = threadIdx.x;
+    const int bid = blockIdx.x;
+    const int slab = bid / (nparts * nchunk), rem = bid - slab * nparts * nchunk;
+    const int h = rem / nchunk, c = rem - h * nchunk;
+    const int o0 = h * kOrPart;
+    const int no = min(kOrPart, ndirs - o0);
+    for (int k = lane; k < kOrPart * kWave; k += kWave) tab[k] = 0.0;
+    const int s = c * kWave + lane;
+    const int v0 = (int)(((int64_t)d.nVOX * slab) / kSlabs), v1 = (int)(((int64_t)d.nVOX * (slab + 1)) / kSlabs);
+    for (int v = v0; v < v1; v++) {
+        const uint32_t b = cp.ptr[v], e = cp.ptr[v + 1];
+        if (b == e) continue;
+        const double g = s < d.nS ? (double)gy[(size_t)v * d.nS + s] : 0.0;
+        for (uint32_t base = b; base < e; base += kWave) {
+            const uint32_t k = base + lane;
+            double cf = 0.0;
+            int o = -1;
+            if (k < e) {
+                o = cp.o ? (int)cp.o[k] : 0;
+                const uint32_t col = cp.col[k];
+                cf = (double)xb[col];
+                if (cp.len) cf *= (double)cp.len[k];
+                if (ev && ev[col] == 0u) cf = 0.0;
+                if (o < o0 || o >= o0 + no) o = -1;                 // another part's orientation
+            }
+            unsigned long long m = __ballot(o >= 0 && cf != 0.0);
+            while (m) {
+                const int t = __builtin_ctzll(m);
+                m &= m - 1;
+                const int ot = __builtin_amdgcn_readlane(o, t) - o0;
+                const long long cb = __double_as_longlong(cf);
+                const int lo = __builtin_amdgcn_readlane((int)(cb & 0xffffffffll), t);
+                const int hi = __builtin_amdgcn_readlane((int)(cb >> 32), t);
+                const double ct = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+                double *p = tab + ot * kWave + lane;
+                *p = fma(ct, g, *p);
+            }
+        }
+    }
+    double *dst = part + ((size_t)bid) * kOrPart * kWave;
+    for (int k = lane; k < no * kWave; k += kWave) dst[k] = tab[k];
+}
+
+// out[o][s] (+ RF offset) = sum over slabs of the partial tables, in slab order.
+__global__ void k_lut_grad_reduce(const double *__restrict__ part, int ndirs, int nS, int nparts, int nchunk,
+                                  double *__restrict__ out)
+{
+    const int64_t total = (int64_t)ndirs * nS;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int o = (int)(t / nS), s = (int)(t - (int64_t)o * nS);
+        const int h = o / kOrPart, c = s / kWave, lane = s - c * kWave;
+        double acc = 0.0;
+        for (int p = 0; p < kSlabs; p++) {
+            const size_t bid = ((size_t)p * nparts + h) * nchunk + c;
+            acc += part[bid * kOrPart * kWave + (size_t)(o - h * kOrPart) * kWave + lane];
+        }
+        out[t] = acc;
+    }
+}
+
+template <typename T>
+int lut_grad(commit_hip_plan *plan, const T *x, const T *gy, double *gwmr, double *gwmh, double *giso,
+             hipStream_t st)
+{
+    const DevDict &d = plan->d;
+    const int nparts = (d.ndirs + kOrPart - 1) / kOrPart;
+    const int nchunk = (d.nS + kWave - 1) / kWave;
+    const size_t blocks = (size_t)kSlabs * nparts * nchunk;
+    const size_t need = blocks * kOrPart * kWave;
+    if (plan->lgrad_elems < need) {
+        void *p = nullptr;
+        int rc = commit_internal::dev_alloc(plan, need * sizeof(double), &p);
+        if (rc) return rc;
+        plan->lgrad = static_cast<double *>(p);
+        plan->lgrad_elems = need;
+    }
+    const unsigned rg = (unsigned)std::min<int64_t>(((int64_t)d.ndirs * d.nS + 255) / 256, 4096);
+    if (gwmr)
+        for (int i = 0; i < d.nIC; i++) {
+            hipLaunchKernelGGL(k_lut_grad_part<T>, dim3((unsigned)blocks), dim3(kWave), 0, st, d, d.ic, d.ndirs,
+                               x + (size_t)i * d.nF, d.eval + (size_t)i * d.nF, gy, nparts, nchunk, plan->lgrad);
+            hipLaunchKernelGGL(k_lut_grad_reduce, dim3(rg), dim3(256), 0, st, plan->lgrad, d.ndirs, d.nS, nparts,
+                               nchunk, gwmr + (size_t)i * d.ndirs * d.nS);
+        }
+    if (gwmh)
+        for (int i = 0; i < d.nEC; i++) {
+            hipLaunchKernelGGL(k_lut_grad_part<T>, dim3((unsigned)blocks), dim3(kWave), 0, st, d, d.ec, d.ndirs,
+                               x + (size_t)d.nIC * d.nF + (size_t)i * d.nE, (const uint32_t *)nullptr, gy, nparts,
+                               nchunk, plan->lgrad);
+            hipLaunchKernelGGL(k_lut_grad_reduce, dim3(rg), dim3(256), 0, st, plan->lgrad, d.ndirs, d.nS, nparts,
+                               nchunk, gwmh + (size_t)i * d.ndirs * d.nS);
+        }
+    if (giso) {   // ISO: one row per RF (a single "orientation"), same scatter-reduction
+        const size_t b1 = (size_t)kSlabs * nchunk;
+        const unsigned r1 = (unsigned)std::max<int64_t>(1, ((int64_t)d.nS + 255) / 256);
+        for (int i = 0; i < d.nISO; i++) {
+            hipLaunchKernelGGL(k_lut_grad_part<T>, dim3((unsigned)b1), dim3(kWave), 0, st, d, d.iso, 1,
+                               x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE + (size_t)i * d.nI,
+                               (const uint32_t *)nullptr, gy, 1, nchunk, plan->lgrad);
+            hipLaunchKernelGGL(k_lut_grad_reduce, dim3(r1), dim3(256), 0, st, plan->lgrad, 1, d.nS, 1, nchunk,
+                               giso + (size_t)i * d.nS);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int commit_hip_plan_set_lut(commit_hip_plan *plan, const float *wmr, const float *wmh, const float *iso, void *stream)
+{
+    if (!plan) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (plan->nolut) return fail(COMMIT_HIP_EUNSUPPORTED, "a LUT-free plan has no tables");
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DevDict &d = plan->d;
+    auto pad = [&](const float *src, const float *dst, int64_t rows) -> int {
+        if (!src || rows == 0) return 0;
+        const unsigned g = (unsigned)std::min<int64_t>((rows * d.nSp + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_pad_lut, dim3(g), dim3(256), 0, st, src, const_cast<float *>(dst), rows, d.nS, d.nSp);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    };
+    int rc;
+    if ((rc = pad(wmr, d.wmr, (int64_t)d.nIC * d.ndirs))) return rc;
+    if ((rc = pad(wmh, d.wmh, (int64_t)d.nEC * d.ndirs))) return rc;
+    if ((rc = pad(iso, d.isol, (int64_t)d.nISO))) return rc;
+    return 0;
+}
+
+int commit_hip_lut_grad(commit_hip_plan *plan, const void *x, const void *gy, double *gwmr, double *gwmh,
+                        double *giso, void *stream)
+{
+    if (!plan || !x || !gy) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (plan->nolut) return fail(COMMIT_HIP_EUNSUPPORTED, "a LUT-free plan has no tables");
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (plan->precision == COMMIT_HIP_FP32)
+        return lut_grad<float>(plan, static_cast<const float *>(x), static_cast<const float *>(gy), gwmr, gwmh, giso,
+                               st);
+    return lut_grad<double>(plan, static_cast<const double *>(x), static_cast<const double *>(gy), gwmr, gwmh, giso,
+                            st);
+}
+
+}  // extern "C"

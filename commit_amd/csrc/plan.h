@@ -55,6 +55,8 @@ struct commit_hip_plan {
     double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
     double *rpart = nullptr;   // [nVOX + kSumBlocks] residual partials
     float *zmulti = nullptr;   // [nIC][n][16] per-entry products of the multi-RHS A^T pass 1
+    double *lgrad = nullptr;   // per-slab partial tables of the LUT gradient
+    size_t lgrad_elems = 0;
     bool eval_ones = true;
     int64_t bytes = 0;
 };

@@ -211,6 +211,28 @@ int commit_hip_fista_sums_f32(int64_t n, const float *x, const float *xhat, cons
 int commit_hip_axpby_f32(int64_t n, double a, const float *x, double b, const float *y, float *out,
                          void *stream);
 
+/*
+ * Differentiable operator support (no counterpart in the reference, whose
+ * response-function tables are fixed inputs, setup_operator.py:93,151,299).
+ *
+ * commit_hip_plan_set_lut: replace the plan's tables from DEVICE float32
+ * arrays wmr [nIC][ndirs][nS], wmh [nEC][ndirs][nS], iso [nISO][nS] (NULL:
+ * keep that table).
+ *
+ * commit_hip_lut_grad: gradient of L(y) with y = A x with respect to the
+ * tables, given x and gy = dL/dy (device vectors of the plan's precision):
+ *   gwmr[i][o][s] = sum_{IC segments (f,v,o,l)} l * x[i*nF+f] * eval * gy[v*nS+s]
+ *   gwmh[i][o][s] = sum_{EC peaks (v,o), column e} x[nIC*nF + i*nE + e] * gy[v*nS+s]
+ *   giso[i][s]    = sum_{ISO voxels v, column k} x[nIC*nF + nEC*nE + i*nV + k] * gy[v*nS+s]
+ * FP64 device outputs (NULL: skip that table).  Deterministic (LDS-private
+ * tables per voxel slab, added in slab order).  Keeps a scratch buffer in the
+ * plan after the first call.
+ */
+int commit_hip_plan_set_lut(commit_hip_plan *plan, const float *wmr, const float *wmh, const float *iso,
+                            void *stream);
+int commit_hip_lut_grad(commit_hip_plan *plan, const void *x, const void *gy, double *gwmr, double *gwmh,
+                        double *giso, void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched FISTA (no counterpart in the reference, which fits one problem per
  * solve call, solvers.py:218-391): nprob x nrhs independent fits advanced

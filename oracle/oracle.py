@@ -15,6 +15,9 @@ CPU restatement of the reference algorithm for the operator + FISTA hot path:
 * proximals                 <- proximals.pyx:13-186
 * ``init_regularisation``   <- solvers.py:16-206
 * ``fista`` / ``solve``     <- solvers.py:218-391
+* ``lut_grad``              <- derivative of the entries above with respect to
+                               the LUT values (no reference counterpart; the
+                               terms are those of setup_operator.py:62-218)
 * ``normalize_kernels``     <- Evaluation.load_kernels        core.pyx:412-431
 * ``sort_and_normalize``    <- Evaluation.load_dictionary     core.pyx:509-525, 571-580
 
@@ -211,6 +214,48 @@ def assemble_A(DICTIONARY, KERNELS, nolut=False):
     return sp.csr_matrix(
         (np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nrows, ncols)
     )
+
+
+def lut_grad(DICTIONARY, KERNELS, x, gy):
+    """Gradient of L(A x) with respect to the LUT values, given gy = dL/dy:
+
+        gwmr[i,o,s] = sum_{IC segments (f,v,o,l)} l * eval[i*nF+f] * x[i*nF+f] * gy[v*nS+s]
+        gwmh[i,o,s] = sum_{EC peaks e in voxel v with orientation o} x[nIC*nF + i*nE + e] * gy[v*nS+s]
+        giso[i,s]   = sum_{ISO voxels k in voxel v} x[nIC*nF + nEC*nE + i*nI + k] * gy[v*nS+s]
+
+    i.e. the coefficient of every LUT entry in the closed form of assemble_A.
+    Pinned in tests/test_oracle.py by central finite differences of
+    0.5 ||A(LUT) x - y||^2 on the dense assembly."""
+    IC, EC, ISO = DICTIONARY["IC"], DICTIONARY["EC"], DICTIONARY["ISO"]
+    wmr, wmh, iso = KERNELS["wmr"], KERNELS["wmh"], KERNELS["iso"]
+    nIC, nEC, nISO = wmr.shape[0], wmh.shape[0], iso.shape[0]
+    nS = wmr.shape[2] if wmr.size else (wmh.shape[2] if wmh.size else iso.shape[1])
+    ndirs = wmr.shape[1] if wmr.ndim == 3 else wmh.shape[1]
+    nF, nE, nI = IC["nSTR"], EC["n"], ISO["n"]
+    x = np.asarray(x, dtype=np.float64)
+    G = np.asarray(gy, dtype=np.float64).reshape(-1, nS)
+    ev = IC.get("eval")
+    if ev is None:
+        ev = np.ones(nF * nIC)
+    gwmr = np.zeros((nIC, ndirs, nS))
+    gwmh = np.zeros((nEC, ndirs, nS))
+    giso = np.zeros((nISO, nS))
+    v = np.asarray(IC["vox"], dtype=np.int64)
+    f = np.asarray(IC["str"], dtype=np.int64)
+    o = np.asarray(IC["dir"], dtype=np.int64)
+    l = np.asarray(IC["len"], dtype=np.float64)
+    for i in range(nIC):
+        c = l * x[i * nF + f] * np.asarray(ev, dtype=np.float64)[i * nF + f]
+        np.add.at(gwmr[i], o, c[:, None] * G[v])
+    ve, oe = np.asarray(EC["vox"], dtype=np.int64), np.asarray(EC["dir"], dtype=np.int64)
+    for i in range(nEC):
+        c = x[nIC * nF + i * nE + np.arange(nE)]
+        np.add.at(gwmh[i], oe, c[:, None] * G[ve])
+    vi = np.asarray(ISO["vox"], dtype=np.int64)
+    for i in range(nISO):
+        c = x[nIC * nF + nEC * nE + i * nI + np.arange(nI)]
+        giso[i] = (c[:, None] * G[vi]).sum(axis=0)
+    return gwmr, gwmh, giso
 
 
 # ------------------------------------------------------- C-backed operator --

@@ -311,3 +311,35 @@ def test_composite_prox_three_compartments():
     assert np.allclose(reg["prox"](x.copy(), sc), exp, rtol=0, atol=1e-15)
     om = 0.3 * np.abs(x[:n1]).sum() + 0.2 * np.abs(x[n1:n1 + n2]).sum() + 0.1 * np.abs(x[n1 + n2:]).sum()
     assert reg["omega"](x) == pytest.approx(om, rel=1e-15)
+
+
+def test_lut_grad_matches_finite_differences():
+    """oracle.lut_grad against central differences of 0.5||A(LUT) x - y||^2 on
+    the dense assembly (every LUT entry of a tiny dictionary)."""
+    D, K = random_dictionary(21, nVOX=5, nF=6, n=25, nS=4, ndirs=3, nIC=2, nEC=1, nISO=2, ec_per_voxel=2)
+    ev = np.ones(12, np.uint32)
+    ev[3] = 0
+    D["IC"]["eval"] = ev
+    rng = np.random.default_rng(2)
+    A = oracle.assemble_A(D, K)
+    x = rng.standard_normal(A.shape[1])
+    y = rng.standard_normal(A.shape[0])
+
+    def loss(KK):
+        r = oracle.assemble_A(D, KK) @ x - y
+        return 0.5 * float(r @ r)
+
+    gy = A @ x - y
+    g = oracle.lut_grad(D, K, x, gy)
+    for name, gk in zip(("wmr", "wmh", "iso"), g):
+        base = K[name].astype(np.float64)
+        fd = np.zeros_like(base)
+        for idx in np.ndindex(base.shape):
+            KK = {k: (v.astype(np.float64) if k in ("wmr", "wmh", "iso") else v) for k, v in K.items()}
+            h = 1e-6
+            KK[name] = base.copy()
+            KK[name][idx] += h
+            lp = loss(KK)
+            KK[name][idx] -= 2 * h
+            fd[idx] = (lp - loss(KK)) / (2 * h)
+        assert np.allclose(gk, fd, rtol=1e-6, atol=1e-8), name
