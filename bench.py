@@ -50,12 +50,23 @@ def main():
     ap.add_argument("--fibres", type=int, default=None)
     ap.add_argument("--cpu-pairs", type=int, default=2, help="timed pairs of the CPU baseline (0: skip)")
     ap.add_argument("--rhs", type=int, default=16, help="right-hand sides of the batched leg (0: skip)")
+    ap.add_argument("--fit-iters", type=int, default=500,
+                    help="FISTA iterations of the mixed-precision fit leg (0: skip); batched-fit mode: max_iter")
+    ap.add_argument("--mode", default="headline", choices=("headline", "batched-fit", "autograd"),
+                    help="headline: the operator pairs/s line; batched-fit: 8 tractograms x 16 lambdas; "
+                         "autograd: differentiable operator on 200k fibres")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.mode == "batched-fit":
+        batched_fit_mode(args)
+        return
+    if args.mode == "autograd":
+        autograd_mode(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -81,6 +92,10 @@ def main():
           f"({A._plan.device_bytes / 2**30:.2f} GiB)", file=sys.stderr, flush=True)
 
     dev = torch.device("cuda", 0)
+    # launch options of the plan chosen by timing the variants on this dictionary
+    from commit_amd import autotune
+    tuned = autotune.tune(A, nrhs=max(args.rhs, 16), reps=3)
+    print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     g = torch.Generator(device=dev).manual_seed(seed)
     x = torch.rand(A.shape[1], dtype=torch.float64, device=dev, generator=g)   # dense: no zero-skip
     y = torch.empty(A.shape[0], dtype=torch.float64, device=dev)
@@ -157,6 +172,10 @@ def main():
     if args.rhs > 0:
         batched = batched_leg(A, At, args.rhs, reps, time_op, ms_A + ms_At, seed, dev)
 
+    fit = None
+    if args.fit_iters > 0:
+        fit = mixed_precision_leg(A, seed, args.fit_iters)
+
     cpu = None
     if args.cpu_pairs > 0:
         cpu = cpu_baseline(D, K, x.cpu().numpy(), args.cpu_pairs)
@@ -187,6 +206,8 @@ def main():
                              "pairs_per_s": round(1e3 / (ms_A32 + ms_At32), 2),
                              "algorithmic_bytes_per_product": int(algorithmic_bytes(D, K, 4, 4))},
             "batched": batched,
+            "autotune": tuned,
+            "mixed_precision_fit": fit,
         },
         "roofline": {
             "bound": "hbm", "kernel": dom_name, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
@@ -215,6 +236,210 @@ def batched_leg(A, At, R, reps, time_op, ms_plain_pair, seed, dev):
     print(f"[bench] batched R={R}: A {ms_Am:.3f} ms, A^T {ms_Atm:.3f} ms -> {per_rhs:.3f} ms per RHS pair "
           f"({out['speedup_vs_plain_per_rhs']:.2f}x the plain pair)", file=sys.stderr, flush=True)
     return out
+
+
+def mixed_precision_leg(A, seed, iters, ngroups=1000):
+    """The same group-lasso + non-negativity fit (1,000 random bundles) run with
+    float32 vectors and with float64 vectors on the same plan; reports the
+    float32 fit's x and objective error against the float64 one."""
+    import torch
+    from commit_amd import _lib, solvers, synth
+    dev = A.device
+    n, m = A.shape[1], A.shape[0]
+    P = A._plan
+    nF, nIC, nEC, nISO = P.nF, P.nIC, P.nEC, P.nISO
+    xt = torch.from_numpy(synth.x_true(n, seed + 11)).to(dev)
+    s = A.matvec(xt)
+    sigma = float(s.mean()) / 30.0
+    g = torch.Generator(device=dev).manual_seed(seed + 12)
+    y = torch.sqrt((s + sigma * torch.randn(m, generator=g, dtype=torch.float64, device=dev)) ** 2 +
+                   (sigma * torch.randn(m, generator=g, dtype=torch.float64, device=dev)) ** 2)
+    del s
+    rng = np.random.default_rng(seed + 13)
+    lab = rng.integers(0, ngroups, nF)
+    order = np.argsort(lab, kind="stable")
+    cuts = np.cumsum(np.bincount(lab, minlength=ngroups))[:-1]
+    groups = np.empty(ngroups, dtype=object)
+    for k, gidx in enumerate(np.split(order, cuts)):
+        groups[k] = gidx.astype(np.int32)
+    gw = np.sqrt([gi.size for gi in groups]).astype(np.float64)
+    aty = A.T.matvec(y).cpu().numpy()
+    lam_max = max(np.linalg.norm(aty[gi]) / w for gi, w in zip(groups, gw) if gi.size)   # (core.pyx:1196-1211)
+    rp = {"startIC": 0, "sizeIC": nF * nIC, "startEC": nF * nIC, "sizeEC": P.nE * nEC,
+          "startISO": nF * nIC + P.nE * nEC, "sizeISO": P.nV * nISO, "regIC": "group_lasso", "regEC": None,
+          "regISO": None, "nnIC": True, "nnEC": True, "nnISO": True, "lambdaIC": 0.05 * lam_max,
+          "dictIC_params": {"group_idx_kept": groups, "group_weights": gw}}
+    out = {}
+    for name, prec in (("fp32", _lib.FP32), ("fp64", _lib.FP64)):
+        A.set_precision(prec)
+        reg = solvers.init_regularisation(dict(rp), n=n, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, d = solvers.solve(y, A, A.T, tol_fun=0.0, tol_x=0.0, max_iter=iters, verbose=0, regularisation=reg,
+                             lipschitz_iters=30)
+        torch.cuda.synchronize()
+        out[name] = (x, d, time.perf_counter() - t0)
+        print(f"[bench] {name} fit: {d['iterations']} iterations in {out[name][2]:.2f}s, cost {d['cost_function']:.9e}",
+              file=sys.stderr, flush=True)
+    A.set_precision(_lib.FP64)
+    x32, d32, t32 = out["fp32"]
+    x64, d64, t64 = out["fp64"]
+    dxf = np.abs(x32[:nF * nIC] - x64[:nF * nIC]) / max(np.abs(x64[:nF * nIC]).max(), 1e-300)
+    rel_x = float(np.linalg.norm(x32 - x64) / max(np.linalg.norm(x64), 1e-300))
+    return {"workload": f"group lasso over {ngroups} random bundles + non-negativity, lambda = 0.05 lambda_max, "
+                        f"y = A x_true + Rician noise (SNR 30); both: Lipschitz by 30 FP64 power iterations",
+            "iterations": iters, "s_fp32": round(t32, 3), "s_fp64": round(t64, 3),
+            "ms_per_iteration_fp32": round(1e3 * t32 / max(d32["iterations"], 1), 3),
+            "ms_per_iteration_fp64": round(1e3 * t64 / max(d64["iterations"], 1), 3),
+            "rel_err_x": rel_x, "per_fibre_err_max": float(dxf.max()), "per_fibre_err_p99": float(np.percentile(dxf, 99)),
+            "rel_err_objective": float(abs(d32["cost_function"] - d64["cost_function"]) / abs(d64["cost_function"])),
+            "fp64_cost": d64["cost_function"], "fp32_cost": d32["cost_function"]}
+
+
+def batched_fit_mode(args):
+    """Eight tractograms of 50k..400k fibres (2.5 mm head, nS=288), each with a
+    16-value group-lasso lambda sweep over 64 random bundles, fitted together:
+    one stacked plan, float32 multi-RHS products, device-side FISTA control,
+    graph-replayed trial steps."""
+    import torch
+    from commit_amd import _lib, autotune, batched, solvers, synth
+    from commit_amd.operator.operator import LinearOperator
+    dev = torch.device("cuda", 0)
+    sizes = [50_000 * (k + 1) for k in range(8)]
+    t0 = time.time()
+    probs = [synth.make_problem("brain25", seed=2000 + k, nF=nf) for k, nf in enumerate(sizes)]
+    K = probs[0][1]
+    C = batched.Combined([p[0] for p in probs], K)
+    A = C.operator(precision=_lib.FP64)
+    print(f"[bench] {len(sizes)} tractograms stacked in {time.time() - t0:.1f}s: nF={C.F} nVOX={C.V} "
+          f"nnz={C.D['IC']['n']} rows={C.m}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(3000)
+    xt = torch.from_numpy(synth.x_true(C.n, 3001)).to(dev)
+    s = A.matvec(xt)
+    g = torch.Generator(device=dev).manual_seed(3002)
+    sig = float(s.mean()) / 30.0
+    y = torch.sqrt((s + sig * torch.randn(C.m, generator=g, dtype=torch.float64, device=dev)) ** 2 +
+                   (sig * torch.randn(C.m, generator=g, dtype=torch.float64, device=dev)) ** 2)
+    aty = A.T.matvec(y).cpu().numpy()
+    groups, gw, lam_max = [], [], np.zeros(C.S)
+    for sidx in range(C.S):
+        lab = rng.integers(0, 64, C.nF[sidx])
+        for b in range(64):
+            gi = (np.nonzero(lab == b)[0] + C.fo[sidx]).astype(np.int32)
+            if gi.size:
+                groups.append(gi)
+                gw.append(np.sqrt(gi.size))
+                lam_max[sidx] = max(lam_max[sidx], np.linalg.norm(aty[gi]) / gw[-1])
+    gidx = np.empty(len(groups), dtype=object)
+    for k, gi in enumerate(groups):
+        gidx[k] = gi
+    perc = np.logspace(-4, -1, 16)
+    lam = lam_max[:, None] * perc[None, :]
+    rp = {"startIC": 0, "sizeIC": C.nIC * C.F, "startEC": C.nIC * C.F, "sizeEC": C.nEC * C.E,
+          "startISO": C.nIC * C.F + C.nEC * C.E, "sizeISO": C.nISO * C.I, "regIC": "group_lasso", "regEC": None,
+          "regISO": None, "nnIC": True, "nnEC": True, "nnISO": True, "lambdaIC": 0.0,
+          "dictIC_params": {"group_idx_kept": gidx, "group_weights": np.asarray(gw)}}
+    tuned = autotune.tune(A, nrhs=16, reps=2)
+    y32 = y.float()
+    bf = batched.BatchedFista(A, y32, rp, lam, 16, layout=C, tol_fun=1e-5, tol_x=1e-6, max_iter=args.fit_iters)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bf.initialise()
+    bf.capture()
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    steps = bf.run(use_graph=True)
+    torch.cuda.synchronize()
+    t_run = time.perf_counter() - t0
+    det = bf.details()
+    iters = [det[s_][r]["iterations"] for s_ in range(C.S) for r in range(16)]
+    print(f"[bench] batched fits: {steps} trial steps in {t_run:.2f}s (init {t_init:.2f}s); iterations "
+          f"min {min(iters)} max {max(iters)} total {sum(iters)}", file=sys.stderr, flush=True)
+    # one fit re-run alone with the float64 single-vector solver on its own tractogram
+    s0, r0 = 0, 8
+    D0, K0 = probs[s0][0], probs[s0][1]
+    A0 = LinearOperator(D0, K0, None)
+    rows0, cols0 = C.rows(s0), C.columns(s0)
+    gi0 = [gi - C.fo[s0] for gi in groups if gi[0] < C.fo[s0] + C.nF[s0] and gi[0] >= C.fo[s0]]
+    ga = np.empty(len(gi0), dtype=object)
+    for k, gi in enumerate(gi0):
+        ga[k] = gi.astype(np.int32)
+    rp0 = {"startIC": 0, "sizeIC": C.nF[s0] * C.nIC, "startEC": C.nF[s0] * C.nIC, "sizeEC": C.nE[s0] * C.nEC,
+           "startISO": C.nF[s0] * C.nIC + C.nE[s0] * C.nEC, "sizeISO": C.nI[s0] * C.nISO, "regIC": "group_lasso",
+           "regEC": None, "regISO": None, "nnIC": True, "nnEC": True, "nnISO": True, "lambdaIC": float(lam[s0, r0]),
+           "dictIC_params": {"group_idx_kept": ga, "group_weights": np.sqrt([gi.size for gi in gi0])}}
+    y0 = y32[torch.from_numpy(rows0).to(dev)].double()
+    x0, d0 = solvers.solve(y0, A0, A0.T, tol_fun=1e-5, tol_x=1e-6, max_iter=args.fit_iters, verbose=0,
+                           regularisation=solvers.init_regularisation(rp0, device=dev))
+    xb = bf.solution()[cols0, r0]
+    err = float(np.linalg.norm(xb - x0) / max(np.linalg.norm(x0), 1e-300))
+    line = {"metric": "batched group-lasso fits: fit iterations/s (8 tractograms x 16 lambdas, nS=288)",
+            "value": round(sum(iters) / t_run, 2), "unit": "fit-iterations/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f32 storage, f64 control", "data": "synthetic",
+            "config": {"workload": f"8 tractograms of {sizes[0]}..{sizes[-1]} fibres (2.5 mm head), "
+                                   f"{C.F} fibres, {C.D['IC']['n']} segments, {C.V} voxels, 16 lambdas "
+                                   "log-spaced 1e-4..1e-1 x lambda_max over 64 random bundles each",
+                       "fits": C.S * 16, "trial_steps": steps, "s_run": round(t_run, 3), "s_init": round(t_init, 3),
+                       "ms_per_trial_step": round(1e3 * t_run / max(steps, 1), 3),
+                       "iterations_min": min(iters), "iterations_max": max(iters),
+                       "check_fit": {"tractogram": s0, "lambda_index": r0, "rel_err_x_vs_fp64_single": err,
+                                     "iterations_batched": det[s0][r0]["iterations"],
+                                     "iterations_single": d0["iterations"]},
+                       "autotune": tuned}}
+    print(json.dumps(line), flush=True)
+
+
+def autograd_mode(args):
+    """Differentiable operator on 200k fibres, nS=288, float32 vectors: time of
+    one forward + backward (x and table gradients) and the diffusivity fit
+    from (1.7e-3, 0.6e-3) to the (1.9e-3, 0.5e-3) that generated y."""
+    import torch
+    from commit_amd import _lib, autograd, synth
+    from commit_amd.operator.operator import LinearOperator
+    dev = torch.device("cuda", 0)
+    D, K, raw = synth.make_problem("brain200k", seed=1003)
+    K = dict(K)
+    K["dirs"] = raw["dirs"]
+    A = LinearOperator(D, K, None, precision=_lib.FP32)
+    xt = torch.from_numpy(synth.x_true(A.shape[1], 1004)).to(dev, torch.float32)
+    truth = (1.9e-3, 0.5e-3)
+    tt = lambda v: torch.tensor(v, dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        tab = autograd.szb_tables(tt(truth[0]), tt(truth[1]), K["bvals"], K["bvecs"], K["dirs"])
+        y = autograd.apply(A, xt, *[t.float() for t in tab]).clone()
+    tab0 = [t.float().requires_grad_(True) for t in
+            autograd.szb_tables(tt(1.7e-3), tt(0.6e-3), K["bvals"], K["bvecs"], K["dirs"])]
+    x = xt.clone().requires_grad_(True)
+
+    def fb():
+        r = autograd.apply(A, x, *tab0) - y
+        loss = 0.5 * torch.dot(r, r)
+        loss.backward()
+
+    fb()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        fb()
+    torch.cuda.synchronize()
+    ms_fb = (time.perf_counter() - t0) * 1e3 / 5
+    hist = []
+    t0 = time.perf_counter()
+    dp, dq = autograd.fit_diffusivities(A, xt, y, K, (1.7e-3, 0.6e-3), iters=40, history=hist)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    print(f"[bench] autograd: forward+backward {ms_fb:.2f} ms; fit d_par {dp:.6e} d_perp {dq:.6e} "
+          f"in {len(hist)} evaluations, {t_fit:.2f}s", file=sys.stderr, flush=True)
+    line = {"metric": "differentiable operator: forward+backward per second (200k fibres, nS=288)",
+            "value": round(1e3 / ms_fb, 3), "unit": "forward+backward/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32 vectors, f64 accumulation and table gradients", "data": "synthetic",
+            "config": {"workload": f"{D['IC']['nSTR']} fibres, {D['IC']['n']} segments, {D['IC']['nVOX']} voxels",
+                       "ms_forward_backward": round(ms_fb, 3),
+                       "fit": {"start": [1.7e-3, 0.6e-3], "truth": list(truth), "recovered": [dp, dq],
+                               "evaluations": len(hist), "s": round(t_fit, 3),
+                               "loss_first": hist[0][0], "loss_last": hist[-1][0]}}}
+    print(json.dumps(line), flush=True)
 
 
 def copy_calibration():

@@ -13,6 +13,8 @@ SYNTH_PATH = os.path.join(_HERE, "libcommit_synth.so")
 
 FP64 = 0
 FP32 = 1
+OPT_XCD_REMAP = 1
+OPT_MULTI_A_WAVES = 2
 
 # every symbol include/commit_hip.h declares, with its ctypes signature
 _P = ctypes.c_void_p
@@ -28,6 +30,8 @@ SIGNATURES = {
     "commit_hip_plan_device_bytes": (ctypes.c_int64, [_P]),
     "commit_hip_plan_set_precision": (_I, [_P, _I]),
     "commit_hip_plan_precision": (_I, [_P]),
+    "commit_hip_plan_set_option": (_I, [_P, _I, _I]),
+    "commit_hip_plan_get_option": (_I, [_P, _I]),
     "commit_hip_A": (_I, [_P, _P, _P, _P]),
     "commit_hip_At": (_I, [_P, _P, _P, _P]),
     "commit_hip_A_residual": (_I, [_P, _P, _P, _P, _P, _P, _P]),

@@ -16,9 +16,9 @@
 //     so a row is NJ unpredicated coalesced loads.
 //
 // Kernels: one wavefront per voxel, the voxel's samples spread over the 64
-// lanes (NJ per lane).  Work-groups are remapped so that each XCD (own L2)
-// processes one contiguous voxel range: a spatial slab, whose fibres, LUT rows
-// and x entries then share that XCD's L2.
+// lanes (NJ per lane).  Optionally (plan option, chosen by the autotuner)
+// work-groups are renumbered so that each XCD (own L2) processes one
+// contiguous voxel range; on the 1M-fibre head the hardware order is faster.
 //   A x   : NJ FP64 accumulators per lane.  Entry records are read one per
 //           lane and walked in order through a 3-stage software pipeline:
 //           while the row of entry t is applied, the rows of t+1 and t+2 are in
@@ -71,7 +71,7 @@ using commit_internal::Comp;
 using commit_internal::DevDict;
 
 // Voxel of this wavefront (one wavefront per voxel).
-__device__ __forceinline__ int voxel_id() { return commit_internal::wave_index(); }
+__device__ __forceinline__ int voxel_id(const DevDict &d) { return commit_internal::wave_index(d.xcd_remap); }
 
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_A(DevDict d, const T *__restrict__ x
                                               AOut<T> out)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int v = voxel_id();
+    const int v = voxel_id(d);
     if (v >= d.nVOX) return;
     double acc[NJ];
 #pragma unroll
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_At(DevDict d, const T *__restrict__ 
     __shared__ __align__(16) double rings[kWavesPerBlock][kRing * kRingStride];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int v = voxel_id();
+    const int v = voxel_id(d);
     if (v >= d.nVOX) return;
     double *ring = rings[w];
     double yv[NJ];
@@ -578,6 +578,7 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
     d.nIC = (int)nIC;
     d.nEC = (int)nEC;
     d.nISO = (int)nISO;
+    d.xcd_remap = 0;       // hardware order: measured faster on the 1M-fibre head; the autotuner may switch
     int rc = 0;
     {
         std::vector<uint32_t> perm;
@@ -692,6 +693,32 @@ int commit_hip_plan_set_precision(commit_hip_plan *plan, int precision)
 }
 
 int commit_hip_plan_precision(const commit_hip_plan *plan) { return plan ? plan->precision : -1; }
+
+int commit_hip_plan_set_option(commit_hip_plan *plan, int option, int value)
+{
+    if (!plan) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    switch (option) {
+        case COMMIT_HIP_OPT_XCD_REMAP:
+            if (value != 0 && value != 1) return fail(COMMIT_HIP_EINVAL, "XCD remap is 0 or 1");
+            plan->d.xcd_remap = value;
+            return 0;
+        case COMMIT_HIP_OPT_MULTI_A_WAVES:
+            if (value != 1 && value != 2) return fail(COMMIT_HIP_EINVAL, "wavefronts per voxel are 1 or 2");
+            plan->multi_a_waves = value;
+            return 0;
+        default: return fail(COMMIT_HIP_EINVAL, "unknown option %d", option);
+    }
+}
+
+int commit_hip_plan_get_option(const commit_hip_plan *plan, int option)
+{
+    if (!plan) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    switch (option) {
+        case COMMIT_HIP_OPT_XCD_REMAP: return plan->d.xcd_remap;
+        case COMMIT_HIP_OPT_MULTI_A_WAVES: return plan->multi_a_waves;
+        default: return fail(COMMIT_HIP_EINVAL, "unknown option %d", option);
+    }
+}
 
 }  // extern "C"
 

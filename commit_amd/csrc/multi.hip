@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
 {
     constexpr int NT = NH * MS;
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = wave_index();
+    const int w = wave_index(d.xcd_remap);
     const int v = w / (nrg * MS), rem = w - v * (nrg * MS);
     const int rg = rem / MS, h = rem - rg * MS;
     if (v >= d.nVOX) return;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
     __shared__ __align__(16) float tiles[kWavesPerBlock][256 * NT];   // 16 rows x 16NT floats per wave
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int v = wave_index();
+    const int v = wave_index(d.xcd_remap);
     if (v >= d.nVOX) return;
     float *T = tiles[w];
     const int g = lane >> 4, q = lane & 15;
@@ -400,7 +400,7 @@ int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, 
     HIP_TRY(hipSetDevice(plan->device));
     const DevDict &d = plan->d;
     const int nrg = (nrhs + kTileR - 1) / kTileR;
-    const size_t waves = (size_t)d.nVOX * nrg * 2;
+    const size_t waves = (size_t)d.nVOX * nrg * plan->multi_a_waves;
     if (waves == 0) return 0;
     if (waves > (size_t)INT32_MAX - kWavesPerBlock) return fail(COMMIT_HIP_EUNSUPPORTED, "too many voxels x RHS groups");
     const unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -408,16 +408,30 @@ int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     const float *x = static_cast<const float *>(X);
     float *y = static_cast<float *>(Y);
-    // two wavefronts per (voxel, RHS group), each owning half of the sample
-    // tiles (NT rounded up to even; the padded LUT rows hold zeros there)
-    switch ((nt_of(d) + 1) / 2) {
-#define CASE(NH)                                                                                                \
-    case NH:                                                                                                    \
-        hipLaunchKernelGGL((k_A_mm<NH, 2>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev);         \
+    // one or two wavefronts per (voxel, RHS group); with two, each owns half
+    // of the sample tiles (NT rounded up to even; the padded LUT rows hold
+    // zeros there).  The choice is a plan option (commit_hip_plan_set_option).
+    if (plan->multi_a_waves == 2) {
+        switch ((nt_of(d) + 1) / 2) {
+#define CASE(NH)                                                                                            \
+    case NH:                                                                                                \
+        hipLaunchKernelGGL((k_A_mm<NH, 2>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev);     \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
-        default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+        }
+    } else {
+        switch (nt_of(d)) {
+#define CASE(NT)                                                                                            \
+    case NT:                                                                                                \
+        hipLaunchKernelGGL((k_A_mm<NT, 1>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev);     \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+            CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
+#undef CASE
+            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+        }
     }
     HIP_TRY(hipGetLastError());
     return 0;

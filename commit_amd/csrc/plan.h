@@ -27,18 +27,19 @@ struct DevDict {
     const uint32_t *fib_perm;       // [n] IC entry index, fibre-major, ascending voxel
     const float *wmr, *wmh, *isol;  // padded LUTs: rows of nSp floats
     int nS, nSp, ndirs, nF, n, nE, nI, nVOX, nIC, nEC, nISO;
+    int xcd_remap;                  // 1: XCD-contiguous work ranges (wave_index)
 };
 
-// Global wavefront index.  Blocks are dealt to the XCDs round-robin; block b
-// is remapped so that XCD (b % 8) walks a contiguous range of the work (a
-// spatial slab of voxels, whose fibres, LUT rows and x entries then share that
-// XCD's L2).  The result is made scalar: the compiler cannot prove
-// threadIdx.x >> 6 uniform.
-__device__ __forceinline__ int wave_index()
+// Global wavefront index.  Blocks are dealt to the XCDs round-robin; with
+// remap, block b is renumbered so that XCD (b % 8) walks a contiguous range of
+// the work (a spatial slab of voxels, whose fibres, LUT rows and x entries
+// then share that XCD's L2).  The result is made scalar: the compiler cannot
+// prove threadIdx.x >> 6 uniform.
+__device__ __forceinline__ int wave_index(int remap)
 {
     const int nb = (int)gridDim.x, b = (int)blockIdx.x;
     const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
-    const int lb = x * q + min(x, r) + b / kXcds;
+    const int lb = remap ? x * q + min(x, r) + b / kXcds : b;
     return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (int)(threadIdx.x >> 6));
 }
 
@@ -58,6 +59,7 @@ struct commit_hip_plan {
     double *lgrad = nullptr;   // per-slab partial tables of the LUT gradient
     size_t lgrad_elems = 0;
     bool eval_ones = true;
+    int multi_a_waves = 2;     // wavefronts per (voxel, 16 RHS) in the multi-RHS A x (1 or 2)
     int64_t bytes = 0;
 };
 

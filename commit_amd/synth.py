@@ -284,6 +284,10 @@ WORKLOADS = {
     "brain200k": dict(dim=(145, 174, 145), pixdim=1.25, nF=200_000, mode=1, step_vox=0.5,
                       len_mm=(20.0, 200.0), shells=(1000, 2000, 3000), ndirs_per_shell=90, n_b0=18,
                       mask=("ellipsoid", (52.0, 62.0, 45.0))),
+    # the same head at 2.5 mm: one tractogram of a batch of eight
+    "brain25": dict(dim=(73, 87, 73), pixdim=2.5, nF=100_000, mode=1, step_vox=0.5,
+                    len_mm=(20.0, 200.0), shells=(1000, 2000, 3000), ndirs_per_shell=90, n_b0=18,
+                    mask=("ellipsoid", (26.0, 31.0, 22.5))),
 }
 
 

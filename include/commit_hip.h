@@ -85,6 +85,20 @@ int commit_hip_plan_set_precision(commit_hip_plan *plan, int precision);
 int commit_hip_plan_precision(const commit_hip_plan *plan);
 
 /*
+ * Launch options, set by the autotuner (commit_amd.autotune) after timing the
+ * variants on the plan's own data; no counterpart in the reference.
+ *   COMMIT_HIP_OPT_XCD_REMAP     1: blocks renumbered so each XCD walks a
+ *                                contiguous voxel range; 0 (default): hardware order
+ *   COMMIT_HIP_OPT_MULTI_A_WAVES 1 or 2 (default): wavefronts per (voxel, 16 RHS)
+ *                                in commit_hip_A_multi
+ * get returns the value, or a negative code.
+ */
+#define COMMIT_HIP_OPT_XCD_REMAP 1
+#define COMMIT_HIP_OPT_MULTI_A_WAVES 2
+int commit_hip_plan_set_option(commit_hip_plan *plan, int option, int value);
+int commit_hip_plan_get_option(const commit_hip_plan *plan, int option);
+
+/*
  * y = A x.  Replaces COMMIT_A (setup_operator.py:314-365) and, for a LUT-free
  * plan, COMMIT_A_nolut (setup_operator.py:779-813).  x: cols, y: rows, device
  * pointers of the plan's precision.  y is fully overwritten (no pre-zeroing

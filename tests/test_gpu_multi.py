@@ -121,3 +121,36 @@ def test_multi_rejects_bad_input(small):
         A.matmat(torch.zeros((A.shape[1] + 1, 4), dtype=torch.float32, device="cuda"))
     with pytest.raises(RuntimeError):
         A.matmat(torch.zeros((A.shape[1], 4), dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("waves,remap", [(1, 1), (2, 0), (1, 0)])
+def test_launch_options_keep_parity(waves, remap):
+    """Every variant the autotuner may pick computes the same products."""
+    from commit_amd import _lib, autotune
+    D, K = random_dictionary(401, nVOX=12, nF=15, n=220, nS=288, ndirs=6, nIC=1, nEC=1, nISO=2)
+    A = LinearOperator(D, K, None)
+    autotune.set_option(A, _lib.OPT_MULTI_A_WAVES, waves)
+    autotune.set_option(A, _lib.OPT_XCD_REMAP, remap)
+    assert autotune.get_option(A, _lib.OPT_MULTI_A_WAVES) == waves
+    rng = np.random.default_rng(waves + 10 * remap)
+    X = rng.standard_normal((A.shape[1], 16)).astype(np.float32)
+    Y = rng.standard_normal((A.shape[0], 16)).astype(np.float32)
+    dense = oracle.assemble_A(D, K)
+    assert _cols_rel(A.matmat(_dev32(X)).cpu().numpy(), dense @ X.astype(np.float64)) <= TOL32
+    assert _cols_rel(A.T.matmat(_dev32(Y)).cpu().numpy(), dense.T @ Y.astype(np.float64)) <= TOL32
+    x = rng.standard_normal(A.shape[1])
+    assert rel(A.dot(x), dense @ x) <= 1e-12
+
+
+def test_autotune_sets_and_caches(small, tmp_path):
+    from commit_amd import _lib, autotune
+    D, K = small
+    A = LinearOperator(D, K, None)
+    cache = str(tmp_path / "tune.json")
+    rep = autotune.tune(A, nrhs=16, reps=2, cache=cache)
+    assert rep["xcd_remap"]["choice"] in (0, 1) and rep["multi_a_waves"]["choice"] in (1, 2)
+    assert autotune.get_option(A, _lib.OPT_MULTI_A_WAVES) == rep["multi_a_waves"]["choice"]
+    B = LinearOperator(D, K, None)
+    rep2 = autotune.tune(B, nrhs=16, cache=cache)            # from the cache, no timing
+    assert rep2 == rep
+    assert autotune.get_option(B, _lib.OPT_XCD_REMAP) == rep["xcd_remap"]["choice"]
