@@ -229,8 +229,11 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (
 }
 
 // Reduce ring entries [tb, tb+8) over the 64 lanes; store the ones < cnt:
-//   IC     : out[i*m + base + t] = l * dot   (eval is applied per fibre in pass 2)
-//   EC/ISO : out[col + i*xs]     = dot
+//   IC     : z[i*m + zpos] = l * dot   (the entry's fibre-major position, so that
+//            pass 2 reads each fibre's products contiguously; eval is applied
+//            per fibre in pass 2)
+//   EC/ISO : out[col + i*xs] = dot
+// colk holds zpos for IC entries, the x column otherwise.
 template <typename T>
 __device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int lane, double l,
                                        uint32_t colk, bool ic, uint32_t m, double *__restrict__ z, T *__restrict__ out,
@@ -253,7 +256,7 @@ __device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int 
     const uint32_t ce = (uint32_t)__shfl((int)colk, src);
     if ((lane & 7) == 0 && te < cnt) {
         if (ic)
-            z[(size_t)i * m + base + te] = le * q;              // (setup_operator.py:412-428)
+            z[(size_t)i * m + ce] = le * q;                     // (setup_operator.py:412-428)
         else
             out[(size_t)ce + (size_t)i * xs] = (T)q;            // (setup_operator.py:444-563)
     }
@@ -262,8 +265,9 @@ __device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int 
 // One compartment of the voxel: per entry a partial dot product into the ring,
 // flushed every 8 entries.  Rows are pipelined as in a_comp.
 template <int NJ, typename T>
-__device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, uint32_t b, uint32_t e,
-                                        double *__restrict__ z, T *__restrict__ out, size_t xs, int nrf,
+__device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, const uint32_t *__restrict__ zpos,
+                                        uint32_t b, uint32_t e, double *__restrict__ z, T *__restrict__ out,
+                                        size_t xs, int nrf,
                                         const float *__restrict__ lut, size_t lut_rf, int nSp, int lane,
                                         double *ring)
 {
@@ -277,7 +281,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
             double l = 0.0;
             if (k < e) {
                 if (cp.o) o = stream_load(cp.o + k);
-                colk = stream_load(cp.col + k);
+                colk = stream_load((ic ? zpos : cp.col) + k);
                 if (ic) l = (double)stream_load(cp.len + k);
             }
             float ra[NJ], rb[NJ], rc[NJ];
@@ -329,18 +333,20 @@ __global__ __launch_bounds__(kBlock) void k_At(DevDict d, const T *__restrict__ 
         if (sqrtw && s < d.nS) yv[j] *= (double)stream_load(sqrtw + (size_t)v * d.nS + s);   // (solvers.py:267)
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    at_comp<NJ>(yv, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], z, x, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
+    at_comp<NJ>(yv, d.ic, d.zpos, d.ic.ptr[v], d.ic.ptr[v + 1], z, x, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
     if (d.nEC > 0)
-        at_comp<NJ>(yv, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], z, x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
-                    d.wmh, lut_rf, d.nSp, lane, ring);
+        at_comp<NJ>(yv, d.ec, d.zpos, d.ec.ptr[v], d.ec.ptr[v + 1], z, x + (size_t)d.nIC * d.nF, (size_t)d.nE,
+                    d.nEC, d.wmh, lut_rf, d.nSp, lane, ring);
     if (d.nISO > 0)
-        at_comp<NJ>(yv, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], z, x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
-                    (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane, ring);
+        at_comp<NJ>(yv, d.iso, d.zpos, d.iso.ptr[v], d.iso.ptr[v + 1], z,
+                    x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0,
+                    lane, ring);
 }
 
 // A^T y pass 2: x[f + i nF] = eval * sum of the fibre's products, ascending voxel
-// order.  eval is 0 or 1, so applying it once per fibre equals the reference's
-// per-segment product (setup_operator.py:428).
+// order (pass 1 stored them fibre-major, so the run is contiguous).  eval is 0
+// or 1, so applying it once per fibre equals the reference's per-segment
+// product (setup_operator.py:428).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_At_fibre(DevDict d, const double *__restrict__ z, T *__restrict__ x)
 {
@@ -353,14 +359,14 @@ __global__ __launch_bounds__(kBlock) void k_At_fibre(DevDict d, const double *__
         const uint32_t kb = d.fib_ptr[f], ke = d.fib_ptr[f + 1];
         double acc = 0.0;
         uint32_t k = kb;
-        for (; k + 8 <= ke; k += 8) {      // 8 gathers in flight, summed in order
+        for (; k + 8 <= ke; k += 8) {      // 8 loads in flight, summed in order
             double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = zi[d.fib_perm[k + u]];
+            for (int u = 0; u < 8; u++) v[u] = stream_load(zi + k + u);
 #pragma unroll
             for (int u = 0; u < 8; u++) acc += v[u];
         }
-        for (; k < ke; k++) acc += zi[d.fib_perm[k]];
+        for (; k < ke; k++) acc += stream_load(zi + k);
         x[g] = (T)(acc * (double)d.eval[g]);
     }
 }
@@ -595,6 +601,9 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
         counting_sort(f.data(), (size_t)n, (uint32_t)nF, fperm, fptr);
         if ((rc = upload(p, fptr.data(), fptr.size(), &d.fib_ptr))) goto done;
         if ((rc = upload(p, fperm.data(), (size_t)n, &d.fib_perm))) goto done;
+        std::vector<uint32_t> zp((size_t)n);
+        for (int k = 0; k < n; k++) zp[fperm[k]] = (uint32_t)k;
+        if ((rc = upload(p, zp.data(), (size_t)n, &d.zpos))) goto done;
         if ((rc = upload(p, ICeval, (size_t)nF * nIC, &d.eval))) goto done;
         for (size_t q = 0; q < (size_t)nF * nIC; q++)
             if (ICeval[q] != 1u) { p->eval_ones = false; break; }

@@ -25,6 +25,7 @@ struct DevDict {
     const uint32_t *eval;           // [nF*nIC]
     const uint32_t *fib_ptr;        // [nF+1]
     const uint32_t *fib_perm;       // [n] IC entry index, fibre-major, ascending voxel
+    const uint32_t *zpos;           // [n] inverse of fib_perm: fibre-major position of each IC entry
     const float *wmr, *wmh, *isol;  // padded LUTs: rows of nSp floats
     int nS, nSp, ndirs, nF, n, nE, nI, nVOX, nIC, nEC, nISO;
     int xcd_remap;                  // 1: XCD-contiguous work ranges (wave_index)
