@@ -87,12 +87,15 @@ __device__ __forceinline__ T stream_load(const T *p)
     return __builtin_nontemporal_load(p);    // read once per product: keep L2 for LUT rows and x
 }
 
-// NJ coalesced loads of one padded LUT row (row points at this lane's sample 0)
+// NJ coalesced loads of one padded LUT row (row points at this lane's sample
+// 0).  tail: this lane's sample of the last chunk is < nS; lanes past nS skip
+// that load (their padded entries are zero), so the padding costs no traffic.
 template <int NJ>
-__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row)
+__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row, bool tail)
 {
 #pragma unroll
-    for (int j = 0; j < NJ; j++) raw[j] = row[kWave * j];
+    for (int j = 0; j < NJ - 1; j++) raw[j] = row[kWave * j];
+    raw[NJ - 1] = tail ? row[kWave * (NJ - 1)] : 0.0f;
 }
 
 // next set bit of m (consumed), or -1
@@ -118,7 +121,7 @@ __device__ __forceinline__ void fma_row(double (&acc)[NJ], double c, const float
 template <int NJ, typename T>
 __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32_t b, uint32_t e,
                                        const T *__restrict__ xb, size_t xs, int nrf,
-                                       const float *__restrict__ lut, size_t lut_rf, int nSp, int lane)
+                                       const float *__restrict__ lut, size_t lut_rf, int nSp, int lane, bool tail)
 {
     for (int i = 0; i < nrf; i++) {
         const float *L = lut + (size_t)i * lut_rf + lane;
@@ -136,18 +139,18 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
             unsigned long long m = __ballot(c != 0.0);
             float ra[NJ], rb[NJ], rc[NJ];
             int ta = take(m), tb = -1, tc = -1;
-            if (ta >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp);
-            if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp);
+            if (ta >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
+            if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
             // three rows rotate through the pipeline; entries keep their order
             while (true) {
                 if (ta < 0) break;
-                if ((tc = take(m)) >= 0) fetch_row<NJ>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp);
+                if ((tc = take(m)) >= 0) fetch_row<NJ>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp, tail);
                 fma_row<NJ>(acc, readlane_f64(c, ta), ra);
                 if (tb < 0) break;
-                if ((ta = take(m)) >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp);
+                if ((ta = take(m)) >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
                 fma_row<NJ>(acc, readlane_f64(c, tb), rb);
                 if (tc < 0) break;
-                if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp);
+                if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
                 fma_row<NJ>(acc, readlane_f64(c, tc), rc);
             }
         }
@@ -178,13 +181,14 @@ __global__ __launch_bounds__(kBlock) void k_A(DevDict d, const T *__restrict__ x
 #pragma unroll
     for (int j = 0; j < NJ; j++) acc[j] = 0.0;
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    a_comp<NJ>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], xic, (size_t)d.nF, d.nIC, d.wmr, lut_rf, d.nSp, lane);
+    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    a_comp<NJ>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], xic, (size_t)d.nF, d.nIC, d.wmr, lut_rf, d.nSp, lane, tail);
     if (d.nEC > 0)
         a_comp<NJ>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
-                   d.wmh, lut_rf, d.nSp, lane);
+                   d.wmh, lut_rf, d.nSp, lane, tail);
     if (d.nISO > 0)
         a_comp<NJ>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
-                   (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane);
+                   (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane, tail);
     const size_t row = (size_t)v * d.nS;
     double sq = 0.0;
 #pragma unroll
@@ -269,7 +273,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
                                         uint32_t b, uint32_t e, double *__restrict__ z, T *__restrict__ out,
                                         size_t xs, int nrf,
                                         const float *__restrict__ lut, size_t lut_rf, int nSp, int lane,
-                                        double *ring)
+                                        bool tail, double *ring)
 {
     const bool ic = cp.len != nullptr;
     for (int i = 0; i < nrf; i++) {
@@ -286,11 +290,11 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
             }
             float ra[NJ], rb[NJ], rc[NJ];
             int t = 0;
-            fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, 0) * nSp);
-            if (cnt > 1) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, 1) * nSp);
+            fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, 0) * nSp, tail);
+            if (cnt > 1) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, 1) * nSp, tail);
 #define AT_STAGE(CUR, NEXT2)                                                                              \
     {                                                                                                     \
-        if (t + 2 < cnt) fetch_row<NJ>(NEXT2, L + (size_t)__builtin_amdgcn_readlane((int)o, t + 2) * nSp); \
+        if (t + 2 < cnt) fetch_row<NJ>(NEXT2, L + (size_t)__builtin_amdgcn_readlane((int)o, t + 2) * nSp, tail); \
         ring[(size_t)(t % kRing) * kRingStride + lane] = dot_row<NJ>(CUR, yv);                            \
         if ((t & 7) == 7) {                                                                               \
             __builtin_amdgcn_wave_barrier();                                                              \
@@ -333,14 +337,16 @@ __global__ __launch_bounds__(kBlock) void k_At(DevDict d, const T *__restrict__ 
         if (sqrtw && s < d.nS) yv[j] *= (double)stream_load(sqrtw + (size_t)v * d.nS + s);   // (solvers.py:267)
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    at_comp<NJ>(yv, d.ic, d.zpos, d.ic.ptr[v], d.ic.ptr[v + 1], z, x, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, ring);
+    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    at_comp<NJ>(yv, d.ic, d.zpos, d.ic.ptr[v], d.ic.ptr[v + 1], z, x, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, tail,
+                ring);
     if (d.nEC > 0)
         at_comp<NJ>(yv, d.ec, d.zpos, d.ec.ptr[v], d.ec.ptr[v + 1], z, x + (size_t)d.nIC * d.nF, (size_t)d.nE,
-                    d.nEC, d.wmh, lut_rf, d.nSp, lane, ring);
+                    d.nEC, d.wmh, lut_rf, d.nSp, lane, tail, ring);
     if (d.nISO > 0)
         at_comp<NJ>(yv, d.iso, d.zpos, d.iso.ptr[v], d.iso.ptr[v + 1], z,
                     x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0,
-                    lane, ring);
+                    lane, tail, ring);
 }
 
 // A^T y pass 2: x[f + i nF] = eval * sum of the fibre's products, ascending voxel
