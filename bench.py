@@ -147,7 +147,8 @@ def main():
     ms_At1 = time_op(lambda: at_pass(1), reps)
     ms_At2 = time_op(lambda: at_pass(2), reps)
     bytes_pass = algorithmic_bytes(D, K)
-    kernels = {"k_A_fp64": ms_A, "k_At_fp64": ms_At1}
+    nj = (int(K["wmr"].shape[2]) + 63) // 64
+    kernels = {f"k_A<{nj}, double>": ms_A, f"k_At<{nj}, double>": ms_At1}
     dom_name = max(kernels, key=kernels.get)
     dom_ms = kernels[dom_name]
     dom_gbs = bytes_pass / (dom_ms * 1e-3) / 1e9

@@ -36,3 +36,19 @@ def random_dictionary(seed, nVOX=6, nF=5, n=14, nS=7, ndirs=4, nIC=1, nEC=1, nIS
 def rel(a, b):
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def load_golden(name="operator_small"):
+    """Committed fixture (tests/golden/make_golden.py): dictionary, tables,
+    x, y and the expected A x, A^T y and table gradient (gy = y)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".npz"))
+    nIC = g["wmr"].shape[0]
+    D = {"IC": {"vox": g["IC_vox"], "str": g["IC_str"], "dir": g["IC_dir"], "len": g["IC_len"],
+                "eval": g["IC_eval"], "n": int(g["IC_vox"].size), "nSTR": int(g["nF"]), "nVOX": int(g["nVOX"])},
+         "EC": {"vox": g["EC_vox"], "dir": g["EC_dir"], "n": int(g["EC_vox"].size)},
+         "ISO": {"vox": g["ISO_vox"], "n": int(g["ISO_vox"].size)},
+         "TRK": {"norm": np.ones(int(g["nF"]), np.float32), "kept": np.ones(int(g["nF"]), np.uint8)}}
+    K = {"wmr": g["wmr"], "wmh": g["wmh"], "iso": g["iso"]}
+    assert D["IC"]["eval"].size == int(g["nF"]) * nIC
+    return D, K, {k: g[k] for k in ("x", "y", "Ax", "Aty", "gwmr", "gwmh", "giso")}

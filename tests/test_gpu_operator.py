@@ -130,3 +130,30 @@ def test_adjointness_brain_geometry():
     rhs = torch.dot(x, A.T.dot(y)).item()
     print(f"adjointness: <Ax,y>={lhs:.15e} <x,A^T y>={rhs:.15e}", flush=True)
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_golden_fixture_all_paths():
+    """The committed loop-evaluated fixture through every device path:
+    FP64 (1e-12), FP32 storage (1e-6), multi-RHS (1e-5), table gradient (1e-12)."""
+    import ctypes
+    from _problems import load_golden
+    from commit_amd import _lib
+    D, K, g = load_golden()
+    A = LinearOperator(D, K, None)
+    assert rel(A.dot(g["x"]), g["Ax"]) <= TOL and rel(A.T.dot(g["y"]), g["Aty"]) <= TOL
+    xd = torch.from_numpy(g["x"]).cuda()
+    yd = torch.from_numpy(g["y"]).cuda()
+    out = [torch.zeros(K[k].shape, dtype=torch.float64, device="cuda") for k in ("wmr", "wmh", "iso")]
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    _lib.check(_lib.load().commit_hip_lut_grad(A._plan.handle, P(xd), P(yd), P(out[0]), P(out[1]), P(out[2]),
+                                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "lut_grad")
+    for o, k in zip(out, ("gwmr", "gwmh", "giso")):
+        assert rel(o.cpu().numpy().ravel(), g[k].ravel()) <= TOL
+    A.set_precision(_lib.FP32)
+    assert rel(A.dot(g["x"]), g["Ax"]) <= 1e-6 and rel(A.T.dot(g["y"]), g["Aty"]) <= 1e-6
+    X = np.stack([g["x"] * (r + 1) for r in range(16)], axis=1).astype(np.float32)
+    Y = np.stack([g["y"] * (r + 1) for r in range(16)], axis=1).astype(np.float32)
+    Ym = A.matmat(torch.from_numpy(X).cuda()).cpu().numpy()
+    Xm = A.T.matmat(torch.from_numpy(Y).cuda()).cpu().numpy()
+    for r in range(16):
+        assert rel(Ym[:, r], (r + 1) * g["Ax"]) <= 1e-5 and rel(Xm[:, r], (r + 1) * g["Aty"]) <= 1e-5

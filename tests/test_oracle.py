@@ -343,3 +343,18 @@ def test_lut_grad_matches_finite_differences():
             KK[name][idx] -= 2 * h
             fd[idx] = (lp - loss(KK)) / (2 * h)
         assert np.allclose(gk, fd, rtol=1e-6, atol=1e-8), name
+
+
+def test_oracle_matches_golden_fixture():
+    """The committed loop-evaluated fixture (tests/golden/make_golden.py)."""
+    from _problems import load_golden
+    D, K, g = load_golden()
+    for threads in (1, 4):
+        R = oracle.OracleOperator(D, K, oracle.set_threads(D, threads))
+        assert rel(R.dot(g["x"]), g["Ax"]) < 1e-14
+        assert rel(R.T.dot(g["y"]), g["Aty"]) < 1e-14
+    M = oracle.assemble_A(D, K)
+    assert rel(M @ g["x"], g["Ax"]) < 1e-14
+    gw = oracle.lut_grad(D, K, g["x"], g["y"])
+    for a, b in zip(gw, (g["gwmr"], g["gwmh"], g["giso"])):
+        assert rel(a.ravel(), b.ravel()) < 1e-13
