@@ -15,6 +15,7 @@ FP64 = 0
 FP32 = 1
 OPT_XCD_REMAP = 1
 OPT_MULTI_A_WAVES = 2
+OPT_A_DENSE = 3
 
 # every symbol include/commit_hip.h declares, with its ctypes signature
 _P = ctypes.c_void_p

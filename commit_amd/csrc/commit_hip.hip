@@ -464,6 +464,10 @@ int launch_A(commit_hip_plan *p, const T *x, AOut<T> out, hipStream_t st)
         HIP_TRY(hipGetLastError());
         xic = xe;
     }
+    if (p->a_dense) {
+        const int rc = commit_internal::launch_A_dense<T>(p, x, xic, out.y, out.yobs, out.sqrtw, out.rpart, st);
+        if (rc != COMMIT_HIP_EUNSUPPORTED) return rc;
+    }
     switch (d.nSp / kWave) {
 #define CASE(NJ) \
     case NJ: hipLaunchKernelGGL((k_A<NJ, T>), dim3(grid), dim3(kBlock), 0, st, d, x, xic, out); break;
@@ -625,7 +629,10 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
         if (nISO && (rc = upload(p, row.data(), row.size(), &d.isol))) goto done;
     } else {
         if (nIC) {
-            const std::vector<float> t = pad_rows(wmrSFP, (size_t)nIC * ndirs, nS, d.nSp);
+            // three zero rows after the last RF: the dense-tile kernel reads
+            // orientations in steps of 4 past ndirs (against zero weights)
+            std::vector<float> t = pad_rows(wmrSFP, (size_t)nIC * ndirs, nS, d.nSp);
+            t.resize(t.size() + 3 * (size_t)d.nSp, 0.0f);
             if ((rc = upload(p, t.data(), t.size(), &d.wmr))) goto done;
         }
         if (nEC) {
@@ -721,6 +728,10 @@ int commit_hip_plan_set_option(commit_hip_plan *plan, int option, int value)
             if (value != 1 && value != 2) return fail(COMMIT_HIP_EINVAL, "wavefronts per voxel are 1 or 2");
             plan->multi_a_waves = value;
             return 0;
+        case COMMIT_HIP_OPT_A_DENSE:
+            if (value != 0 && value != 1) return fail(COMMIT_HIP_EINVAL, "dense A is 0 or 1");
+            plan->a_dense = value;
+            return 0;
         default: return fail(COMMIT_HIP_EINVAL, "unknown option %d", option);
     }
 }
@@ -731,6 +742,7 @@ int commit_hip_plan_get_option(const commit_hip_plan *plan, int option)
     switch (option) {
         case COMMIT_HIP_OPT_XCD_REMAP: return plan->d.xcd_remap;
         case COMMIT_HIP_OPT_MULTI_A_WAVES: return plan->multi_a_waves;
+        case COMMIT_HIP_OPT_A_DENSE: return plan->a_dense;
         default: return fail(COMMIT_HIP_EINVAL, "unknown option %d", option);
     }
 }

@@ -61,10 +61,17 @@ struct commit_hip_plan {
     size_t lgrad_elems = 0;
     bool eval_ones = true;
     int multi_a_waves = 2;     // wavefronts per (voxel, 16 RHS) in the multi-RHS A x (1 or 2)
+    int a_dense = 0;           // 1: A x by the dense-tile matrix-core kernel (dense.hip)
     int64_t bytes = 0;
 };
 
 namespace commit_internal {
 // hipMalloc tracked by the plan (freed by commit_hip_plan_destroy)
 int dev_alloc(commit_hip_plan *p, size_t bytes, void **out);
+
+// Dense-tile A x (dense.hip); COMMIT_HIP_EUNSUPPORTED when the plan's sizes
+// are outside what it is built for.
+template <typename T>
+int launch_A_dense(commit_hip_plan *p, const T *x, const T *xic, T *y, const T *yobs, const T *sqrtw, double *rpart,
+                   hipStream_t st);
 }  // namespace commit_internal

@@ -91,10 +91,14 @@ int commit_hip_plan_precision(const commit_hip_plan *plan);
  *                                contiguous voxel range; 0 (default): hardware order
  *   COMMIT_HIP_OPT_MULTI_A_WAVES 1 or 2 (default): wavefronts per (voxel, 16 RHS)
  *                                in commit_hip_A_multi
+ *   COMMIT_HIP_OPT_A_DENSE       0 (default): A x per segment; 1: per tile of 16
+ *                                voxels as a dense FP64 matrix-core product over
+ *                                all orientations (ndirs <= 512, nS <= 320)
  * get returns the value, or a negative code.
  */
 #define COMMIT_HIP_OPT_XCD_REMAP 1
 #define COMMIT_HIP_OPT_MULTI_A_WAVES 2
+#define COMMIT_HIP_OPT_A_DENSE 3
 int commit_hip_plan_set_option(commit_hip_plan *plan, int option, int value);
 int commit_hip_plan_get_option(const commit_hip_plan *plan, int option);
 

@@ -123,14 +123,15 @@ def test_multi_rejects_bad_input(small):
         A.matmat(torch.zeros((A.shape[1], 4), dtype=torch.float64, device="cuda"))
 
 
-@pytest.mark.parametrize("waves,remap", [(1, 1), (2, 0), (1, 0)])
-def test_launch_options_keep_parity(waves, remap):
+@pytest.mark.parametrize("waves,remap,dense", [(1, 1, 0), (2, 0, 1), (1, 0, 1)])
+def test_launch_options_keep_parity(waves, remap, dense):
     """Every variant the autotuner may pick computes the same products."""
     from commit_amd import _lib, autotune
     D, K = random_dictionary(401, nVOX=12, nF=15, n=220, nS=288, ndirs=6, nIC=1, nEC=1, nISO=2)
     A = LinearOperator(D, K, None)
     autotune.set_option(A, _lib.OPT_MULTI_A_WAVES, waves)
     autotune.set_option(A, _lib.OPT_XCD_REMAP, remap)
+    autotune.set_option(A, _lib.OPT_A_DENSE, dense)
     assert autotune.get_option(A, _lib.OPT_MULTI_A_WAVES) == waves
     rng = np.random.default_rng(waves + 10 * remap)
     X = rng.standard_normal((A.shape[1], 16)).astype(np.float32)
@@ -149,6 +150,7 @@ def test_autotune_sets_and_caches(small, tmp_path):
     cache = str(tmp_path / "tune.json")
     rep = autotune.tune(A, nrhs=16, reps=2, cache=cache)
     assert rep["xcd_remap"]["choice"] in (0, 1) and rep["multi_a_waves"]["choice"] in (1, 2)
+    assert autotune.get_option(A, _lib.OPT_A_DENSE) == rep["a_dense"]["choice"]
     assert autotune.get_option(A, _lib.OPT_MULTI_A_WAVES) == rep["multi_a_waves"]["choice"]
     B = LinearOperator(D, K, None)
     rep2 = autotune.tune(B, nrhs=16, cache=cache)            # from the cache, no timing

@@ -157,3 +157,39 @@ def test_golden_fixture_all_paths():
     Xm = A.T.matmat(torch.from_numpy(Y).cuda()).cpu().numpy()
     for r in range(16):
         assert rel(Ym[:, r], (r + 1) * g["Ax"]) <= 1e-5 and rel(Xm[:, r], (r + 1) * g["Aty"]) <= 1e-5
+
+
+@pytest.mark.parametrize("ndirs", [3, 250, 500, 511])
+@pytest.mark.parametrize("nS", [1, 17, 60, 288, 320])
+def test_dense_tile_A(ndirs, nS):
+    """The dense-tile matrix-core A x (plan option) against the dense assembly:
+    plain, residual form with weights, eval mask, float32 storage."""
+    import ctypes
+    from commit_amd import _lib, autotune
+    D, K = random_dictionary(ndirs + 3 * nS, nVOX=37, nF=23, n=700, nS=nS, ndirs=ndirs, nIC=2, nEC=1, nISO=2,
+                             empty_voxels=(5, 17), ec_per_voxel=2)
+    ev = np.ones(46, np.uint32)
+    ev[[0, 9, 30]] = 0
+    D["IC"]["eval"] = ev
+    A = LinearOperator(D, K, None)
+    autotune.set_option(A, _lib.OPT_A_DENSE, 1)
+    rng = np.random.default_rng(ndirs + nS)
+    x = rng.standard_normal(A.shape[1])
+    x[rng.random(x.size) < 0.3] = 0.0
+    dense = oracle.assemble_A(D, K)
+    assert rel(A.dot(x), dense @ x) <= TOL
+    # residual form: sqrtw (A x - y) and its squared norm
+    yobs = rng.standard_normal(A.shape[0])
+    w = rng.uniform(0.5, 1.5, A.shape[0])
+    res = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
+    rsq = torch.empty(1, dtype=torch.float64, device="cuda")
+    P = lambda a: ctypes.c_void_p(a.data_ptr())
+    xd, yd, wd = (torch.from_numpy(a).cuda() for a in (x, yobs, w))
+    _lib.check(_lib.load().commit_hip_A_residual(A._plan.handle, P(xd), P(yd), P(wd), P(res), P(rsq),
+                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "res")
+    r_ref = w * (dense @ x - yobs)
+    assert rel(res.cpu().numpy(), r_ref) <= TOL
+    assert abs(rsq.item() - r_ref @ r_ref) <= 1e-12 * (r_ref @ r_ref)
+    A.set_precision(_lib.FP32)
+    x32 = x.astype(np.float32).astype(np.float64)
+    assert rel(A.dot(x32), dense @ x32) <= 1e-6
