@@ -70,14 +70,31 @@ __device__ __forceinline__ void dense_build(const DevDict &d, const T *__restric
     uint32_t p2 = va + 2 < vz ? d.ic.ptr[va + 2] : e;
     uint32_t p3 = va + 3 < vz ? d.ic.ptr[va + 3] : e;
     const int vb = 4 * wb;
-#pragma unroll 4
-    for (uint32_t base = b; base < e; base += kWave) {
-        const uint32_t k = base + lane;
-        if (k < e) {
-            const int o = d.ic.o[k];
-            const double c = (double)xb[d.ic.col[k]] * (double)d.ic.len[k];   // (setup_operator.py:63-96)
+    // four chunks of 64 entries per round, all loads issued before any use
+    // (entries past e read entry b and are dropped)
+    for (uint32_t base = b; base < e; base += 4 * kWave) {
+        int o[4];
+        uint32_t col[4];
+        float len[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t k = base + u * kWave + lane;
+            ok[u] = k < e;
+            const uint32_t ks = ok[u] ? k : b;
+            o[u] = d.ic.o[ks];
+            col[u] = d.ic.col[ks];
+            len[u] = d.ic.len[ks];
+        }
+        T xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) xv[u] = xb[col[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t k = base + u * kWave + lane;
+            const double c = ok[u] ? (double)xv[u] * (double)len[u] : 0.0;   // (setup_operator.py:63-96)
             const int vl = vb + (k >= p1) + (k >= p2) + (k >= p3);
-            if (c != 0.0) atomicAdd(&Cb[o * kTV + vl], c);
+            if (c != 0.0) atomicAdd(&Cb[o[u] * kTV + vl], c);
         }
     }
 }

@@ -105,6 +105,31 @@ __device__ __forceinline__ void entry_rec(const Comp &cp, uint32_t k, uint32_t e
     }
 }
 
+// Coefficients of a chunk's 16 K-steps for this lane: entry 4t+g, RHS column
+// of the lane.  Straight-line code (one version with eval, one without) so
+// that the 16 gathers are in flight together; entries past the chunk and RHS
+// columns past R read column 0 and are zeroed after.
+template <bool EV>
+__device__ __forceinline__ void gather16(float (&xv)[16], const StepIn &in, uint32_t col, float len, int g, int cnt,
+                                         int i)
+{
+    float lv[16];
+    uint32_t ev[16];
+    bool okv[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int ent = 4 * t + g;
+        const uint32_t ce = (uint32_t)__shfl((int)col, ent);
+        lv[t] = __shfl(len, ent);
+        okv[t] = ent < cnt && in.rhs_ok;
+        const uint32_t cs = okv[t] ? ce : 0u;
+        xv[t] = in.X[((size_t)cs + (size_t)i * in.xs) * (size_t)in.R];
+        if (EV) ev[t] = in.eval[cs + (size_t)i * in.nF];
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++) xv[t] = (okv[t] && (!EV || ev[t] != 0u)) ? lv[t] * xv[t] : 0.0f;
+}
+
 __device__ __forceinline__ const float *row_of(const StepIn &in, uint32_t oc, int ent, int cnt)
 {
     const uint32_t oe = (uint32_t)__shfl((int)oc, ent);
@@ -128,20 +153,14 @@ __device__ __forceinline__ void mm_comp_A(f4 (&acc)[NH], const Comp &cp, uint32_
             const int cnt = (int)min((uint32_t)kWave, e - base);
             const int nsteps = (cnt + 3) >> 2;
             // coefficients l * x * eval of this lane's entries     (setup_operator.py:63-96)
+            // Branch-free: every lane loads (entries past the chunk, and RHS
+            // columns past R, read column 0 of the block) so that the 16
+            // gathers are in flight together; invalid ones are zeroed after.
             float xv[16];
-#pragma unroll
-            for (int t = 0; t < 16; t++) {
-                const int ent = 4 * t + g;
-                const uint32_t ce = (uint32_t)__shfl((int)col, ent);
-                const float le = __shfl(len, ent);
-                float c = 0.0f;
-                if (ent < cnt && in.rhs_ok) {
-                    const size_t cc = (size_t)ce + (size_t)i * in.xs;
-                    c = le * in.X[cc * (size_t)in.R];
-                    if (in.eval && in.eval[ce + (size_t)i * in.nF] == 0u) c = 0.0f;
-                }
-                xv[t] = c;
-            }
+            if (in.eval)
+                gather16<true>(xv, in, col, len, g, cnt, i);
+            else
+                gather16<false>(xv, in, col, len, g, cnt, i);
             const uint32_t oc = o;
             if (base + kWave < e) entry_rec(cp, base + kWave + lane, e, o, col, len);
             float a0[NH], a1[NH];
@@ -182,14 +201,15 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
 #pragma unroll
     for (int m = 0; m < NH; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    StepIn in{eval, X + rhs, nullptr, (size_t)d.nF, R, d.nSp, d.nF, 0, rhs < R};
+    const int rc = rhs < R ? rhs : 0;                      // columns past R read column 0 (zeroed)
+    StepIn in{eval, X + rc, nullptr, (size_t)d.nF, R, d.nSp, d.nF, 0, rhs < R};
     mm_comp_A<NH, NT>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], in, d.nIC, d.wmr, lut_rf, lane, soff);
     if (d.nEC > 0) {
-        StepIn ie{nullptr, X + (size_t)d.nIC * d.nF * R + rhs, nullptr, (size_t)d.nE, R, d.nSp, d.nF, 0, rhs < R};
+        StepIn ie{nullptr, X + (size_t)d.nIC * d.nF * R + rc, nullptr, (size_t)d.nE, R, d.nSp, d.nF, 0, rhs < R};
         mm_comp_A<NH, NT>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], ie, d.nEC, d.wmh, lut_rf, lane, soff);
     }
     if (d.nISO > 0) {
-        StepIn is{nullptr, X + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R + rhs, nullptr, (size_t)d.nI, R,
+        StepIn is{nullptr, X + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R + rc, nullptr, (size_t)d.nI, R,
                   0, d.nF, 0, rhs < R};
         mm_comp_A<NH, NT>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], is, d.nISO, d.isol, (size_t)d.nSp, lane, soff);
     }
@@ -235,11 +255,14 @@ __device__ __forceinline__ void at_request(const Comp &cp, uint32_t e0, uint32_t
 // length (IC) or the x column (EC/ISO), fetched with the tile's LUT rows.
 __device__ __forceinline__ void at_aux(const Comp &cp, uint32_t e0, uint32_t e, int g, uint32_t (&aux)[4])
 {
+    // e0 < e here; rows past e read entry e-1 and are zeroed, so the 4 loads
+    // are unconditional and in flight together.
+    const uint32_t *src = cp.len ? reinterpret_cast<const uint32_t *>(cp.len) : cp.col;
+    uint32_t v[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint32_t eo = e0 + 4 * g + k;
-        aux[k] = eo < e ? (cp.len ? __float_as_uint(cp.len[eo]) : cp.col[eo]) : 0u;
-    }
+    for (int k = 0; k < 4; k++) v[k] = src[min(e0 + 4 * g + k, e - 1)];
+#pragma unroll
+    for (int k = 0; k < 4; k++) aux[k] = e0 + 4 * g + k < e ? v[k] : 0u;
 }
 
 // Write the D tile of entries e0.. (deferred by one tile, so that the wait for
@@ -322,18 +345,30 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
     const int g = lane >> 4, q = lane & 15;
     const int rhs = rg * kTileR + q;
     const bool rhs_ok = rhs < R;
+    // B operand: samples past nS and RHS columns past R read sample 0 / column
+    // 0 of this voxel and are zeroed, so the loads are unconditional and in
+    // flight together.
     float yb[4 * NT];
     const size_t row0 = (size_t)v * d.nS;
+    const int rc = rhs_ok ? rhs : 0;
 #pragma unroll
     for (int t = 0; t < 4 * NT; t++) {
         const int s = g * 4 * NT + t;
-        float yv = 0.0f;
-        if (s < d.nS && rhs_ok) {
-            yv = Y[(row0 + s) * R + rhs];
-            if (sqrtw) yv *= sqrtw[row0 + s];
-        }
-        yb[t] = yv;
+        yb[t] = Y[(row0 + (s < d.nS ? s : 0)) * R + rc];
     }
+    if (sqrtw) {
+        float wv[4 * NT];
+#pragma unroll
+        for (int t = 0; t < 4 * NT; t++) {
+            const int s = g * 4 * NT + t;
+            wv[t] = sqrtw[row0 + (s < d.nS ? s : 0)];
+        }
+#pragma unroll
+        for (int t = 0; t < 4 * NT; t++) yb[t] *= wv[t];
+    }
+#pragma unroll
+    for (int t = 0; t < 4 * NT; t++)
+        if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
     float *Xr = X + rhs;
     mm_comp_At<NT>(yb, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], d.nIC, d.wmr, lut_rf, d.nSp, lane, z, (uint32_t)d.n,
