@@ -148,7 +148,12 @@ def main():
     ms_At2 = time_op(lambda: at_pass(2), reps)
     bytes_pass = algorithmic_bytes(D, K)
     nj = (int(K["wmr"].shape[2]) + 63) // 64
-    kernels = {f"k_A<{nj}, double>": ms_A, f"k_At<{nj}, double>": ms_At1}
+    slab = tuned["slab"]["choice"]
+    wpb = 8 if slab == 1 else 16
+    name_A = ("k_A_dense<double>" if tuned["a_dense"]["choice"] else
+              f"k_A_slab<{nj}, 8, {wpb}, double>" if slab else f"k_A<{nj}, double>")
+    name_At = f"k_At_slab<{nj}, 4, {wpb}, double>" if slab else f"k_At<{nj}, double>"
+    kernels = {name_A: ms_A, name_At: ms_At1}
     dom_name = max(kernels, key=kernels.get)
     dom_ms = kernels[dom_name]
     dom_gbs = bytes_pass / (dom_ms * 1e-3) / 1e9

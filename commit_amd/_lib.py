@@ -16,6 +16,7 @@ FP32 = 1
 OPT_XCD_REMAP = 1
 OPT_MULTI_A_WAVES = 2
 OPT_A_DENSE = 3
+OPT_SLAB = 4
 
 # every symbol include/commit_hip.h declares, with its ctypes signature
 _P = ctypes.c_void_p

@@ -9,6 +9,10 @@ the fastest in the plan:
                          timed on the single-vector A x + A^T y pair;
 * ``a_dense``         -- A x per segment (0) or per tile of 16 voxels as a
                          dense FP64 matrix-core product (1); timed with it;
+* ``slab``            -- IC rows of A x and A^T y from L2 per entry (0), or the
+                         IC table staged in LDS by orientation slabs with
+                         work-groups of 8 (1) or 16 (2) wavefronts; timed with
+                         them;
 * ``multi_a_waves``   -- one or two wavefronts per (voxel, 16 right-hand
                          sides) in the multi-RHS A X; timed on A X.
 
@@ -61,8 +65,8 @@ def tune(A, nrhs=16, reps=3, cache=None, multi=True):
         with open(cache) as f:
             known = json.load(f)
         if key in known:
-            for name, opt in (("xcd_remap", _lib.OPT_XCD_REMAP), ("a_dense", _lib.OPT_A_DENSE),
-                              ("multi_a_waves", _lib.OPT_MULTI_A_WAVES)):
+            for name, opt in (("slab", _lib.OPT_SLAB), ("xcd_remap", _lib.OPT_XCD_REMAP),
+                              ("a_dense", _lib.OPT_A_DENSE), ("multi_a_waves", _lib.OPT_MULTI_A_WAVES)):
                 if name in known[key]:
                     set_option(A, opt, known[key][name]["choice"])
             return known[key]
@@ -79,17 +83,21 @@ def tune(A, nrhs=16, reps=3, cache=None, multi=True):
         At.matvec(y, out=xo)
 
     ms = {}
-    for remap in (1, 0):
-        for dense in (0, 1):
-            set_option(A, _lib.OPT_XCD_REMAP, remap)
-            set_option(A, _lib.OPT_A_DENSE, dense)
-            ms[(remap, dense)] = _time(pair, reps, dev)
+    for slab in (0, 1, 2):
+        set_option(A, _lib.OPT_SLAB, slab)
+        for remap in (1, 0):
+            for dense in (0, 1):
+                set_option(A, _lib.OPT_XCD_REMAP, remap)
+                set_option(A, _lib.OPT_A_DENSE, dense)
+                ms[(remap, dense, slab)] = _time(pair, reps, dev)
     best = min(ms, key=ms.get)
+    set_option(A, _lib.OPT_SLAB, best[2])
     set_option(A, _lib.OPT_XCD_REMAP, best[0])
     set_option(A, _lib.OPT_A_DENSE, best[1])
-    report["xcd_remap"] = {"choice": best[0], "ms": {f"remap={k[0]},dense={k[1]}": round(t, 4)
+    report["xcd_remap"] = {"choice": best[0], "ms": {f"remap={k[0]},dense={k[1]},slab={k[2]}": round(t, 4)
                                                      for k, t in ms.items()}}
     report["a_dense"] = {"choice": best[1]}
+    report["slab"] = {"choice": best[2]}
     del x, y, xo
     if multi and nrhs > 0:
         X = torch.rand((A.shape[1], nrhs), dtype=torch.float32, device=dev, generator=g)

@@ -45,6 +45,7 @@
 #include "../../include/commit_hip.h"
 #include "common.h"
 #include "plan.h"
+#include "rows.h"
 
 namespace commit_internal {
 thread_local std::string g_err;
@@ -73,100 +74,6 @@ using commit_internal::DevDict;
 // Voxel of this wavefront (one wavefront per voxel).
 __device__ __forceinline__ int voxel_id(const DevDict &d) { return commit_internal::wave_index(d.xcd_remap); }
 
-__device__ __forceinline__ double readlane_f64(double v, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-template <typename T>
-__device__ __forceinline__ T stream_load(const T *p)
-{
-    return __builtin_nontemporal_load(p);    // read once per product: keep L2 for LUT rows and x
-}
-
-// NJ coalesced loads of one padded LUT row (row points at this lane's sample
-// 0).  tail: this lane's sample of the last chunk is < nS; lanes past nS skip
-// that load (their padded entries are zero), so the padding costs no traffic.
-template <int NJ>
-__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row, bool tail)
-{
-#pragma unroll
-    for (int j = 0; j < NJ - 1; j++) raw[j] = row[kWave * j];
-    raw[NJ - 1] = tail ? row[kWave * (NJ - 1)] : 0.0f;
-}
-
-// next set bit of m (consumed), or -1
-__device__ __forceinline__ int take(unsigned long long &m)
-{
-    if (!m) return -1;
-    const int t = __builtin_ctzll(m);
-    m &= m - 1;
-    return t;
-}
-
-// ---------------------------------------------------------------- y = A x ---
-template <int NJ>
-__device__ __forceinline__ void fma_row(double (&acc)[NJ], double c, const float (&raw)[NJ])
-{
-#pragma unroll
-    for (int j = 0; j < NJ; j++) acc[j] = fma(c, (double)raw[j], acc[j]);
-}
-
-// Accumulate the entries [b, e) of one compartment into acc.
-//   xb: x (or x*eval) block of RF 0, xs: stride between RF blocks;
-//   lut: padded RF 0 table, lut_rf: stride between RF tables.
-template <int NJ, typename T>
-__device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32_t b, uint32_t e,
-                                       const T *__restrict__ xb, size_t xs, int nrf,
-                                       const float *__restrict__ lut, size_t lut_rf, int nSp, int lane, bool tail)
-{
-    for (int i = 0; i < nrf; i++) {
-        const float *L = lut + (size_t)i * lut_rf + lane;
-        for (uint32_t base = b; base < e; base += kWave) {
-            const uint32_t k = base + lane;
-            double c = 0.0;
-            uint32_t o = 0;
-            if (k < e) {
-                if (cp.o) o = stream_load(cp.o + k);
-                c = (double)xb[(size_t)stream_load(cp.col + k) + (size_t)i * xs];
-                // IC coefficient l * x * eval                      (setup_operator.py:63-96)
-                if (cp.len) c *= (double)stream_load(cp.len + k);
-            }
-            // zero coefficients are skipped                      (setup_operator.py:88)
-            unsigned long long m = __ballot(c != 0.0);
-            float ra[NJ], rb[NJ], rc[NJ];
-            int ta = take(m), tb = -1, tc = -1;
-            if (ta >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
-            if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
-            // three rows rotate through the pipeline; entries keep their order
-            while (true) {
-                if (ta < 0) break;
-                if ((tc = take(m)) >= 0) fetch_row<NJ>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp, tail);
-                fma_row<NJ>(acc, readlane_f64(c, ta), ra);
-                if (tb < 0) break;
-                if ((ta = take(m)) >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
-                fma_row<NJ>(acc, readlane_f64(c, tb), rb);
-                if (tc < 0) break;
-                if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
-                fma_row<NJ>(acc, readlane_f64(c, tc), rc);
-            }
-        }
-    }
-}
-
-// Output stage of A: y = A x, or the FISTA residual r = sqrtw .* (A x - yobs)
-// with a per-voxel partial of ||r||^2 (reduced later in voxel order).
-template <typename T>
-struct AOut {
-    T *y;
-    const T *yobs;        // null: plain product
-    const T *sqrtw;       // null: unweighted
-    double *rpart;        // [nVOX] partial sums of r^2, or null
-};
-
 // xic: the IC block of x, or x*eval when eval is not all ones.
 // T: storage type of x and y (double, or float for the FP32 mode); the
 // accumulation is FP64 in both (exact products of the FP32 LUT entries).
@@ -189,24 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_A(DevDict d, const T *__restrict__ x
     if (d.nISO > 0)
         a_comp<NJ>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE,
                    (size_t)d.nI, d.nISO, d.isol, (size_t)d.nSp, 0, lane, tail);
-    const size_t row = (size_t)v * d.nS;
-    double sq = 0.0;
-#pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const int s = lane + kWave * j;
-        if (s < d.nS) {
-            double r = acc[j];
-            if (out.yobs) r -= (double)stream_load(out.yobs + row + s);      // (solvers.py:308)
-            if (out.sqrtw) r *= (double)stream_load(out.sqrtw + row + s);    // (solvers.py:306)
-            __builtin_nontemporal_store((T)r, out.y + row + s);
-            sq = fma(r, r, sq);
-        }
-    }
-    if (out.rpart) {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
-        if (lane == 0) out.rpart[v] = sq;
-    }
+    a_store<NJ, T>(acc, d, v, out, lane);
 }
 
 // x*eval for the IC block (only when eval is not all ones)
@@ -468,6 +358,10 @@ int launch_A(commit_hip_plan *p, const T *x, AOut<T> out, hipStream_t st)
         const int rc = commit_internal::launch_A_dense<T>(p, x, xic, out.y, out.yobs, out.sqrtw, out.rpart, st);
         if (rc != COMMIT_HIP_EUNSUPPORTED) return rc;
     }
+    if (p->slab) {
+        const int rc = commit_internal::launch_A_slab<T>(p, x, xic, out, st);
+        if (rc != COMMIT_HIP_EUNSUPPORTED) return rc;
+    }
     switch (d.nSp / kWave) {
 #define CASE(NJ) \
     case NJ: hipLaunchKernelGGL((k_A<NJ, T>), dim3(grid), dim3(kBlock), 0, st, d, x, xic, out); break;
@@ -484,7 +378,12 @@ int launch_At(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t 
 {
     const DevDict &d = p->d;
     const unsigned grid = vox_blocks(d);
-    if (grid > 0 && (pass & 1)) {
+    int slab_rc = COMMIT_HIP_EUNSUPPORTED;
+    if (grid > 0 && (pass & 1) && p->slab) {
+        slab_rc = commit_internal::launch_At_slab<T>(p, y, sqrtw, x, st);
+        if (slab_rc && slab_rc != COMMIT_HIP_EUNSUPPORTED) return slab_rc;
+    }
+    if (grid > 0 && (pass & 1) && slab_rc == COMMIT_HIP_EUNSUPPORTED) {
         switch (d.nSp / kWave) {
 #define CASE(NJ) \
     case NJ: hipLaunchKernelGGL((k_At<NJ, T>), dim3(grid), dim3(kBlock), 0, st, d, y, sqrtw, p->zbuf, x); break;
@@ -503,13 +402,26 @@ int launch_At(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t 
     return 0;
 }
 
-// Voxel-major CSR of one compartment: entries stably sorted by voxel.
+// Voxel-major CSR of one compartment: entries stably sorted by voxel and, with
+// by_o, by orientation inside each voxel (the LDS-slab kernels need a voxel's
+// entries of one orientation slab to be contiguous; every other kernel is
+// indifferent to the order inside a voxel up to rounding).
 // col_in null: the column is the entry's input position.
 int build_comp(commit_hip_plan *p, const uint32_t *vox, size_t m, uint32_t nVOX, const uint16_t *o,
-               const uint32_t *col_in, const float *len, Comp &c, std::vector<uint32_t> *perm_out)
+               const uint32_t *col_in, const float *len, Comp &c, std::vector<uint32_t> *perm_out, bool by_o = false)
 {
     std::vector<uint32_t> perm, ptr;
-    if (!counting_sort(vox, m, nVOX, perm, ptr)) return fail(COMMIT_HIP_EINVAL, "voxel index >= nVOX=%u", nVOX);
+    if (by_o && o) {                                  // LSD: orientation, then voxel (both stable)
+        std::vector<uint32_t> key(m), p1, p2, ptr1;
+        for (size_t k = 0; k < m; k++) key[k] = o[k];
+        counting_sort(key.data(), m, 65536u, p1, ptr1);
+        for (size_t k = 0; k < m; k++) key[k] = vox[p1[k]];
+        if (!counting_sort(key.data(), m, nVOX, p2, ptr)) return fail(COMMIT_HIP_EINVAL, "voxel index >= nVOX=%u", nVOX);
+        perm.resize(m);
+        for (size_t k = 0; k < m; k++) perm[k] = p1[p2[k]];
+    } else if (!counting_sort(vox, m, nVOX, perm, ptr)) {
+        return fail(COMMIT_HIP_EINVAL, "voxel index >= nVOX=%u", nVOX);
+    }
     std::vector<uint32_t> col(m);
     for (size_t k = 0; k < m; k++) col[k] = col_in ? col_in[perm[k]] : perm[k];
     int rc;
@@ -604,7 +516,7 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
             zero_o.assign((size_t)n, 0);
             o = zero_o.data();
         }
-        if ((rc = build_comp(p, ICv, (size_t)n, (uint32_t)nVOX, o, ICf, ICl, d.ic, &perm))) goto done;
+        if ((rc = build_comp(p, ICv, (size_t)n, (uint32_t)nVOX, o, ICf, ICl, d.ic, &perm, true))) goto done;
         // fibre-major permutation of the voxel-sorted entries
         std::vector<uint32_t> f(n), fperm, fptr;
         for (int k = 0; k < n; k++) f[k] = ICf[perm[k]];
@@ -732,6 +644,11 @@ int commit_hip_plan_set_option(commit_hip_plan *plan, int option, int value)
             if (value != 0 && value != 1) return fail(COMMIT_HIP_EINVAL, "dense A is 0 or 1");
             plan->a_dense = value;
             return 0;
+        case COMMIT_HIP_OPT_SLAB:
+            if (value < 0 || value > 2) return fail(COMMIT_HIP_EINVAL, "slab mode is 0, 1 or 2");
+            if (value == plan->slab) return 0;
+            HIP_TRY(hipSetDevice(plan->device));
+            return commit_internal::slab_configure(plan, value);
         default: return fail(COMMIT_HIP_EINVAL, "unknown option %d", option);
     }
 }
@@ -743,6 +660,7 @@ int commit_hip_plan_get_option(const commit_hip_plan *plan, int option)
         case COMMIT_HIP_OPT_XCD_REMAP: return plan->d.xcd_remap;
         case COMMIT_HIP_OPT_MULTI_A_WAVES: return plan->multi_a_waves;
         case COMMIT_HIP_OPT_A_DENSE: return plan->a_dense;
+        case COMMIT_HIP_OPT_SLAB: return plan->slab;
         default: return fail(COMMIT_HIP_EINVAL, "unknown option %d", option);
     }
 }

@@ -44,6 +44,16 @@ __device__ __forceinline__ int wave_index(int remap)
     return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (int)(threadIdx.x >> 6));
 }
 
+// Output stage of A: y = A x, or the FISTA residual r = sqrtw .* (A x - yobs)
+// with a per-voxel partial of ||r||^2 (reduced later in voxel order).
+template <typename T>
+struct AOut {
+    T *y;
+    const T *yobs;        // null: plain product
+    const T *sqrtw;       // null: unweighted
+    double *rpart;        // [nVOX] partial sums of r^2, or null
+};
+
 }  // namespace commit_internal
 
 struct commit_hip_plan {
@@ -62,12 +72,43 @@ struct commit_hip_plan {
     bool eval_ones = true;
     int multi_a_waves = 2;     // wavefronts per (voxel, 16 RHS) in the multi-RHS A x (1 or 2)
     int a_dense = 0;           // 1: A x by the dense-tile matrix-core kernel (dense.hip)
+    // LDS-slab kernels (slab.hip): 0 off, 1: 8 wavefronts x 80 KiB per work-group,
+    // 2: 16 wavefronts x 160 KiB; S orientations per slab, nsl slabs
+    int slab = 0;
+    int slab_S = 0, slab_nsl = 0;
+    unsigned slab_blocks = 0;
+    unsigned slab_blocks_at = 0;
+    int32_t *slab_vlist = nullptr;     // A x:   [slab_blocks][W][kSlabVA] voxel ids (-1: none)
+    int32_t *slab_vlist_at = nullptr;  // A^T y: [slab_blocks_at][W][kSlabVAt]
+    uint32_t *slab_ofs = nullptr;      // [nVOX][nsl+1] first IC entry of each slab
     int64_t bytes = 0;
 };
 
 namespace commit_internal {
 // hipMalloc tracked by the plan (freed by commit_hip_plan_destroy)
 int dev_alloc(commit_hip_plan *p, size_t bytes, void **out);
+
+// LDS-slab kernels (slab.hip): voxels per wavefront (A x holds NJ FP64
+// accumulators per voxel, A^T y an FP64 y row) and the largest NJ.
+constexpr int kSlabVA = 8;
+constexpr int kSlabVAt = 4;
+constexpr int kSlabMaxNJ = 5;
+
+struct SlabArgs {
+    const uint32_t *ofs;    // [nVOX][nsl+1]
+    const int32_t *vlist;   // [blocks][W][V]
+    int S, nsl;
+};
+
+// (Re)build the slab tables for mode 0 (off), 1 or 2; the plan's IC entries
+// must be sorted by orientation inside each voxel.
+int slab_configure(commit_hip_plan *p, int mode);
+void slab_release(commit_hip_plan *p);
+// COMMIT_HIP_EUNSUPPORTED when slabs are off or the sizes are outside the build.
+template <typename T>
+int launch_A_slab(commit_hip_plan *p, const T *x, const T *xic, const AOut<T> &out, hipStream_t st);
+template <typename T>
+int launch_At_slab(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t st);
 
 // Dense-tile A x (dense.hip); COMMIT_HIP_EUNSUPPORTED when the plan's sizes
 // are outside what it is built for.

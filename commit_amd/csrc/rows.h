@@ -1,0 +1,202 @@
+// Device helpers shared by the per-voxel kernels (commit_hip.hip) and the
+// LDS-slab kernels (slab.hip): LUT row fetches, the A x entry pipeline, the
+// output stage of A and lane reductions built from VALU cross-lane moves.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.h"
+#include "plan.h"
+
+namespace {
+
+using commit_internal::AOut;
+using commit_internal::Comp;
+using commit_internal::DevDict;
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename T>
+__device__ __forceinline__ T stream_load(const T *p)
+{
+    return __builtin_nontemporal_load(p);    // read once per product: keep L2 for LUT rows and x
+}
+
+// NJ coalesced loads of one padded LUT row (row points at this lane's sample
+// 0).  tail: this lane's sample of the last chunk is < nS; lanes past nS skip
+// that load (their padded entries are zero), so the padding costs no traffic.
+template <int NJ>
+__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row, bool tail)
+{
+#pragma unroll
+    for (int j = 0; j < NJ - 1; j++) raw[j] = row[kWave * j];
+    raw[NJ - 1] = tail ? row[kWave * (NJ - 1)] : 0.0f;
+}
+
+// next set bit of m (consumed), or -1
+__device__ __forceinline__ int take(unsigned long long &m)
+{
+    if (!m) return -1;
+    const int t = __builtin_ctzll(m);
+    m &= m - 1;
+    return t;
+}
+
+// ---------------------------------------------------------------- y = A x ---
+template <int NJ>
+__device__ __forceinline__ void fma_row(double (&acc)[NJ], double c, const float (&raw)[NJ])
+{
+#pragma unroll
+    for (int j = 0; j < NJ; j++) acc[j] = fma(c, (double)raw[j], acc[j]);
+}
+
+// Accumulate the entries [b, e) of one compartment into acc.
+//   xb: x (or x*eval) block of RF 0, xs: stride between RF blocks;
+//   lut: padded RF 0 table, lut_rf: stride between RF tables.
+template <int NJ, typename T>
+__device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32_t b, uint32_t e,
+                                       const T *__restrict__ xb, size_t xs, int nrf,
+                                       const float *__restrict__ lut, size_t lut_rf, int nSp, int lane, bool tail)
+{
+    for (int i = 0; i < nrf; i++) {
+        const float *L = lut + (size_t)i * lut_rf + lane;
+        for (uint32_t base = b; base < e; base += kWave) {
+            const uint32_t k = base + lane;
+            double c = 0.0;
+            uint32_t o = 0;
+            if (k < e) {
+                if (cp.o) o = stream_load(cp.o + k);
+                c = (double)xb[(size_t)stream_load(cp.col + k) + (size_t)i * xs];
+                // IC coefficient l * x * eval                      (setup_operator.py:63-96)
+                if (cp.len) c *= (double)stream_load(cp.len + k);
+            }
+            // zero coefficients are skipped                      (setup_operator.py:88)
+            unsigned long long m = __ballot(c != 0.0);
+            float ra[NJ], rb[NJ], rc[NJ];
+            int ta = take(m), tb = -1, tc = -1;
+            if (ta >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
+            if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
+            // three rows rotate through the pipeline; entries keep their order
+            while (true) {
+                if (ta < 0) break;
+                if ((tc = take(m)) >= 0) fetch_row<NJ>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp, tail);
+                fma_row<NJ>(acc, readlane_f64(c, ta), ra);
+                if (tb < 0) break;
+                if ((ta = take(m)) >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
+                fma_row<NJ>(acc, readlane_f64(c, tb), rb);
+                if (tc < 0) break;
+                if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
+                fma_row<NJ>(acc, readlane_f64(c, tc), rc);
+            }
+        }
+    }
+}
+
+// Output stage of A for voxel v: y row (or the residual sqrtw .* (A x - yobs))
+// and the voxel's partial of ||r||^2.                  (solvers.py:306-308)
+template <int NJ, typename T>
+__device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &d, int v, const AOut<T> &out, int lane)
+{
+    const size_t row = (size_t)v * d.nS;
+    double sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int s = lane + kWave * j;
+        if (s < d.nS) {
+            double r = acc[j];
+            if (out.yobs) r -= (double)stream_load(out.yobs + row + s);
+            if (out.sqrtw) r *= (double)stream_load(out.sqrtw + row + s);
+            __builtin_nontemporal_store((T)r, out.y + row + s);
+            sq = fma(r, r, sq);
+        }
+    }
+    if (out.rpart) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
+        if (lane == 0) out.rpart[v] = sq;
+    }
+}
+
+// ---- cross-lane moves of doubles on the VALU (no LDS traffic) --------------
+// DPP controls: quad_perm [1,0,3,2] and [2,3,0,1], row_mirror (lane i <- 15-i
+// inside each row of 16), row_half_mirror (lane i <- 7-i inside each 8).
+constexpr int kDppQuadXor1 = 0xB1;
+constexpr int kDppQuadXor2 = 0x4E;
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppHalfMirror = 0x141;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// v_permlane32_swap on a pair of doubles (a, b): lanes 32-63 of a trade places
+// with lanes 0-31 of b; returns a' + b', i.e. lanes 0-31: a(l) + a(l+32),
+// lanes 32-63: b(l-32) + b(l).
+__device__ __forceinline__ double swap32_add(double a, double b)
+{
+    const long long ab = __double_as_longlong(a), bb = __double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)(ab & 0xffffffffll), (unsigned)(bb & 0xffffffffll),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ab >> 32), (unsigned)(bb >> 32), false, false);
+    const double a2 = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+    const double b2 = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+    return a2 + b2;
+}
+
+// v_permlane16_swap likewise on rows of 16 lanes: odd rows of a trade places
+// with even rows of b; returns a' + b' (even rows: a(l) + a(l+16), odd rows:
+// b(l-16) + b(l)).
+__device__ __forceinline__ double swap16_add(double a, double b)
+{
+    const long long ab = __double_as_longlong(a), bb = __double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)(ab & 0xffffffffll), (unsigned)(bb & 0xffffffffll),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ab >> 32), (unsigned)(bb >> 32), false, false);
+    const double a2 = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+    const double b2 = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+    return a2 + b2;
+}
+
+// Sum of 8 per-lane values over the 64 lanes, transposed: on return every lane
+// of the group of 8 lanes g = lane/8 holds the wave total of p[g].  Fixed
+// order, so bitwise repeatable.  30-odd VALU operations for 8 sums.
+__device__ __forceinline__ double reduce8(const double (&p)[8], int lane)
+{
+    double q[4], r[2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) q[t] = swap32_add(p[t], p[t + 4]);   // lanes 0-31: p[t], 32-63: p[t+4]
+#pragma unroll
+    for (int t = 0; t < 2; t++) r[t] = swap16_add(q[t], q[t + 2]);   // + rows 1,3: q[t+2]
+    const bool hi8 = (lane & 8) != 0;                                 // lanes 8-15 of each row: r[1]
+    const double keep = hi8 ? r[1] : r[0], send = hi8 ? r[0] : r[1];
+    double s = keep + dpp_f64<kDppRowMirror>(send);
+    s += dpp_f64<kDppQuadXor1>(s);
+    s += dpp_f64<kDppQuadXor2>(s);
+    s += dpp_f64<kDppHalfMirror>(s);
+    return s;
+}
+
+// Sum of one per-lane value over the 64 lanes, in every lane.
+__device__ __forceinline__ double wave_sum(double s)
+{
+    s += dpp_f64<kDppQuadXor1>(s);
+    s += dpp_f64<kDppQuadXor2>(s);
+    s += dpp_f64<kDppHalfMirror>(s);
+    s += dpp_f64<kDppRowMirror>(s);
+    s = swap16_add(s, s);
+    return swap32_add(s, s);
+}
+
+}  // namespace

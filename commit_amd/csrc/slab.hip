@@ -1,0 +1,550 @@
+// LDS-slab products: y = A x and pass 1 of x = A^T y with the IC table served
+// from LDS instead of L2.
+//
+// Reference algorithm: setup_operator.py:48-109 (IC part of COMMIT_A__block)
+// and :370-442 (IC part of COMMIT_At__block).
+//
+// The per-voxel kernels of commit_hip.hip read one padded LUT row (nSp floats)
+// per IC segment from L2; on the 1M-fibre head that is ~100 GB per product,
+// and the L2-to-CU path bounds them.  Here a work-group of W wavefronts owns
+// W*V voxels (V per wavefront, accumulators or y rows in registers) and walks
+// the IC table in slabs of S orientations: the slab is copied once into LDS
+// (global_load_lds, 16 bytes per lane), then every wavefront applies the
+// entries of its voxels whose orientation falls in the slab, reading rows from
+// LDS.  The plan keeps each voxel's IC entries sorted by orientation
+// (stable), so a voxel's entries of one slab are a contiguous range; their
+// offsets are a [nVOX][nsl+1] table.  Voxels are dealt to the wavefronts of a
+// work-group longest-first onto the least-loaded wavefront, so the wavefronts
+// reach each slab barrier together.  EC and ISO (a few entries per voxel) use
+// rows from global memory, as in commit_hip.hip.
+//
+//   A x   : per entry, NJ FP64 FMAs per lane from an LDS row; zero
+//           coefficients skipped with a ballot (setup_operator.py:88).
+//   A^T y : per entry, a per-lane partial dot product; 8 entries are reduced
+//           together over the 64 lanes with VALU cross-lane moves
+//           (permlane32/16 swaps and DPP), no LDS traffic.  The IC product
+//           goes to its fibre-major scratch slot, summed per fibre in
+//           ascending voxel order by pass 2 (k_At_fibre).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "../../include/commit_hip.h"
+#include "common.h"
+#include "plan.h"
+#include "rows.h"
+
+using commit_internal::fail;
+
+namespace {
+
+using commit_internal::SlabArgs;
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// LDS floats per work-group: 160 KiB shared by the 16 wavefronts of a CU.
+// The last W*kBcastFloats floats hold one broadcast table per wavefront; the
+// rest holds the slab (S rows of nSp floats).
+template <int W>
+struct SlabLds {
+    static constexpr int floats = 40960 * W / 16;
+};
+
+// Per-wavefront table of the current chunk: coefficients (A x) and row
+// offsets inside the slab, read back by the whole wavefront at one address
+// (an LDS broadcast) instead of lane reads on the VALU.
+struct Bcast {
+    double c[kWave];
+    int32_t off[kWave];
+};
+constexpr int kBcastFloats = (int)(sizeof(Bcast) / sizeof(float));
+static_assert(sizeof(Bcast) % 16 == 0, "broadcast tables must stay 16-byte aligned");
+
+template <int W>
+__device__ __forceinline__ Bcast *bcast_table(float *L, int w)
+{
+    return reinterpret_cast<Bcast *>(L + SlabLds<W>::floats - (w + 1) * kBcastFloats);
+}
+
+// Copy nfloats contiguous floats (a multiple of 64) into L: 1 KiB pieces,
+// wavefront w takes pieces w, w+W, ...; the caller waits and synchronises.
+template <int W>
+__device__ __forceinline__ void slab_load(float *L, const float *__restrict__ src, int nfloats, int w, int lane)
+{
+    const int npieces = (nfloats + 255) / 256;
+    for (int pc = w; pc < npieces; pc += W) {
+        const int f = pc * 256 + 4 * lane;
+        if (f < nfloats) __builtin_amdgcn_global_load_lds(src + f, (lds_void *)(L + pc * 256), 16, 0, 0);
+    }
+}
+
+template <int NJ>
+__device__ __forceinline__ void lds_row(float (&raw)[NJ], const float *R)
+{
+#pragma unroll
+    for (int j = 0; j < NJ; j++) raw[j] = R[kWave * j];
+}
+
+// Move each lane's value to lane dst (a permutation of the 64 lanes).
+__device__ __forceinline__ uint32_t lane_push(uint32_t v, int dst)
+{
+    return (uint32_t)__builtin_amdgcn_ds_permute(dst << 2, (int)v);
+}
+
+__device__ __forceinline__ double lane_push_f64(double v, int dst)
+{
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = lane_push((uint32_t)(b & 0xffffffffll), dst);
+    const uint32_t hi = lane_push((uint32_t)(b >> 32), dst);
+    return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+// Entry records of the first chunk (up to 64 entries) of a range [b, e): one
+// per lane, clamped to a valid entry (b, or entry 0 for an empty range).
+struct Rec {
+    uint32_t o, col;      // orientation; x column (A x) or fibre-major slot (A^T y)
+    float len;
+};
+
+__device__ __forceinline__ Rec rec_load(const Comp &cp, const uint32_t *__restrict__ colv, uint32_t b, uint32_t e,
+                                        int lane)
+{
+    const uint32_t k = b + lane;
+    const uint32_t ks = k < e ? k : (b < e ? b : 0u);
+    return Rec{stream_load(cp.o + ks), stream_load(colv + ks), stream_load(cp.len + ks)};
+}
+
+// A x: apply up to 64 entries of one voxel and one slab, rows in LDS (Lb: this
+// lane's sample 0 of the slab's first orientation o0); lane l holds entry b+l
+// (record r, x value xv), valid when ok.  The nonzero coefficients and their
+// row offsets go, in order, to slots 0..nz-1 of the wavefront's broadcast
+// table (zero coefficients are skipped, setup_operator.py:88); the other lanes
+// fill the slots past nz with zero coefficients and valid rows.  Entries are
+// then applied in groups of 4 with all four rows in flight; a group's tail
+// adds exact zeros.
+template <int NJ, typename T>
+__device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T xv, bool ok, const float *Lb,
+                                             Bcast *bt, int o0, int nSp, int lane)
+{
+    // IC coefficient l * x * eval                                  (setup_operator.py:63-96)
+    double c = (double)xv;
+    c *= (double)r.len;
+    if (!ok) c = 0.0;
+    const bool nzl = c != 0.0;
+    const unsigned long long m = __ballot(nzl);
+    const int nz = __popcll(m);
+    if (nz == 0) return;
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const int dst = nzl ? below : nz + (lane - below);
+    bt->c[dst] = c;
+    bt->off[dst] = ((int)r.o - o0) * nSp;
+    for (int t = 0; t < nz; t += 4) {
+        const int4 of = *reinterpret_cast<const int4 *>(&bt->off[t]);
+        const double2 c01 = *reinterpret_cast<const double2 *>(&bt->c[t]);
+        const double2 c23 = *reinterpret_cast<const double2 *>(&bt->c[t + 2]);
+        float ra[NJ], rb[NJ], rc[NJ], rd[NJ];
+        lds_row<NJ>(ra, Lb + of.x);
+        lds_row<NJ>(rb, Lb + of.y);
+        lds_row<NJ>(rc, Lb + of.z);
+        lds_row<NJ>(rd, Lb + of.w);
+        fma_row<NJ>(acc, c01.x, ra);
+        fma_row<NJ>(acc, c01.y, rb);
+        fma_row<NJ>(acc, c23.x, rc);
+        fma_row<NJ>(acc, c23.y, rd);
+    }
+}
+
+// One slab pass of A x over the wavefront's V voxels (ranges rb[u]..re[u]).
+// The first chunk of every voxel is software-pipelined: the entry records of
+// voxel u+2 and the x gathers of voxel u+1 are in flight while voxel u is
+// applied.  Later chunks of a long range are loaded in place.
+template <int NJ, int V, typename T>
+__device__ __forceinline__ void slab_a_pass(double (&acc)[V][NJ], const Comp &cp, const uint32_t (&rb)[V],
+                                            const uint32_t (&re)[V], const T *__restrict__ xb, const float *Lb,
+                                            Bcast *bt, int o0, int nSp, int lane, Rec r0, Rec r1)
+{
+    T x0 = xb[r0.col];
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        Rec r2 = r1;
+        if (u + 2 < V) r2 = rec_load(cp, cp.col, rb[u + 2 < V ? u + 2 : 0], re[u + 2 < V ? u + 2 : 0], lane);
+        T x1 = x0;
+        if (u + 1 < V) x1 = xb[r1.col];
+        if (rb[u] < re[u]) {
+            slab_a_chunk<NJ, T>(acc[u], r0, x0, rb[u] + lane < re[u], Lb, bt, o0, nSp, lane);
+            for (uint32_t base = rb[u] + kWave; base < re[u]; base += kWave) {
+                const Rec r = rec_load(cp, cp.col, base, re[u], lane);
+                slab_a_chunk<NJ, T>(acc[u], r, xb[r.col], base + lane < re[u], Lb, bt, o0, nSp, lane);
+            }
+        }
+        r0 = r1;
+        r1 = r2;
+        x0 = x1;
+    }
+}
+
+template <int NJ, int V, int W, typename T>
+__global__ __launch_bounds__(W * kWave, 4) void k_A_slab(DevDict d, const T *__restrict__ x, const T *__restrict__ xic,
+                                                     AOut<T> out, SlabArgs sa)
+{
+    __shared__ __align__(16) float L[SlabLds<W>::floats];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    Bcast *bt = bcast_table<W>(L, w);
+    const int32_t *vl = sa.vlist + ((size_t)blockIdx.x * W + w) * V;
+    int vox[V];
+#pragma unroll
+    for (int u = 0; u < V; u++) vox[u] = __builtin_amdgcn_readfirstlane(vl[u]);
+    double acc[V][NJ];
+#pragma unroll
+    for (int u = 0; u < V; u++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) acc[u][j] = 0.0;
+    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
+    for (int i = 0; i < d.nIC; i++) {
+        const T *xb = xic + (size_t)i * d.nF;
+        for (int s = 0; s < sa.nsl; s++) {
+            const int o0 = s * sa.S, rows = min(sa.S, d.ndirs - o0);
+            uint32_t rb[V], re[V];
+#pragma unroll
+            for (int u = 0; u < V; u++) {
+                const uint32_t *of = sa.ofs + (size_t)(vox[u] < 0 ? 0 : vox[u]) * (sa.nsl + 1) + s;
+                rb[u] = vox[u] < 0 ? 0u : of[0];
+                re[u] = vox[u] < 0 ? 0u : of[1];
+            }
+            // records of the first two voxels load while the slab is copied
+            const Rec r0 = rec_load(d.ic, d.ic.col, rb[0], re[0], lane);
+            const Rec r1 = rec_load(d.ic, d.ic.col, rb[V > 1 ? 1 : 0], re[V > 1 ? 1 : 0], lane);
+            __syncthreads();                                         // previous slab consumed
+            slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows * d.nSp, w, lane);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();                                         // slab in LDS
+            slab_a_pass<NJ, V, T>(acc, d.ic, rb, re, xb, L + lane, bt, o0, d.nSp, lane, r0, r1);
+        }
+    }
+    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const int v = vox[u];
+        if (v < 0) continue;
+        if (d.nEC > 0)
+            a_comp<NJ>(acc[u], d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
+                       d.wmh, lut_rf, d.nSp, lane, tail);
+        if (d.nISO > 0)
+            a_comp<NJ>(acc[u], d.iso, d.iso.ptr[v], d.iso.ptr[v + 1],
+                       x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
+                       (size_t)d.nSp, 0, lane, tail);
+        a_store<NJ, T>(acc[u], d, v, out, lane);
+    }
+}
+
+template <int NJ>
+__device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (&yv)[NJ])
+{
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) a = fma((double)raw[j], yv[j], a);
+    return a;
+}
+
+// A^T y pass 1: up to 64 IC entries of one voxel and one slab, lane l holding
+// record r of entry b+l, cnt of them valid; z[zpos] = l * dot.  The row
+// offsets go to the wavefront's broadcast table.  Groups of 8 entries:
+// per-lane partial dots, then reduce8 (entry t0+g ends in lanes 8g..8g+7;
+// lane 8g stores it).  Entries past cnt in the last group read valid rows and
+// are not stored.                                     (setup_operator.py:384-428)
+template <int NJ>
+__device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec &r, int cnt, double *__restrict__ zi,
+                                              const float *Lb, Bcast *bt, int o0, int nSp, int lane)
+{
+    bt->off[lane] = ((int)r.o - o0) * nSp;
+    for (int t0 = 0; t0 < cnt; t0 += 8) {
+        const int4 oa = *reinterpret_cast<const int4 *>(&bt->off[t0]);
+        const int4 ob = *reinterpret_cast<const int4 *>(&bt->off[t0 + 4]);
+        const int of[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
+        double p[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            float raw[NJ];
+            lds_row<NJ>(raw, Lb + of[q]);
+            p[q] = dot_row<NJ>(raw, yv);
+        }
+        const double sum = reduce8(p, lane);
+        const int te = t0 + (lane >> 3);
+        const float le = __shfl(r.len, te);
+        const uint32_t ze = (uint32_t)__shfl((int)r.col, te);
+        if ((lane & 7) == 0 && te < cnt) zi[ze] = (double)le * sum;
+    }
+}
+
+// EC / ISO entries [b, e) of one voxel, rows from global memory:
+// out[col + i*xs] = dot                                 (setup_operator.py:444-563)
+template <int NJ, typename T>
+__device__ __forceinline__ void glob_at(const double (&yv)[NJ], const Comp &cp, uint32_t b, uint32_t e,
+                                        T *__restrict__ out, size_t xs, int nrf, const float *__restrict__ lut,
+                                        size_t lut_rf, int nSp, int lane, bool tail)
+{
+    for (int i = 0; i < nrf; i++) {
+        const float *Li = lut + (size_t)i * lut_rf + lane;
+        for (uint32_t k = b; k < e; k++) {
+            const uint32_t o = cp.o ? (uint32_t)cp.o[k] : 0u;
+            float raw[NJ];
+            fetch_row<NJ>(raw, Li + (size_t)o * nSp, tail);
+            const double sum = wave_sum(dot_row<NJ>(raw, yv));
+            if (lane == 0) out[(size_t)cp.col[k] + (size_t)i * xs] = (T)sum;
+        }
+    }
+}
+
+template <int NJ, int V, int W, typename T>
+__global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__restrict__ y, const T *__restrict__ sqrtw,
+                                                      double *__restrict__ z, T *__restrict__ x, SlabArgs sa)
+{
+    __shared__ __align__(16) float L[SlabLds<W>::floats];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    Bcast *bt = bcast_table<W>(L, w);
+    const int32_t *vl = sa.vlist + ((size_t)blockIdx.x * W + w) * V;
+    int vox[V];
+#pragma unroll
+    for (int u = 0; u < V; u++) vox[u] = __builtin_amdgcn_readfirstlane(vl[u]);
+    // y rows of the wavefront's voxels (times sqrtw)          (solvers.py:267)
+    double yv[V][NJ];
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const size_t row = (size_t)(vox[u] < 0 ? 0 : vox[u]) * d.nS;
+        T raw[NJ], wv[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int s = lane + kWave * j;
+            raw[j] = stream_load(y + row + (s < d.nS ? s : 0));
+            wv[j] = sqrtw ? stream_load(sqrtw + row + (s < d.nS ? s : 0)) : (T)1;
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            double t = (double)raw[j];
+            if (sqrtw) t *= (double)wv[j];
+            yv[u][j] = (vox[u] >= 0 && lane + kWave * j < d.nS) ? t : 0.0;
+        }
+    }
+    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
+    for (int i = 0; i < d.nIC; i++) {
+        double *zi = z + (size_t)i * d.n;
+        for (int s = 0; s < sa.nsl; s++) {
+            const int o0 = s * sa.S, rows = min(sa.S, d.ndirs - o0);
+            uint32_t rb[V], re[V];
+#pragma unroll
+            for (int u = 0; u < V; u++) {
+                const uint32_t *of = sa.ofs + (size_t)(vox[u] < 0 ? 0 : vox[u]) * (sa.nsl + 1) + s;
+                rb[u] = vox[u] < 0 ? 0u : of[0];
+                re[u] = vox[u] < 0 ? 0u : of[1];
+            }
+            Rec r0 = rec_load(d.ic, d.zpos, rb[0], re[0], lane);       // loads while the slab is copied
+            __syncthreads();
+            slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows * d.nSp, w, lane);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            const float *Lb = L + lane;
+#pragma unroll
+            for (int u = 0; u < V; u++) {
+                Rec r1 = r0;                                            // next voxel's records in flight
+                if (u + 1 < V) r1 = rec_load(d.ic, d.zpos, rb[u + 1 < V ? u + 1 : 0], re[u + 1 < V ? u + 1 : 0], lane);
+                if (rb[u] < re[u]) {
+                    slab_at_chunk<NJ>(yv[u], r0, (int)min((uint32_t)kWave, re[u] - rb[u]), zi, Lb, bt, o0, d.nSp,
+                                      lane);
+                    for (uint32_t base = rb[u] + kWave; base < re[u]; base += kWave)
+                        slab_at_chunk<NJ>(yv[u], rec_load(d.ic, d.zpos, base, re[u], lane),
+                                          (int)min((uint32_t)kWave, re[u] - base), zi, Lb, bt, o0, d.nSp, lane);
+                }
+                r0 = r1;
+            }
+        }
+    }
+    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const int v = vox[u];
+        if (v < 0) continue;
+        if (d.nEC > 0)
+            glob_at<NJ>(yv[u], d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
+                        d.wmh, lut_rf, d.nSp, lane, tail);
+        if (d.nISO > 0)
+            glob_at<NJ>(yv[u], d.iso, d.iso.ptr[v], d.iso.ptr[v + 1],
+                        x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
+                        (size_t)d.nSp, 0, lane, tail);
+    }
+}
+
+// ofs[v*(nsl+1) + s] = first IC entry of voxel v with orientation >= s*S
+// (entries of a voxel are sorted by orientation); ofs[v*(nsl+1) + nsl] = end.
+__global__ void k_slab_ofs(const uint32_t *__restrict__ ptr, const uint16_t *__restrict__ o, int nVOX, int nsl,
+                           int S, uint32_t *__restrict__ ofs)
+{
+    const size_t total = (size_t)nVOX * (nsl + 1);
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t v = t / (nsl + 1);
+        const int s = (int)(t - v * (nsl + 1));
+        uint32_t lo = ptr[v], hi = ptr[v + 1];
+        if (s == nsl) {
+            ofs[t] = hi;
+            continue;
+        }
+        const uint32_t key = (uint32_t)(s * S);
+        while (lo < hi) {                           // lower bound of key in o[lo, hi)
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((uint32_t)o[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        ofs[t] = lo;
+    }
+}
+
+}  // namespace
+
+namespace commit_internal {
+
+namespace {
+
+// Voxels dealt to the W wavefronts of each work-group (W*V consecutive voxels
+// per group), longest first onto the least-loaded wavefront with room.
+int build_vlist(commit_hip_plan *p, const std::vector<uint32_t> &ptr, int W, int V, int32_t **out, unsigned *blocks)
+{
+    const int nVOX = p->d.nVOX, per = W * V;
+    const size_t nb = ((size_t)nVOX + per - 1) / per;
+    std::vector<int32_t> vl(nb * per, -1);
+    std::vector<int> order(per);
+    std::vector<uint64_t> load(W);
+    std::vector<int> used(W);
+    for (size_t b = 0; b < nb; b++) {
+        const int v0 = (int)(b * per), nv = std::min(per, nVOX - v0);
+        std::iota(order.begin(), order.begin() + nv, v0);
+        std::stable_sort(order.begin(), order.begin() + nv, [&](int a, int c) {
+            return ptr[a + 1] - ptr[a] > ptr[c + 1] - ptr[c];
+        });
+        std::fill(load.begin(), load.end(), 0);
+        std::fill(used.begin(), used.end(), 0);
+        for (int r = 0; r < nv; r++) {
+            int best = -1;
+            for (int w = 0; w < W; w++)
+                if (used[w] < V && (best < 0 || load[w] < load[best])) best = w;
+            const int v = order[r];
+            vl[b * per + (size_t)best * V + used[best]++] = v;
+            load[best] += ptr[v + 1] - ptr[v];
+        }
+    }
+    void *mem = nullptr;
+    const int rc = dev_alloc(p, vl.size() * sizeof(int32_t), &mem);
+    if (rc) return rc;
+    *out = static_cast<int32_t *>(mem);
+    HIP_TRY(hipMemcpy(mem, vl.data(), vl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    *blocks = (unsigned)nb;
+    return 0;
+}
+
+}  // namespace
+
+int slab_configure(commit_hip_plan *p, int mode)
+{
+    slab_release(p);
+    if (mode == 0) return 0;
+    const DevDict &d = p->d;
+    if (mode != 1 && mode != 2) return fail(COMMIT_HIP_EINVAL, "slab mode %d is not 0, 1 or 2", mode);
+    p->slab = mode;                                 // with no IC entries the launches fall back
+    if (d.n == 0 || d.nIC == 0 || d.nVOX == 0 || d.nSp / kWave > kSlabMaxNJ) return 0;
+    const int W = mode == 1 ? 8 : 16;
+    const int ldsf = (mode == 1 ? SlabLds<8>::floats : SlabLds<16>::floats);
+    const int S = std::min(d.ndirs, (ldsf - W * kBcastFloats) / d.nSp);
+    const int nsl = (d.ndirs + S - 1) / S;
+    std::vector<uint32_t> ptr((size_t)d.nVOX + 1);
+    HIP_TRY(hipMemcpy(ptr.data(), d.ic.ptr, ptr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int rc;
+    if ((rc = build_vlist(p, ptr, W, kSlabVA, &p->slab_vlist, &p->slab_blocks))) return rc;
+    if ((rc = build_vlist(p, ptr, W, kSlabVAt, &p->slab_vlist_at, &p->slab_blocks_at))) return rc;
+    const size_t nofs = (size_t)d.nVOX * (nsl + 1);
+    void *mem = nullptr;
+    if ((rc = dev_alloc(p, nofs * sizeof(uint32_t), &mem))) return rc;
+    p->slab_ofs = static_cast<uint32_t *>(mem);
+    const unsigned g = (unsigned)std::min<size_t>((nofs + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_slab_ofs, dim3(g), dim3(256), 0, 0, d.ic.ptr, d.ic.o, d.nVOX, nsl, S, p->slab_ofs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    p->slab_S = S;
+    p->slab_nsl = nsl;
+    return 0;
+}
+
+void slab_release(commit_hip_plan *p)
+{
+    for (void *q : {static_cast<void *>(p->slab_vlist), static_cast<void *>(p->slab_vlist_at),
+                    static_cast<void *>(p->slab_ofs)}) {
+        if (!q) continue;
+        for (size_t k = 0; k < p->allocs.size(); k++)
+            if (p->allocs[k] == q) {
+                p->allocs.erase(p->allocs.begin() + (long)k);
+                break;
+            }
+        (void)hipFree(q);
+    }
+    p->slab_vlist = nullptr;
+    p->slab_vlist_at = nullptr;
+    p->slab_ofs = nullptr;
+    p->slab = 0;
+    p->slab_blocks = p->slab_blocks_at = 0;
+}
+
+#define SLAB_SWITCH(KERNEL, kSlabV, VLIST, BLOCKS, ...)                                                           \
+    do {                                                                                                          \
+        const SlabArgs sa{p->slab_ofs, VLIST, p->slab_S, p->slab_nsl};                                            \
+        const dim3 grid(BLOCKS);                                                                                  \
+        const int NJv = d.nSp / kWave;                                                                            \
+        if (p->slab == 1) {                                                                                       \
+            switch (NJv) {                                                                                        \
+                case 1: hipLaunchKernelGGL((KERNEL<1, kSlabV, 8, T>), grid, dim3(8 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 2: hipLaunchKernelGGL((KERNEL<2, kSlabV, 8, T>), grid, dim3(8 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 3: hipLaunchKernelGGL((KERNEL<3, kSlabV, 8, T>), grid, dim3(8 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 4: hipLaunchKernelGGL((KERNEL<4, kSlabV, 8, T>), grid, dim3(8 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 5: hipLaunchKernelGGL((KERNEL<5, kSlabV, 8, T>), grid, dim3(8 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                default: return COMMIT_HIP_EUNSUPPORTED;                                                          \
+            }                                                                                                     \
+        } else {                                                                                                  \
+            switch (NJv) {                                                                                        \
+                case 1: hipLaunchKernelGGL((KERNEL<1, kSlabV, 16, T>), grid, dim3(16 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 2: hipLaunchKernelGGL((KERNEL<2, kSlabV, 16, T>), grid, dim3(16 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 3: hipLaunchKernelGGL((KERNEL<3, kSlabV, 16, T>), grid, dim3(16 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 4: hipLaunchKernelGGL((KERNEL<4, kSlabV, 16, T>), grid, dim3(16 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                case 5: hipLaunchKernelGGL((KERNEL<5, kSlabV, 16, T>), grid, dim3(16 * kWave), 0, st, __VA_ARGS__, sa); break; \
+                default: return COMMIT_HIP_EUNSUPPORTED;                                                          \
+            }                                                                                                     \
+        }                                                                                                         \
+        HIP_TRY(hipGetLastError());                                                                               \
+    } while (0)
+
+template <typename T>
+int launch_A_slab(commit_hip_plan *p, const T *x, const T *xic, const AOut<T> &out, hipStream_t st)
+{
+    const DevDict &d = p->d;
+    if (!p->slab || !p->slab_vlist || d.nSp / kWave > kSlabMaxNJ) return COMMIT_HIP_EUNSUPPORTED;
+    SLAB_SWITCH(k_A_slab, kSlabVA, p->slab_vlist, p->slab_blocks, d, x, xic, out);
+    return 0;
+}
+
+template <typename T>
+int launch_At_slab(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t st)
+{
+    const DevDict &d = p->d;
+    if (!p->slab || !p->slab_vlist_at || d.nSp / kWave > kSlabMaxNJ) return COMMIT_HIP_EUNSUPPORTED;
+    SLAB_SWITCH(k_At_slab, kSlabVAt, p->slab_vlist_at, p->slab_blocks_at, d, y, sqrtw, p->zbuf, x);
+    return 0;
+}
+
+#undef SLAB_SWITCH
+
+template int launch_A_slab<double>(commit_hip_plan *, const double *, const double *, const AOut<double> &,
+                                   hipStream_t);
+template int launch_A_slab<float>(commit_hip_plan *, const float *, const float *, const AOut<float> &, hipStream_t);
+template int launch_At_slab<double>(commit_hip_plan *, const double *, const double *, double *, hipStream_t);
+template int launch_At_slab<float>(commit_hip_plan *, const float *, const float *, float *, hipStream_t);
+
+}  // namespace commit_internal

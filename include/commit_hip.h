@@ -94,11 +94,18 @@ int commit_hip_plan_precision(const commit_hip_plan *plan);
  *   COMMIT_HIP_OPT_A_DENSE       0 (default): A x per segment; 1: per tile of 16
  *                                voxels as a dense FP64 matrix-core product over
  *                                all orientations (ndirs <= 512, nS <= 320)
+ *   COMMIT_HIP_OPT_SLAB          0 (default): IC rows of A x and A^T y read from
+ *                                L2 per entry; 1 or 2: IC table staged in LDS in
+ *                                slabs of orientations by work-groups of 8 or 16
+ *                                wavefronts (80 or 160 KiB; nS <= 320); setting
+ *                                it builds the slab tables (a host copy of the
+ *                                voxel pointers and one small kernel)
  * get returns the value, or a negative code.
  */
 #define COMMIT_HIP_OPT_XCD_REMAP 1
 #define COMMIT_HIP_OPT_MULTI_A_WAVES 2
 #define COMMIT_HIP_OPT_A_DENSE 3
+#define COMMIT_HIP_OPT_SLAB 4
 int commit_hip_plan_set_option(commit_hip_plan *plan, int option, int value);
 int commit_hip_plan_get_option(const commit_hip_plan *plan, int option);
 

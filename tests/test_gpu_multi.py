@@ -150,9 +150,12 @@ def test_autotune_sets_and_caches(small, tmp_path):
     cache = str(tmp_path / "tune.json")
     rep = autotune.tune(A, nrhs=16, reps=2, cache=cache)
     assert rep["xcd_remap"]["choice"] in (0, 1) and rep["multi_a_waves"]["choice"] in (1, 2)
+    assert rep["slab"]["choice"] in (0, 1, 2)
+    assert autotune.get_option(A, _lib.OPT_SLAB) == rep["slab"]["choice"]
     assert autotune.get_option(A, _lib.OPT_A_DENSE) == rep["a_dense"]["choice"]
     assert autotune.get_option(A, _lib.OPT_MULTI_A_WAVES) == rep["multi_a_waves"]["choice"]
     B = LinearOperator(D, K, None)
     rep2 = autotune.tune(B, nrhs=16, cache=cache)            # from the cache, no timing
     assert rep2 == rep
     assert autotune.get_option(B, _lib.OPT_XCD_REMAP) == rep["xcd_remap"]["choice"]
+    assert autotune.get_option(B, _lib.OPT_SLAB) == rep["slab"]["choice"]

@@ -1,0 +1,128 @@
+"""GPU parity of the LDS-slab products (plan option COMMIT_HIP_OPT_SLAB, modes 1
+and 2: the IC table staged in LDS by orientation slabs) against the oracle.
+
+Bar: relative error <= 1e-12 in FP64 against the oracle's FP64 products
+(A x, A^T y, the FISTA residual sqrtw (A x - y) with its squared norm and the
+weighted gradient A^T (sqrtw r)), <= 1e-6 with FP32 storage; bitwise repeatable
+run to run; <= 1e-12 from the per-voxel kernels (slab off).  Cases cover
+partial last slabs, several IC RFs, empty voxels, a heavy voxel whose
+single-slab ranges span several 64-entry chunks, eval masks, the LUT-free
+plan, and nS past the slab kernels' limit (per-voxel fallback).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from _problems import random_dictionary, rel
+from commit_amd import _lib, autotune
+from commit_amd.operator.operator import LinearOperator
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _dictionary(seed, nVOX, n, nS, ndirs, nIC, nEC, nISO, heavy=None):
+    """random_dictionary with ~n/nVOX entries per voxel; heavy = (voxel, count)
+    moves count entries into one voxel (the arrays stay voxel-sorted)."""
+    D, K = random_dictionary(seed, nVOX=nVOX, nF=max(8, n // 25), n=n, nS=nS, ndirs=ndirs, nIC=nIC, nEC=nEC,
+                             nISO=nISO, empty_voxels=(1, nVOX - 2), ec_per_voxel=2)
+    if heavy is not None:
+        hv, cnt = heavy
+        v = D["IC"]["vox"].copy()
+        rng = np.random.default_rng(seed + 1)
+        pick = np.sort(rng.choice(v.size, cnt, replace=False))
+        v[pick] = hv
+        idx = np.argsort(v, kind="mergesort")
+        for k in ("vox", "str", "dir", "len"):
+            D["IC"][k] = D["IC"][k][idx] if k != "vox" else v[idx]
+    ev = np.ones(D["IC"]["nSTR"] * nIC, np.uint32)
+    ev[::7] = 0
+    D["IC"]["eval"] = ev
+    return D, K
+
+
+def _products(A, x, y, w):
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = _lib.load()
+    xd, yd, wd = (torch.from_numpy(a).cuda() for a in (x, y, w))
+    res = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
+    rsq = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.check(lib.commit_hip_A_residual(A._plan.handle, P(xd), P(yd), P(wd), P(res), P(rsq), st), "A_residual")
+    g = torch.empty(A.shape[1], dtype=torch.float64, device="cuda")
+    _lib.check(lib.commit_hip_At_weighted(A._plan.handle, P(res), P(wd), P(g), st), "At_weighted")
+    return A.dot(x), A.T.dot(y), res.cpu().numpy(), rsq.item(), g.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("nS,ndirs", [(1, 3), (7, 500), (60, 130), (65, 500), (288, 500), (320, 37)])
+@pytest.mark.parametrize("rf", [(1, 1, 2), (2, 0, 1), (3, 2, 0)])
+def test_slab_vs_oracle(mode, nS, ndirs, rf):
+    nIC, nEC, nISO = rf
+    D, K = _dictionary(nS * 7 + ndirs + mode, nVOX=150, n=9000, nS=nS, ndirs=ndirs, nIC=nIC, nEC=nEC, nISO=nISO,
+                       heavy=(3, 1500))
+    T = oracle.set_threads(D, 4)
+    A = LinearOperator(D, K, T)
+    autotune.set_option(A, _lib.OPT_SLAB, mode)
+    assert autotune.get_option(A, _lib.OPT_SLAB) == mode
+    O = oracle.OracleOperator(D, K, T)
+    rng = np.random.default_rng(nS + ndirs)
+    x = rng.standard_normal(A.shape[1])
+    x[rng.random(x.size) < 0.3] = 0.0
+    y = rng.standard_normal(A.shape[0])
+    w = rng.uniform(0.5, 1.5, A.shape[0])
+    Ax, Aty, res, rsq, g = _products(A, x, y, w)
+    Ax_o, Aty_o = O.dot(x), O.T.dot(y)
+    r_o = w * (Ax_o - y)
+    g_o = O.T.dot(w * r_o)
+    e = (rel(Ax, Ax_o), rel(Aty, Aty_o), rel(res, r_o), abs(rsq - r_o @ r_o) / (r_o @ r_o), rel(g, g_o))
+    print(f"mode={mode} nS={nS} ndirs={ndirs} rf={rf}: " + " ".join(f"{v:.1e}" for v in e), flush=True)
+    assert max(e) <= TOL
+    # bitwise repeatable, and the per-voxel kernels agree
+    Ax2, Aty2, res2, _, g2 = _products(A, x, y, w)
+    assert np.array_equal(Ax, Ax2) and np.array_equal(Aty, Aty2) and np.array_equal(g, g2)
+    autotune.set_option(A, _lib.OPT_SLAB, 0)
+    assert rel(A.dot(x), Ax) <= TOL and rel(A.T.dot(y), Aty) <= TOL
+    # FP32 storage
+    autotune.set_option(A, _lib.OPT_SLAB, mode)
+    A.set_precision(_lib.FP32)
+    x32 = x.astype(np.float32).astype(np.float64)
+    y32 = y.astype(np.float32).astype(np.float64)
+    assert rel(A.dot(x32), O.dot(x32)) <= 1e-6 and rel(A.T.dot(y32), O.T.dot(y32)) <= 1e-6
+
+
+def test_slab_nolut_and_fallback():
+    """LUT-free plan with slabs on; nS = 400 (past the slab kernels) falls back
+    to the per-voxel kernels and stays right."""
+    D, K = _dictionary(5, nVOX=90, n=4000, nS=7, ndirs=9, nIC=1, nEC=0, nISO=1)
+    T = oracle.set_threads(D, 4)
+    A = LinearOperator(D, K, T, nolut=True)
+    autotune.set_option(A, _lib.OPT_SLAB, 2)
+    O = oracle.OracleOperator(D, K, T, nolut=True)
+    rng = np.random.default_rng(9)
+    x, y = rng.random(A.shape[1]), rng.random(A.shape[0])
+    assert rel(A.dot(x), O.dot(x)) <= TOL and rel(A.T.dot(y), O.T.dot(y)) <= TOL
+    D, K = _dictionary(6, nVOX=40, n=2000, nS=400, ndirs=20, nIC=1, nEC=1, nISO=1)
+    T = oracle.set_threads(D, 4)
+    A = LinearOperator(D, K, T)
+    autotune.set_option(A, _lib.OPT_SLAB, 1)
+    O = oracle.OracleOperator(D, K, T)
+    x, y = rng.random(A.shape[1]), rng.random(A.shape[0])
+    assert rel(A.dot(x), O.dot(x)) <= TOL and rel(A.T.dot(y), O.T.dot(y)) <= TOL
+
+
+def test_slab_option_errors_and_toggling():
+    D, K = _dictionary(8, nVOX=60, n=3000, nS=60, ndirs=40, nIC=1, nEC=1, nISO=1)
+    A = LinearOperator(D, K, None)
+    with pytest.raises(RuntimeError):
+        autotune.set_option(A, _lib.OPT_SLAB, 3)
+    rng = np.random.default_rng(2)
+    x = rng.random(A.shape[1])
+    ref = A.dot(x)
+    for mode in (1, 2, 2, 0, 1, 0):
+        autotune.set_option(A, _lib.OPT_SLAB, mode)
+        assert autotune.get_option(A, _lib.OPT_SLAB) == mode
+        assert rel(A.dot(x), ref) <= TOL
