@@ -151,8 +151,8 @@ def main():
     slab = tuned["slab"]["choice"]
     wpb = 8 if slab == 1 else 16
     name_A = ("k_A_dense<double>" if tuned["a_dense"]["choice"] else
-              f"k_A_slab<{nj}, 8, {wpb}, double>" if slab else f"k_A<{nj}, double>")
-    name_At = f"k_At_slab<{nj}, 4, {wpb}, double>" if slab else f"k_At<{nj}, double>"
+              f"k_A_slab<{nj}, {_lib.SLAB_VA}, {wpb}, double>" if slab else f"k_A<{nj}, double>")
+    name_At = f"k_At_slab<{nj}, {_lib.SLAB_VAT}, {wpb}, double>" if slab else f"k_At<{nj}, double>"
     kernels = {name_A: ms_A, name_At: ms_At1}
     dom_name = max(kernels, key=kernels.get)
     dom_ms = kernels[dom_name]

@@ -17,6 +17,8 @@ OPT_XCD_REMAP = 1
 OPT_MULTI_A_WAVES = 2
 OPT_A_DENSE = 3
 OPT_SLAB = 4
+# voxels per wavefront of the LDS-slab kernels (A x, A^T y): part of their names
+SLAB_VA, SLAB_VAT = 6, 6
 
 # every symbol include/commit_hip.h declares, with its ctypes signature
 _P = ctypes.c_void_p

@@ -90,8 +90,8 @@ int dev_alloc(commit_hip_plan *p, size_t bytes, void **out);
 
 // LDS-slab kernels (slab.hip): voxels per wavefront (A x holds NJ FP64
 // accumulators per voxel, A^T y an FP64 y row) and the largest NJ.
-constexpr int kSlabVA = 8;
-constexpr int kSlabVAt = 4;
+constexpr int kSlabVA = 6;
+constexpr int kSlabVAt = 6;
 constexpr int kSlabMaxNJ = 5;
 
 struct SlabArgs {

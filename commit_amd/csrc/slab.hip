@@ -260,9 +260,9 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
                                               const float *Lb, Bcast *bt, int o0, int nSp, int lane)
 {
     bt->off[lane] = ((int)r.o - o0) * nSp;
+    int4 oa = *reinterpret_cast<const int4 *>(&bt->off[0]);
+    int4 ob = *reinterpret_cast<const int4 *>(&bt->off[4]);
     for (int t0 = 0; t0 < cnt; t0 += 8) {
-        const int4 oa = *reinterpret_cast<const int4 *>(&bt->off[t0]);
-        const int4 ob = *reinterpret_cast<const int4 *>(&bt->off[t0 + 4]);
         const int of[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
         double p[8];
 #pragma unroll
@@ -271,6 +271,9 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
             lds_row<NJ>(raw, Lb + of[q]);
             p[q] = dot_row<NJ>(raw, yv);
         }
+        const int tn = t0 + 8 < kWave ? t0 + 8 : t0;      // next group's offsets load under the reduction
+        oa = *reinterpret_cast<const int4 *>(&bt->off[tn]);
+        ob = *reinterpret_cast<const int4 *>(&bt->off[tn + 4]);
         const double sum = reduce8(p, lane);
         const int te = t0 + (lane >> 3);
         const float le = __shfl(r.len, te);
@@ -455,8 +458,9 @@ int slab_configure(commit_hip_plan *p, int mode)
     if (d.n == 0 || d.nIC == 0 || d.nVOX == 0 || d.nSp / kWave > kSlabMaxNJ) return 0;
     const int W = mode == 1 ? 8 : 16;
     const int ldsf = (mode == 1 ? SlabLds<8>::floats : SlabLds<16>::floats);
-    const int S = std::min(d.ndirs, (ldsf - W * kBcastFloats) / d.nSp);
-    const int nsl = (d.ndirs + S - 1) / S;
+    const int Smax = std::min(d.ndirs, (ldsf - W * kBcastFloats) / d.nSp);
+    const int nsl = (d.ndirs + Smax - 1) / Smax;
+    const int S = (d.ndirs + nsl - 1) / nsl;           // equal slabs: no short last step
     std::vector<uint32_t> ptr((size_t)d.nVOX + 1);
     HIP_TRY(hipMemcpy(ptr.data(), d.ic.ptr, ptr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     int rc;
