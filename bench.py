@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def algorithmic_bytes(D, K, bx=8, by=8):
@@ -39,6 +40,14 @@ def algorithmic_bytes(D, K, bx=8, by=8):
     nrows = D["IC"]["nVOX"] * nS
     return (10 * nnz + 6 * nE + 4 * nI + ncols * bx + nrows * by
             + 4 * nS * (ndirs * (nIC + nEC) + nISO))
+
+
+def algorithmic_flops(D, K):
+    """Multiply-adds of one product (A x or A^T y), 2 flops each: nS per IC
+    segment and RF, per EC peak and RF, per ISO voxel and RF."""
+    nIC, nEC, nISO = K["wmr"].shape[0], K["wmh"].shape[0], K["iso"].shape[0]
+    nS = K["wmr"].shape[2]
+    return 2 * nS * (D["IC"]["n"] * nIC + D["EC"]["n"] * nEC + D["ISO"]["n"] * nISO)
 
 
 def main():
@@ -157,6 +166,8 @@ def main():
     dom_name = max(kernels, key=kernels.get)
     dom_ms = kernels[dom_name]
     dom_gbs = bytes_pass / (dom_ms * 1e-3) / 1e9
+    flops_pass = algorithmic_flops(D, K)
+    dom_tflops = flops_pass / (dom_ms * 1e-3) / 1e12
     print(f"[bench] A x {ms_A:.3f} ms; A^T y {ms_At:.3f} ms (pass 1 {ms_At1:.3f}, pass 2 {ms_At2:.3f}); "
           f"dominant {dom_name}: {dom_gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
     traffic = measured_traffic(dom_name)
@@ -218,6 +229,10 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom_name, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            # the same launch against the FP64 arithmetic peak (vector and matrix FP64 peaks
+            # coincide on MI355X): multiply-adds of one product, 2 flops each
+            "fp64": {"achieved": round(dom_tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(dom_tflops / FP64_PEAK_TFLOPS, 4), "flops": int(flops_pass)},
         },
         "cpu_baseline": cpu,
     }
