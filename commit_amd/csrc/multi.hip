@@ -226,109 +226,141 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
 }
 
 // ------------------------------------------------------------ X = A^T Y -----
-// Entries [b, e) of one compartment, 16 per M-tile.  The 16 LUT rows of a tile
-// (4NT 16-byte chunks each) go straight from memory into a wavefront-private
-// LDS tile (global_load_lds_dwordx4: lane l of a load writes 16 bytes at
-// base + 16 l), chunk-major: slot cc*16 + r holds chunk cc of row r, so each
-// load covers 64 contiguous bytes of 16 rows and the 16 lanes of one operand
-// read hit 16 consecutive slots (no bank conflicts).  A tile's operands are
-// read into registers, the next tile is requested into the same LDS, and the
-// matrix-core work of the current tile runs under that load.
+// Every row of the voxel's A^T product is one LUT row dotted with Y_v: its IC
+// entries for each IC RF, then its EC peaks for each EC RF, then its ISO
+// entries for each ISO RF.  They are taken as one list, 16 per M-tile, so the
+// few EC and ISO rows fill the free rows of the voxel's last IC tile instead
+// of a tile each.  The 16 LUT rows of a tile (4NT 16-byte chunks each) go
+// straight from memory into a wavefront-private LDS tile (global_load_lds_dwordx4:
+// lane l of a load writes 16 bytes at base + 16 l), chunk-major: slot cc*16 + r
+// holds chunk cc of row r, so each load covers 64 contiguous bytes of 16 rows
+// and the 16 lanes of one operand read hit 16 consecutive slots (no bank
+// conflicts).  A tile's operands are read into registers, the next tile is
+// requested into the same LDS, and the matrix-core work of the current tile
+// runs under that load.
 // yb: this lane's B operand, Y_v[g*4NT + t][rhs] for t < 4NT.
-//   IC     : z[(i*n + entry)*16 + q] = len * dot
-//   EC/ISO : X[(col + i*xs)*R + rhs]  = dot
+//   IC row (RF i, entry k) : z[(i*n + k)*16 + q]                        = len * dot
+//   EC row (RF i, peak k)  : X[(nIC*nF + i*nE + col)*R + rhs]            = dot
+//   ISO row (RF i, entry k): X[(nIC*nF + nEC*nE + i*nI + col)*R + rhs]   = dot
 typedef __attribute__((address_space(3))) void lds_void;
 
-template <int NT>
-__device__ __forceinline__ void at_request(const Comp &cp, uint32_t e0, uint32_t e, const float *__restrict__ Lb,
-                                           int nSp, int lane, float *T)
+struct VRows {
+    uint32_t icb, nic, ecb, nec, isob, niso;   // the voxel's entry ranges
+    uint32_t n1, n2, total;                    // row counts: IC, IC+EC, all
+};
+
+// Row r (< total) of the voxel: its LUT row, and its output slot and scale
+// (IC: slot i*n + k of z, scale len >= 0; EC/ISO: x column, scale -1).
+struct RowRef {
+    const float *row;
+    uint32_t slot;
+    float scale;
+};
+
+__device__ __forceinline__ RowRef row_ref(const DevDict &d, const VRows &vr, uint32_t r, size_t lut_rf)
 {
-    const uint32_t eq = e0 + (lane & 15);                        // row lane%16 of the tile
-    const uint32_t oe = (eq < e && cp.o) ? (uint32_t)cp.o[eq] : 0u;
-    const float *src = Lb + (size_t)oe * nSp + 4 * (lane >> 4);
+    if (r < vr.n1) {
+        const uint32_t i = d.nIC == 1 ? 0u : r / vr.nic, k = vr.icb + (r - i * vr.nic);
+        return RowRef{d.wmr + (size_t)i * lut_rf + (size_t)d.ic.o[k] * d.nSp, i * (uint32_t)d.n + k, d.ic.len[k]};
+    }
+    if (r < vr.n2) {
+        const uint32_t t = r - vr.n1, i = vr.nec == 1 ? t : t / vr.nec, k = vr.ecb + (t - i * vr.nec);
+        return RowRef{d.wmh + (size_t)i * lut_rf + (size_t)d.ec.o[k] * d.nSp,
+                      (uint32_t)d.nIC * (uint32_t)d.nF + i * (uint32_t)d.nE + d.ec.col[k], -1.0f};
+    }
+    const uint32_t t = r - vr.n2, i = vr.niso == 1 ? t : t / vr.niso, k = vr.isob + (t - i * vr.niso);
+    return RowRef{d.isol + (size_t)i * d.nSp,
+                  (uint32_t)d.nIC * (uint32_t)d.nF + (uint32_t)d.nEC * (uint32_t)d.nE + i * (uint32_t)d.nI +
+                      d.iso.col[k],
+                  -1.0f};
+}
+
+// Request the LUT rows of tile e0 (rows past the end repeat the last row).
+template <int NT>
+__device__ __forceinline__ void at_request(const DevDict &d, const VRows &vr, uint32_t e0, size_t lut_rf, int lane,
+                                           float *T)
+{
+    const uint32_t r = min(e0 + (uint32_t)(lane & 15), vr.total - 1);   // row lane%16 of the tile
+    const float *src = row_ref(d, vr, r, lut_rf).row + 4 * (lane >> 4);
 #pragma unroll
     for (int u = 0; u < NT; u++)                                  // chunks 4u .. 4u+3 of every row
         __builtin_amdgcn_global_load_lds(src + 16 * u, (lds_void *)(T + 256 * u), 16, 0, 0);
 }
 
-// Per-lane record of the 4 D rows (entries e0+4g+k) a tile produces: the
-// length (IC) or the x column (EC/ISO), fetched with the tile's LUT rows.
-__device__ __forceinline__ void at_aux(const Comp &cp, uint32_t e0, uint32_t e, int g, uint32_t (&aux)[4])
+// Output slot and scale of the 4 D rows (e0+4g+k) a tile produces for this lane.
+struct Aux {
+    uint32_t slot[4];
+    float scale[4];
+};
+
+__device__ __forceinline__ Aux at_aux(const DevDict &d, const VRows &vr, uint32_t e0, int g, size_t lut_rf)
 {
-    // e0 < e here; rows past e read entry e-1 and are zeroed, so the 4 loads
-    // are unconditional and in flight together.
-    const uint32_t *src = cp.len ? reinterpret_cast<const uint32_t *>(cp.len) : cp.col;
-    uint32_t v[4];
+    Aux a;
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = src[min(e0 + 4 * g + k, e - 1)];
-#pragma unroll
-    for (int k = 0; k < 4; k++) aux[k] = e0 + 4 * g + k < e ? v[k] : 0u;
+    for (int k = 0; k < 4; k++) {
+        const RowRef rr = row_ref(d, vr, min(e0 + 4 * g + k, vr.total - 1), lut_rf);
+        a.slot[k] = rr.slot;
+        a.scale[k] = rr.scale;
+    }
+    return a;
 }
 
-// Write the D tile of entries e0.. (deferred by one tile, so that the wait for
+// Write the D tile of rows e0.. (deferred by one tile, so that the wait for
 // the next LUT tile does not also wait for these stores).
-__device__ __forceinline__ void at_store(const f4 &acc, const uint32_t (&aux)[4], bool ic, uint32_t e0, uint32_t e,
-                                         int g, int q, int i, float *__restrict__ z, uint32_t n,
-                                         float *__restrict__ Xc, size_t xs, int R, bool rhs_ok)
+__device__ __forceinline__ void at_store(const f4 &acc, const Aux &a, uint32_t e0, uint32_t total, int g, int q,
+                                         float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok)
 {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {                        // D row 4g+k = entry e0+4g+k, column q
-        const uint32_t eo = e0 + 4 * g + k;
-        if (eo < e) {
-            if (ic)
-                z[((size_t)i * n + eo) * kTileR + q] = __uint_as_float(aux[k]) * acc[k];   // (setup_operator.py:412-428)
+    for (int k = 0; k < 4; k++) {                        // D row 4g+k = row e0+4g+k, column q
+        if (e0 + 4 * g + k < total) {
+            if (a.scale[k] >= 0.0f)
+                z[(size_t)a.slot[k] * kTileR + q] = a.scale[k] * acc[k];    // (setup_operator.py:412-428)
             else if (rhs_ok)
-                Xc[((size_t)aux[k] + (size_t)i * xs) * R] = acc[k];                       // (setup_operator.py:444-563)
+                Xr[(size_t)a.slot[k] * R] = acc[k];                         // (setup_operator.py:444-563)
         }
     }
 }
 
 template <int NT>
-__device__ __forceinline__ void mm_comp_At(const float (&yb)[4 * NT], const Comp &cp, uint32_t b, uint32_t e,
-                                           int nrf, const float *__restrict__ lut, size_t lut_rf, int nSp,
-                                           int lane, float *__restrict__ z, uint32_t n, float *__restrict__ Xc,
-                                           size_t xs, int R, bool rhs_ok, float *T)
+__device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d, const VRows &vr,
+                                          size_t lut_rf, int lane, float *__restrict__ z, float *__restrict__ Xr,
+                                          int R, bool rhs_ok, float *T)
 {
     const int g = lane >> 4, q = lane & 15;
-    const bool ic = cp.len != nullptr;
-    for (int i = 0; i < nrf; i++) {
-        const float *Lb = lut + (size_t)i * lut_rf;
-        uint32_t aux[4] = {0u, 0u, 0u, 0u}, auxn[4] = {0u, 0u, 0u, 0u}, auxp[4] = {0u, 0u, 0u, 0u};
-        if (b < e) {
-            at_request<NT>(cp, b, e, Lb, nSp, lane, T);
-            at_aux(cp, b, e, g, aux);
+    if (vr.total == 0) return;
+    at_request<NT>(d, vr, 0, lut_rf, lane, T);
+    Aux aux = at_aux(d, vr, 0, g, lut_rf), auxp = aux;
+    f4 pend = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t pe0 = vr.total;                             // total: nothing pending
+    for (uint32_t e0 = 0; e0 < vr.total; e0 += 16) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");     // tile e0 is in LDS
+        __builtin_amdgcn_wave_barrier();
+        float4 a[NT];
+#pragma unroll
+        for (int tb = 0; tb < NT; tb++)                      // chunk g*NT + tb of row q
+            a[tb] = *reinterpret_cast<const float4 *>(T + 4 * ((g * NT + tb) * 16 + q));
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // operands in registers
+        __builtin_amdgcn_wave_barrier();
+        if (pe0 < vr.total) at_store(pend, auxp, pe0, vr.total, g, q, z, Xr, R, rhs_ok);
+        Aux auxn = aux;
+        if (e0 + 16 < vr.total) {
+            at_request<NT>(d, vr, e0 + 16, lut_rf, lane, T);
+            auxn = at_aux(d, vr, e0 + 16, g, lut_rf);
         }
-        f4 pend = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t pe0 = e;                                 // e: nothing pending
-        for (uint32_t e0 = b; e0 < e; e0 += 16) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");     // tile e0 is in LDS
-            __builtin_amdgcn_wave_barrier();
-            float4 a[NT];
+        f4 acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
 #pragma unroll
-            for (int tb = 0; tb < NT; tb++)                      // chunk g*NT + tb of row q
-                a[tb] = *reinterpret_cast<const float4 *>(T + 4 * ((g * NT + tb) * 16 + q));
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // operands in registers
-            __builtin_amdgcn_wave_barrier();
-            if (pe0 < e) at_store(pend, auxp, ic, pe0, e, g, q, i, z, n, Xc, xs, R, rhs_ok);
-            if (e0 + 16 < e) {
-                at_request<NT>(cp, e0 + 16, e, Lb, nSp, lane, T);
-                at_aux(cp, e0 + 16, e, g, auxn);
-            }
-            f4 acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
-#pragma unroll
-            for (int tb = 0; tb < NT; tb++) {
-                acc0 = mfma4(a[tb].x, yb[4 * tb + 0], acc0);
-                acc1 = mfma4(a[tb].y, yb[4 * tb + 1], acc1);
-                acc0 = mfma4(a[tb].z, yb[4 * tb + 2], acc0);
-                acc1 = mfma4(a[tb].w, yb[4 * tb + 3], acc1);
-            }
-            pend = acc0 + acc1;
-            pe0 = e0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) auxp[k] = aux[k], aux[k] = auxn[k];
+        for (int tb = 0; tb < NT; tb++) {
+            acc0 = mfma4(a[tb].x, yb[4 * tb + 0], acc0);
+            acc1 = mfma4(a[tb].y, yb[4 * tb + 1], acc1);
+            acc0 = mfma4(a[tb].z, yb[4 * tb + 2], acc0);
+            acc1 = mfma4(a[tb].w, yb[4 * tb + 3], acc1);
         }
-        if (pe0 < e) at_store(pend, auxp, ic, pe0, e, g, q, i, z, n, Xc, xs, R, rhs_ok);
+        pend = acc0 + acc1;
+        pe0 = e0;
+        auxp = aux;
+        aux = auxn;
     }
+    if (pe0 < vr.total) at_store(pend, auxp, pe0, vr.total, g, q, z, Xr, R, rhs_ok);
 }
 
 template <int NT>
@@ -369,17 +401,17 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
 #pragma unroll
     for (int t = 0; t < 4 * NT; t++)
         if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
-    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    float *Xr = X + rhs;
-    mm_comp_At<NT>(yb, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], d.nIC, d.wmr, lut_rf, d.nSp, lane, z, (uint32_t)d.n,
-                   Xr, 0, R, rhs_ok, T);
-    if (d.nEC > 0)
-        mm_comp_At<NT>(yb, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], d.nEC, d.wmh, lut_rf, d.nSp, lane, z,
-                       (uint32_t)d.n, Xr + (size_t)d.nIC * d.nF * R, (size_t)d.nE, R, rhs_ok, T);
-    if (d.nISO > 0)
-        mm_comp_At<NT>(yb, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], d.nISO, d.isol, (size_t)d.nSp, 0, lane, z,
-                       (uint32_t)d.n, Xr + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R, (size_t)d.nI, R,
-                       rhs_ok, T);
+    VRows vr;
+    vr.icb = d.ic.ptr[v];
+    vr.nic = d.nIC > 0 ? d.ic.ptr[v + 1] - vr.icb : 0u;
+    vr.ecb = d.nEC > 0 ? d.ec.ptr[v] : 0u;
+    vr.nec = d.nEC > 0 ? d.ec.ptr[v + 1] - vr.ecb : 0u;
+    vr.isob = d.nISO > 0 ? d.iso.ptr[v] : 0u;
+    vr.niso = d.nISO > 0 ? d.iso.ptr[v + 1] - vr.isob : 0u;
+    vr.n1 = (uint32_t)d.nIC * vr.nic;
+    vr.n2 = vr.n1 + (uint32_t)d.nEC * vr.nec;
+    vr.total = vr.n2 + (uint32_t)d.nISO * vr.niso;
+    mm_vox_At<NT>(yb, d, vr, (size_t)d.ndirs * d.nSp, lane, z, X + rhs, R, rhs_ok, T);
 }
 
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
