@@ -185,6 +185,55 @@ __device__ __forceinline__ void slab_a_pass(double (&acc)[V][NJ], const Comp &cp
     }
 }
 
+// EC and ISO items (entry x RF) of the wavefront's voxels: lane 8u + j holds
+// item j of voxel u (EC items RF-major, then ISO items RF-major), so the
+// records and x values of all of them load together, and each voxel's rows
+// are then fetched together instead of one dependent chain per entry.  A
+// voxel with more than kItemLanes items takes the per-entry path.
+constexpr int kItemLanes = 8;
+
+struct Item {
+    const float *row;   // the item's LUT row (sample 0)
+    uint32_t col;       // the item's x column
+    int m;              // items of this lane's voxel
+};
+
+template <int V>
+__device__ __forceinline__ Item item_decode(const DevDict &d, const int (&vox)[V], size_t lut_rf, int lane)
+{
+    const int u = lane >> 3, j = lane & (kItemLanes - 1);
+    int v = -1;
+#pragma unroll
+    for (int uu = 0; uu < V; uu++)
+        if (u == uu) v = vox[uu];
+    Item it{d.wmr, 0u, 0};
+    if (v < 0) return it;
+    const uint32_t eb = d.nEC > 0 ? d.ec.ptr[v] : 0u, ne = d.nEC > 0 ? d.ec.ptr[v + 1] - eb : 0u;
+    const uint32_t ib = d.nISO > 0 ? d.iso.ptr[v] : 0u, ni = d.nISO > 0 ? d.iso.ptr[v + 1] - ib : 0u;
+    const uint32_t nec = (uint32_t)d.nEC * ne;
+    it.m = (int)(nec + (uint32_t)d.nISO * ni);
+    if (j >= it.m) return it;
+    if ((uint32_t)j < nec) {                      // EC RF i, peak k     (setup_operator.py:111-165)
+        const uint32_t i = (uint32_t)j / ne, k = eb + ((uint32_t)j - i * ne);
+        it.row = d.wmh + (size_t)i * lut_rf + (size_t)d.ec.o[k] * d.nSp;
+        it.col = (uint32_t)d.nIC * (uint32_t)d.nF + i * (uint32_t)d.nE + d.ec.col[k];
+    } else {                                      // ISO RF i, entry k    (setup_operator.py:167-218)
+        const uint32_t t = (uint32_t)j - nec, i = t / ni, k = ib + (t - i * ni);
+        it.row = d.isol + (size_t)i * d.nSp;
+        it.col = (uint32_t)d.nIC * (uint32_t)d.nF + (uint32_t)d.nEC * (uint32_t)d.nE + i * (uint32_t)d.nI +
+                 d.iso.col[k];
+    }
+    return it;
+}
+
+__device__ __forceinline__ const float *readlane_ptr(const float *p, int l)
+{
+    const uint64_t b = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return reinterpret_cast<const float *>(((uint64_t)hi << 32) | lo);
+}
+
 template <int NJ, int V, int W, typename T>
 __global__ __launch_bounds__(W * kWave, 4) void k_A_slab(DevDict d, const T *__restrict__ x, const T *__restrict__ xic,
                                                      AOut<T> out, SlabArgs sa)
@@ -225,17 +274,35 @@ __global__ __launch_bounds__(W * kWave, 4) void k_A_slab(DevDict d, const T *__r
         }
     }
     const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    // EC and ISO: coefficient x of every item of the wavefront's voxels at once
+    const Item it = item_decode<V>(d, vox, lut_rf, lane);
+    double cit = 0.0;
+    if ((lane & (kItemLanes - 1)) < it.m) cit = (double)x[it.col];
 #pragma unroll
     for (int u = 0; u < V; u++) {
         const int v = vox[u];
         if (v < 0) continue;
-        if (d.nEC > 0)
-            a_comp<NJ>(acc[u], d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
-                       d.wmh, lut_rf, d.nSp, lane, tail);
-        if (d.nISO > 0)
-            a_comp<NJ>(acc[u], d.iso, d.iso.ptr[v], d.iso.ptr[v + 1],
-                       x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
-                       (size_t)d.nSp, 0, lane, tail);
+        const int m = __builtin_amdgcn_readlane(it.m, kItemLanes * u);
+        if (m <= kItemLanes) {
+            for (int j0 = 0; j0 < m; j0 += 4) {                  // up to 4 rows in flight
+                float r[4][NJ];
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++)
+                    if (j0 + jj < m)
+                        fetch_row<NJ>(r[jj], readlane_ptr(it.row, kItemLanes * u + j0 + jj) + lane, tail);
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++)
+                    if (j0 + jj < m) fma_row<NJ>(acc[u], readlane_f64(cit, kItemLanes * u + j0 + jj), r[jj]);
+            }
+        } else {
+            if (d.nEC > 0)
+                a_comp<NJ>(acc[u], d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE,
+                           d.nEC, d.wmh, lut_rf, d.nSp, lane, tail);
+            if (d.nISO > 0)
+                a_comp<NJ>(acc[u], d.iso, d.iso.ptr[v], d.iso.ptr[v + 1],
+                           x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
+                           (size_t)d.nSp, 0, lane, tail);
+        }
         a_store<NJ, T>(acc[u], d, v, out, lane);
     }
 }
@@ -366,17 +433,39 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
         }
     }
     const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    // EC and ISO: a voxel's items (at most 8) reduced together by reduce8;
+    // item j ends in lanes 8j..8j+7 and lane 8j stores it (setup_operator.py:444-563)
+    const Item it = item_decode<V>(d, vox, lut_rf, lane);
 #pragma unroll
     for (int u = 0; u < V; u++) {
         const int v = vox[u];
         if (v < 0) continue;
-        if (d.nEC > 0)
-            glob_at<NJ>(yv[u], d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
-                        d.wmh, lut_rf, d.nSp, lane, tail);
-        if (d.nISO > 0)
-            glob_at<NJ>(yv[u], d.iso, d.iso.ptr[v], d.iso.ptr[v + 1],
-                        x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
-                        (size_t)d.nSp, 0, lane, tail);
+        const int m = __builtin_amdgcn_readlane(it.m, kItemLanes * u);
+        if (m == 0) continue;
+        if (m <= kItemLanes) {
+            double p[8];
+#pragma unroll
+            for (int jj = 0; jj < 8; jj++) {
+                p[jj] = 0.0;
+                if (jj < m) {
+                    float raw[NJ];
+                    fetch_row<NJ>(raw, readlane_ptr(it.row, kItemLanes * u + jj) + lane, tail);
+                    p[jj] = dot_row<NJ>(raw, yv[u]);
+                }
+            }
+            const double sum = reduce8(p, lane);
+            const int jj = lane >> 3;
+            const uint32_t col = (uint32_t)__shfl((int)it.col, kItemLanes * u + jj);
+            if ((lane & 7) == 0 && jj < m) x[col] = (T)sum;
+        } else {
+            if (d.nEC > 0)
+                glob_at<NJ>(yv[u], d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE,
+                            d.nEC, d.wmh, lut_rf, d.nSp, lane, tail);
+            if (d.nISO > 0)
+                glob_at<NJ>(yv[u], d.iso, d.iso.ptr[v], d.iso.ptr[v + 1],
+                            x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
+                            (size_t)d.nSp, 0, lane, tail);
+        }
     }
 }
 

@@ -24,11 +24,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _dictionary(seed, nVOX, n, nS, ndirs, nIC, nEC, nISO, heavy=None):
+def _dictionary(seed, nVOX, n, nS, ndirs, nIC, nEC, nISO, heavy=None, ec_per_voxel=2):
     """random_dictionary with ~n/nVOX entries per voxel; heavy = (voxel, count)
     moves count entries into one voxel (the arrays stay voxel-sorted)."""
     D, K = random_dictionary(seed, nVOX=nVOX, nF=max(8, n // 25), n=n, nS=nS, ndirs=ndirs, nIC=nIC, nEC=nEC,
-                             nISO=nISO, empty_voxels=(1, nVOX - 2), ec_per_voxel=2)
+                             nISO=nISO, empty_voxels=(1, nVOX - 2), ec_per_voxel=ec_per_voxel)
     if heavy is not None:
         hv, cnt = heavy
         v = D["IC"]["vox"].copy()
@@ -126,3 +126,17 @@ def test_slab_option_errors_and_toggling():
         autotune.set_option(A, _lib.OPT_SLAB, mode)
         assert autotune.get_option(A, _lib.OPT_SLAB) == mode
         assert rel(A.dot(x), ref) <= TOL
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_slab_many_ec_iso_items(mode):
+    """Voxels with more EC/ISO items (peaks x RFs) than one lane group holds
+    (5 peaks x 2 EC RFs + 1 ISO RF = 11) take the per-entry path."""
+    D, K = _dictionary(17 + mode, nVOX=80, n=4000, nS=288, ndirs=60, nIC=1, nEC=2, nISO=1, ec_per_voxel=5)
+    T = oracle.set_threads(D, 4)
+    A = LinearOperator(D, K, T)
+    autotune.set_option(A, _lib.OPT_SLAB, mode)
+    O = oracle.OracleOperator(D, K, T)
+    rng = np.random.default_rng(mode)
+    x, y = rng.standard_normal(A.shape[1]), rng.standard_normal(A.shape[0])
+    assert rel(A.dot(x), O.dot(x)) <= TOL and rel(A.T.dot(y), O.T.dot(y)) <= TOL
