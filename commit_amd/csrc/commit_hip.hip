@@ -239,31 +239,40 @@ __global__ __launch_bounds__(kBlock) void k_At(DevDict d, const T *__restrict__ 
                     lane, tail, ring);
 }
 
-// A^T y pass 2: x[f + i nF] = eval * sum of the fibre's products, ascending voxel
-// order (pass 1 stored them fibre-major, so the run is contiguous).  eval is 0
-// or 1, so applying it once per fibre equals the reference's per-segment
-// product (setup_operator.py:428).
+// A^T y pass 2: x[f + i nF] = eval * sum of the fibre's products (pass 1 stored
+// them fibre-major, ascending voxel, so the run is contiguous).  16 lanes (one
+// DPP row) per (fibre, RF): lane j sums entries j, j+16, ... in ascending
+// order, 4 loads in flight, and the 16 partials are combined in a fixed tree
+// of row moves, so each step of the group reads one contiguous 128-byte line
+// and the result is bitwise repeatable.  eval is 0 or 1, so applying it once
+// per fibre equals the reference's per-segment product (setup_operator.py:428).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_At_fibre(DevDict d, const double *__restrict__ z, T *__restrict__ x)
 {
     const size_t total = (size_t)d.nF * d.nIC;
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t j = threadIdx.x & 15;
+    const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 4;
+    for (size_t g = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; g < total; g += stride) {
         const int i = (int)(g / d.nF);
         const uint32_t f = (uint32_t)(g - (size_t)i * d.nF);
         const double *zi = z + (size_t)i * d.n;
         const uint32_t kb = d.fib_ptr[f], ke = d.fib_ptr[f + 1];
         double acc = 0.0;
-        uint32_t k = kb;
-        for (; k + 8 <= ke; k += 8) {      // 8 loads in flight, summed in order
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = stream_load(zi + k + u);
-#pragma unroll
-            for (int u = 0; u < 8; u++) acc += v[u];
+        uint32_t k = kb + j;
+        for (; k + 48 < ke; k += 64) {
+            const double a = stream_load(zi + k), b = stream_load(zi + k + 16);
+            const double c = stream_load(zi + k + 32), e = stream_load(zi + k + 48);
+            acc += a;
+            acc += b;
+            acc += c;
+            acc += e;
         }
-        for (; k < ke; k++) acc += stream_load(zi + k);
-        x[g] = (T)(acc * (double)d.eval[g]);
+        for (; k < ke; k += 16) acc += stream_load(zi + k);
+        acc += dpp_f64<kDppQuadXor1>(acc);          // all 16 lanes of the group are active here
+        acc += dpp_f64<kDppQuadXor2>(acc);
+        acc += dpp_f64<kDppHalfMirror>(acc);
+        acc += dpp_f64<kDppRowMirror>(acc);
+        if (j == 0) x[g] = (T)(acc * (double)d.eval[g]);
     }
 }
 
@@ -395,7 +404,7 @@ int launch_At(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t 
     }
     const size_t total = (size_t)d.nF * d.nIC;
     if (total > 0 && (pass & 2)) {
-        const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
+        const unsigned g2 = (unsigned)std::min<size_t>((total * 16 + kBlock - 1) / kBlock, 1u << 20);
         hipLaunchKernelGGL(k_At_fibre<T>, dim3(g2), dim3(kBlock), 0, st, d, p->zbuf, x);
         HIP_TRY(hipGetLastError());
     }

@@ -23,8 +23,8 @@
 //   A^T y : per entry, a per-lane partial dot product; 8 entries are reduced
 //           together over the 64 lanes with VALU cross-lane moves
 //           (permlane32/16 swaps and DPP), no LDS traffic.  The IC product
-//           goes to its fibre-major scratch slot, summed per fibre in
-//           ascending voxel order by pass 2 (k_At_fibre).
+//           goes to its fibre-major scratch slot, summed per fibre by
+//           pass 2 (k_At_fibre).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
