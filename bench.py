@@ -423,6 +423,9 @@ def autograd_mode(args):
     K = dict(K)
     K["dirs"] = raw["dirs"]
     A = LinearOperator(D, K, None, precision=_lib.FP32)
+    from commit_amd import autotune
+    tuned = autotune.tune(A, nrhs=0, reps=3, multi=False)          # launch options of the products
+    print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     xt = torch.from_numpy(synth.x_true(A.shape[1], 1004)).to(dev, torch.float32)
     truth = (1.9e-3, 0.5e-3)
     tt = lambda v: torch.tensor(v, dtype=torch.float64, device=dev)
@@ -459,7 +462,8 @@ def autograd_mode(args):
                        "ms_forward_backward": round(ms_fb, 3),
                        "fit": {"start": [1.7e-3, 0.6e-3], "truth": list(truth), "recovered": [dp, dq],
                                "evaluations": len(hist), "s": round(t_fit, 3),
-                               "loss_first": hist[0][0], "loss_last": hist[-1][0]}}}
+                               "loss_first": hist[0][0], "loss_last": hist[-1][0]},
+                       "autotune": tuned}}
     print(json.dumps(line), flush=True)
 
 

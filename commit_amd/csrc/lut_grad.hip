@@ -11,11 +11,13 @@
 // (the LUT entry multiplies exactly these terms in COMMIT_A, setup_operator.py:62-218).
 //
 // Scatter-reduction with privatised LDS tables: a work-group is one wavefront
-// that owns an FP64 table of 250 orientations x 64 samples (128 KB of LDS) for
-// one (RF, sample chunk, orientation part) and walks a slab of voxels; lane s
-// only ever touches column s, entries are applied in order, so no atomics are
-// needed and the result is deterministic.  The slab tables go to a partial
-// buffer and a second kernel adds the slabs in order.
+// that owns an FP64 table of 64 orientations x 64 samples (32 KB of LDS, so
+// five work-groups share a CU) for one (RF, sample chunk, orientation part)
+// and walks a slab of voxels.  Lane s only ever touches column s, and an entry
+// is one LDS add without return (ds_add_f64) issued in entry order by the
+// table's only wavefront, so the wavefront never waits on the table and the
+// result is deterministic.  The slab tables go to a partial buffer and a
+// second kernel adds the slabs in order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +33,8 @@ namespace {
 using commit_internal::Comp;
 using commit_internal::DevDict;
 
-constexpr int kOrPart = 250;        // orientations per LDS table
-constexpr int kSlabs = 48;          // voxel slabs (work-groups per table)
+constexpr int kOrPart = 64;         // orientations per LDS table
+constexpr int kSlabs = 64;          // voxel slabs (work-groups per table)
 
 // Pad [rows][nS] float tables into the plan's [rows][nSp] layout.
 __global__ void k_pad_lut(const float *__restrict__ src, float *__restrict__ dst, int64_t rows, int nS, int nSp)
@@ -61,38 +63,84 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
     const int o0 = h * kOrPart;
     const int no = min(kOrPart, ndirs - o0);
     for (int k = lane; k < kOrPart * kWave; k += kWave) tab[k] = 0.0;
+    __builtin_amdgcn_wave_barrier();
     const int s = c * kWave + lane;
+    const int sc = s < d.nS ? s : 0;
     const int v0 = (int)(((int64_t)d.nVOX * slab) / kSlabs), v1 = (int)(((int64_t)d.nVOX * (slab + 1)) / kSlabs);
-    for (int v = v0; v < v1; v++) {
-        const uint32_t b = cp.ptr[v], e = cp.ptr[v + 1];
-        if (b == e) continue;
-        const double g = s < d.nS ? (double)gy[(size_t)v * d.nS + s] : 0.0;
-        for (uint32_t base = b; base < e; base += kWave) {
-            const uint32_t k = base + lane;
-            double cf = 0.0;
-            int o = -1;
-            if (k < e) {
-                o = cp.o ? (int)cp.o[k] : 0;
-                const uint32_t col = cp.col[k];
-                cf = (double)xb[col];
-                if (cp.len) cf *= (double)cp.len[k];
-                if (ev && ev[col] == 0u) cf = 0.0;
-                if (o < o0 || o >= o0 + no) o = -1;                 // another part's orientation
+    // Software pipeline over the slab's voxels (first 64 entries of each):
+    // the entry range of voxel v+3, the records and gy of v+2 and the x
+    // gathers of v+1 load while voxel v is applied.  Later chunks of a long
+    // voxel load in place.  Stage data of voxels past v1 is never applied.
+    auto range = [&](int v, uint32_t &b, uint32_t &e) {
+        const int vc = v < v1 ? v : v1 - 1;
+        b = cp.ptr[vc];
+        e = v < v1 ? cp.ptr[vc + 1] : b;
+    };
+    auto recs = [&](uint32_t b, uint32_t e, uint32_t base, int &o, uint32_t &col, float &len) {
+        const uint32_t k = base + lane;
+        const uint32_t ks = k < e ? k : (b < e ? b : 0u);
+        o = cp.o ? (int)cp.o[ks] : 0;
+        col = cp.col[ks];
+        len = cp.len ? cp.len[ks] : 1.0f;
+    };
+    auto apply = [&](uint32_t b, uint32_t e, uint32_t base, int o, uint32_t col, float len, T xv, double g) {
+        const bool mine = base + lane < e && o >= o0 && o < o0 + no;        // this part's orientations
+        double cf = (double)xv;
+        if (cp.len) cf *= (double)len;
+        if (ev && ev[col] == 0u) cf = 0.0;
+        if (!mine) cf = 0.0;
+        unsigned long long m = __ballot(cf != 0.0);
+        while (m) {
+            const int t = __builtin_ctzll(m);
+            m &= m - 1;
+            const int ot = __builtin_amdgcn_readlane(o, t) - o0;
+            const long long cb = __double_as_longlong(cf);
+            const int lo = __builtin_amdgcn_readlane((int)(cb & 0xffffffffll), t);
+            const int hi = __builtin_amdgcn_readlane((int)(cb >> 32), t);
+            const double ct = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+            atomicAdd(tab + ot * kWave + lane, ct * g);          // lanes hit distinct addresses
+        }
+    };
+    if (v0 < v1 && cp.m > 0) {                                  // (record loads clamp to entry 0)
+        uint32_t b0, e0, b1, e1, b2, e2;
+        range(v0, b0, e0);
+        range(v0 + 1, b1, e1);
+        range(v0 + 2, b2, e2);
+        int o0r, o1r;
+        uint32_t c0, c1;
+        float l0, l1;
+        recs(b0, e0, b0, o0r, c0, l0);
+        recs(b1, e1, b1, o1r, c1, l1);
+        double g0 = (double)gy[(size_t)v0 * d.nS + sc];
+        double g1 = (double)gy[(size_t)min(v0 + 1, v1 - 1) * d.nS + sc];
+        T x0 = xb[c0];
+        for (int v = v0; v < v1; v++) {
+            uint32_t b3, e3;
+            range(v + 3, b3, e3);                                // stage: entry range of v+3
+            int o2r;
+            uint32_t c2;
+            float l2;
+            recs(b2, e2, b2, o2r, c2, l2);                       // stage: records and gy of v+2
+            const double g2 = (double)gy[(size_t)min(v + 2, v1 - 1) * d.nS + sc];
+            const T x1 = xb[c1];                                 // stage: x of v+1
+            if (b0 < e0) {
+                const double g = s < d.nS ? g0 : 0.0;
+                apply(b0, e0, b0, o0r, c0, l0, x0, g);
+                for (uint32_t base = b0 + kWave; base < e0; base += kWave) {
+                    int o;
+                    uint32_t col;
+                    float len;
+                    recs(b0, e0, base, o, col, len);
+                    apply(b0, e0, base, o, col, len, xb[col], g);
+                }
             }
-            unsigned long long m = __ballot(o >= 0 && cf != 0.0);
-            while (m) {
-                const int t = __builtin_ctzll(m);
-                m &= m - 1;
-                const int ot = __builtin_amdgcn_readlane(o, t) - o0;
-                const long long cb = __double_as_longlong(cf);
-                const int lo = __builtin_amdgcn_readlane((int)(cb & 0xffffffffll), t);
-                const int hi = __builtin_amdgcn_readlane((int)(cb >> 32), t);
-                const double ct = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-                double *p = tab + ot * kWave + lane;
-                *p = fma(ct, g, *p);
-            }
+            b0 = b1, e0 = e1, b1 = b2, e1 = e2, b2 = b3, e2 = e3;
+            o0r = o1r, c0 = c1, l0 = l1, o1r = o2r, c1 = c2, l1 = l2;
+            g0 = g1, g1 = g2;
+            x0 = x1;
         }
     }
+    __builtin_amdgcn_wave_barrier();
     double *dst = part + ((size_t)bid) * kOrPart * kWave;
     for (int k = lane; k < no * kWave; k += kWave) dst[k] = tab[k];
 }
