@@ -509,6 +509,7 @@ int build_vlist(commit_hip_plan *p, const std::vector<uint32_t> &ptr, int W, int
     std::vector<int> order(per);
     std::vector<uint64_t> load(W);
     std::vector<int> used(W);
+    std::vector<uint64_t> cost(nb);
     for (size_t b = 0; b < nb; b++) {
         const int v0 = (int)(b * per), nv = std::min(per, nVOX - v0);
         std::iota(order.begin(), order.begin() + nv, v0);
@@ -525,7 +526,17 @@ int build_vlist(commit_hip_plan *p, const std::vector<uint32_t> &ptr, int W, int
             vl[b * per + (size_t)best * V + used[best]++] = v;
             load[best] += ptr[v + 1] - ptr[v];
         }
+        cost[b] = *std::max_element(load.begin(), load.end());
     }
+    // work-groups are dispatched in index order: the heaviest first, so that
+    // the last ones to start are short and the launch ends evenly
+    std::vector<uint32_t> bo(nb);
+    std::iota(bo.begin(), bo.end(), 0u);
+    std::stable_sort(bo.begin(), bo.end(), [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
+    std::vector<int32_t> sorted(vl.size());
+    for (size_t b = 0; b < nb; b++)
+        std::copy(vl.begin() + (size_t)bo[b] * per, vl.begin() + (size_t)(bo[b] + 1) * per, sorted.begin() + b * per);
+    vl.swap(sorted);
     void *mem = nullptr;
     const int rc = dev_alloc(p, vl.size() * sizeof(int32_t), &mem);
     if (rc) return rc;
