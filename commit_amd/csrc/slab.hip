@@ -15,8 +15,10 @@
 // (stable), so a voxel's entries of one slab are a contiguous range; their
 // offsets are a [nVOX][nsl+1] table.  Voxels are dealt to the wavefronts of a
 // work-group longest-first onto the least-loaded wavefront, so the wavefronts
-// reach each slab barrier together.  EC and ISO (a few entries per voxel) use
-// rows from global memory, as in commit_hip.hip.
+// reach each slab barrier together; work-groups are numbered heaviest first.
+// EC and ISO (a few entries per voxel) use rows from global memory after the
+// slab walk: the items of all of a wavefront's voxels are decoded lane-parallel
+// and each voxel's rows are fetched together.
 //
 //   A x   : per entry, NJ FP64 FMAs per lane from an LDS row; zero
 //           coefficients skipped with a ballot (setup_operator.py:88).
@@ -85,20 +87,6 @@ __device__ __forceinline__ void lds_row(float (&raw)[NJ], const float *R)
 {
 #pragma unroll
     for (int j = 0; j < NJ; j++) raw[j] = R[kWave * j];
-}
-
-// Move each lane's value to lane dst (a permutation of the 64 lanes).
-__device__ __forceinline__ uint32_t lane_push(uint32_t v, int dst)
-{
-    return (uint32_t)__builtin_amdgcn_ds_permute(dst << 2, (int)v);
-}
-
-__device__ __forceinline__ double lane_push_f64(double v, int dst)
-{
-    const long long b = __double_as_longlong(v);
-    const uint32_t lo = lane_push((uint32_t)(b & 0xffffffffll), dst);
-    const uint32_t hi = lane_push((uint32_t)(b >> 32), dst);
-    return __longlong_as_double(((long long)hi << 32) | lo);
 }
 
 // Entry records of the first chunk (up to 64 entries) of a range [b, e): one
