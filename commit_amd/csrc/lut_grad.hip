@@ -83,11 +83,16 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
         col = cp.col[ks];
         len = cp.len ? cp.len[ks] : 1.0f;
     };
-    auto apply = [&](uint32_t b, uint32_t e, uint32_t base, int o, uint32_t col, float len, T xv, double g) {
+    // x gather and eval of an entry's column, loaded in the same stage
+    auto xload = [&](uint32_t col, T &xv, uint32_t &evv) {
+        xv = xb[col];
+        evv = ev ? ev[col] : 1u;
+    };
+    auto apply = [&](uint32_t e, uint32_t base, int o, float len, T xv, uint32_t evv, double g) {
         const bool mine = base + lane < e && o >= o0 && o < o0 + no;        // this part's orientations
         double cf = (double)xv;
         if (cp.len) cf *= (double)len;
-        if (ev && ev[col] == 0u) cf = 0.0;
+        if (evv == 0u) cf = 0.0;
         if (!mine) cf = 0.0;
         unsigned long long m = __ballot(cf != 0.0);
         while (m) {
@@ -113,7 +118,9 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
         recs(b1, e1, b1, o1r, c1, l1);
         double g0 = (double)gy[(size_t)v0 * d.nS + sc];
         double g1 = (double)gy[(size_t)min(v0 + 1, v1 - 1) * d.nS + sc];
-        T x0 = xb[c0];
+        T x0;
+        uint32_t ev0;
+        xload(c0, x0, ev0);
         for (int v = v0; v < v1; v++) {
             uint32_t b3, e3;
             range(v + 3, b3, e3);                                // stage: entry range of v+3
@@ -122,22 +129,28 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
             float l2;
             recs(b2, e2, b2, o2r, c2, l2);                       // stage: records and gy of v+2
             const double g2 = (double)gy[(size_t)min(v + 2, v1 - 1) * d.nS + sc];
-            const T x1 = xb[c1];                                 // stage: x of v+1
+            T x1;                                                // stage: x and eval of v+1
+            uint32_t ev1;
+            xload(c1, x1, ev1);
             if (b0 < e0) {
                 const double g = s < d.nS ? g0 : 0.0;
-                apply(b0, e0, b0, o0r, c0, l0, x0, g);
+                apply(e0, b0, o0r, l0, x0, ev0, g);
                 for (uint32_t base = b0 + kWave; base < e0; base += kWave) {
                     int o;
                     uint32_t col;
                     float len;
                     recs(b0, e0, base, o, col, len);
-                    apply(b0, e0, base, o, col, len, xb[col], g);
+                    T xv;
+                    uint32_t evv;
+                    xload(col, xv, evv);
+                    apply(e0, base, o, len, xv, evv, g);
                 }
             }
             b0 = b1, e0 = e1, b1 = b2, e1 = e2, b2 = b3, e2 = e3;
             o0r = o1r, c0 = c1, l0 = l1, o1r = o2r, c1 = c2, l1 = l2;
             g0 = g1, g1 = g2;
             x0 = x1;
+            ev0 = ev1;
         }
     }
     __builtin_amdgcn_wave_barrier();
