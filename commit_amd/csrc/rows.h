@@ -188,6 +188,19 @@ __device__ __forceinline__ double reduce8(const double (&p)[8], int lane)
     return s;
 }
 
+// Sum of 4 per-lane values over the 64 lanes, transposed: on return every
+// lane of the row of 16 lanes j = lane/16 holds the wave total of p[j].
+__device__ __forceinline__ double reduce4(const double (&p)[4])
+{
+    const double q0 = swap32_add(p[0], p[2]), q1 = swap32_add(p[1], p[3]);   // lanes 32-63: p[2], p[3]
+    double s = swap16_add(q0, q1);                                          // rows 1, 3: q1
+    s += dpp_f64<kDppQuadXor1>(s);
+    s += dpp_f64<kDppQuadXor2>(s);
+    s += dpp_f64<kDppHalfMirror>(s);
+    s += dpp_f64<kDppRowMirror>(s);
+    return s;
+}
+
 // Sum of one per-lane value over the 64 lanes, in every lane.
 __device__ __forceinline__ double wave_sum(double s)
 {

@@ -317,7 +317,8 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
     bt->off[lane] = ((int)r.o - o0) * nSp;
     int4 oa = *reinterpret_cast<const int4 *>(&bt->off[0]);
     int4 ob = *reinterpret_cast<const int4 *>(&bt->off[4]);
-    for (int t0 = 0; t0 < cnt; t0 += 8) {
+    int t0 = 0;
+    for (; t0 + 4 < cnt; t0 += 8) {
         const int of[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
         double p[8];
 #pragma unroll
@@ -334,6 +335,21 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
         const float le = __shfl(r.len, te);
         const uint32_t ze = (uint32_t)__shfl((int)r.col, te);
         if ((lane & 7) == 0 && te < cnt) zi[ze] = (double)le * sum;
+    }
+    if (t0 < cnt) {                          // a tail of at most 4 entries: reduce4 (entry t0+j in row j)
+        const int of[4] = {oa.x, oa.y, oa.z, oa.w};
+        double p[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float raw[NJ];
+            lds_row<NJ>(raw, Lb + of[q]);
+            p[q] = dot_row<NJ>(raw, yv);
+        }
+        const double sum = reduce4(p);
+        const int te = t0 + (lane >> 4);
+        const float le = __shfl(r.len, te);
+        const uint32_t ze = (uint32_t)__shfl((int)r.col, te);
+        if ((lane & 15) == 0 && te < cnt) zi[ze] = (double)le * sum;
     }
 }
 
