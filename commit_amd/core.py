@@ -376,6 +376,11 @@ class Evaluation:
             raise ValueError('CONFIG["precision"] must be "fp64" or "fp32"')
         self.A = operator.LinearOperator(self.DICTIONARY, self.KERNELS, self.THREADS, nolut,
                                          precision=_lib.FP32 if prec == "fp32" else _lib.FP64)
+        if self.get_config("autotune") is not False:
+            # launch options of the products (LDS slabs, dense-tile A, XCD order) timed on this
+            # dictionary; CONFIG["autotune"] = False keeps the defaults
+            from . import autotune
+            autotune.tune(self.A, nrhs=0, reps=2, multi=False)
 
     def get_y(self):
         """Measured signal at the mask voxels, voxel-major (core.pyx:751-766)."""
