@@ -35,6 +35,7 @@ using commit_internal::DevDict;
 
 constexpr int kOrPart = 64;         // orientations per LDS table
 constexpr int kSlabs = 64;          // voxel slabs (work-groups per table)
+constexpr int kDepth = 4;           // voxels in flight in the walk of k_lut_grad_part
 
 // Pad [rows][nS] float tables into the plan's [rows][nSp] layout.
 __global__ void k_pad_lut(const float *__restrict__ src, float *__restrict__ dst, int64_t rows, int nS, int nSp)
@@ -67,10 +68,13 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
     const int s = c * kWave + lane;
     const int sc = s < d.nS ? s : 0;
     const int v0 = (int)(((int64_t)d.nVOX * slab) / kSlabs), v1 = (int)(((int64_t)d.nVOX * (slab + 1)) / kSlabs);
-    // Software pipeline over the slab's voxels (first 64 entries of each):
-    // the entry range of voxel v+3, the records and gy of v+2 and the x
-    // gathers of v+1 load while voxel v is applied.  Later chunks of a long
-    // voxel load in place.  Stage data of voxels past v1 is never applied.
+    // Software pipeline over the slab's voxels (first 64 entries of each),
+    // kDepth voxels deep: while voxel v is applied, the records and gy of
+    // voxel v+kDepth, the x gathers of v+kDepth-1 and the entry range of
+    // v+kDepth+1 load.  Stage data lives in rings of kDepth+1 slots indexed at
+    // compile time (the voxel loop is unrolled by the ring size).  Later
+    // chunks of a long voxel load in place; stage data of voxels past v1 is
+    // never applied.
     auto range = [&](int v, uint32_t &b, uint32_t &e) {
         const int vc = v < v1 ? v : v1 - 1;
         b = cp.ptr[vc];
@@ -83,6 +87,7 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
         col = cp.col[ks];
         len = cp.len ? cp.len[ks] : 1.0f;
     };
+    auto gyv = [&](int v) { return (double)gy[(size_t)min(v, v1 - 1) * d.nS + sc]; };
     // x gather and eval of an entry's column, loaded in the same stage
     auto xload = [&](uint32_t col, T &xv, uint32_t &evv) {
         xv = xb[col];
@@ -106,51 +111,47 @@ __global__ __launch_bounds__(kWave) void k_lut_grad_part(DevDict d, Comp cp, int
             atomicAdd(tab + ot * kWave + lane, ct * g);          // lanes hit distinct addresses
         }
     };
+    constexpr int R = kDepth + 1;                                 // ring slots
     if (v0 < v1 && cp.m > 0) {                                  // (record loads clamp to entry 0)
-        uint32_t b0, e0, b1, e1, b2, e2;
-        range(v0, b0, e0);
-        range(v0 + 1, b1, e1);
-        range(v0 + 2, b2, e2);
-        int o0r, o1r;
-        uint32_t c0, c1;
-        float l0, l1;
-        recs(b0, e0, b0, o0r, c0, l0);
-        recs(b1, e1, b1, o1r, c1, l1);
-        double g0 = (double)gy[(size_t)v0 * d.nS + sc];
-        double g1 = (double)gy[(size_t)min(v0 + 1, v1 - 1) * d.nS + sc];
-        T x0;
-        uint32_t ev0;
-        xload(c0, x0, ev0);
-        for (int v = v0; v < v1; v++) {
-            uint32_t b3, e3;
-            range(v + 3, b3, e3);                                // stage: entry range of v+3
-            int o2r;
-            uint32_t c2;
-            float l2;
-            recs(b2, e2, b2, o2r, c2, l2);                       // stage: records and gy of v+2
-            const double g2 = (double)gy[(size_t)min(v + 2, v1 - 1) * d.nS + sc];
-            T x1;                                                // stage: x and eval of v+1
-            uint32_t ev1;
-            xload(c1, x1, ev1);
-            if (b0 < e0) {
-                const double g = s < d.nS ? g0 : 0.0;
-                apply(e0, b0, o0r, l0, x0, ev0, g);
-                for (uint32_t base = b0 + kWave; base < e0; base += kWave) {
-                    int o;
-                    uint32_t col;
-                    float len;
-                    recs(b0, e0, base, o, col, len);
-                    T xv;
-                    uint32_t evv;
-                    xload(col, xv, evv);
-                    apply(e0, base, o, len, xv, evv, g);
+        uint32_t rb[R], re[R], rc[R], rev[R];
+        int ro[R];
+        float rl[R];
+        double rg[R];
+        T rx[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) range(v0 + j, rb[j], re[j]);
+#pragma unroll
+        for (int j = 0; j < kDepth; j++) {
+            recs(rb[j], re[j], rb[j], ro[j], rc[j], rl[j]);
+            rg[j] = gyv(v0 + j);
+        }
+#pragma unroll
+        for (int j = 0; j < kDepth - 1; j++) xload(rc[j], rx[j], rev[j]);
+        for (int vb = v0; vb < v1; vb += R) {
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                const int v = vb + j;
+                if (v >= v1) break;
+                const int sr = (j + kDepth) % R, sx = (j + kDepth - 1) % R;
+                recs(rb[sr], re[sr], rb[sr], ro[sr], rc[sr], rl[sr]);   // stage: records and gy of v+kDepth
+                rg[sr] = gyv(v + kDepth);
+                xload(rc[sx], rx[sx], rev[sx]);                    // stage: x and eval of v+kDepth-1
+                if (rb[j] < re[j]) {
+                    const double g = s < d.nS ? rg[j] : 0.0;
+                    apply(re[j], rb[j], ro[j], rl[j], rx[j], rev[j], g);
+                    for (uint32_t base = rb[j] + kWave; base < re[j]; base += kWave) {
+                        int o;
+                        uint32_t col;
+                        float len;
+                        recs(rb[j], re[j], base, o, col, len);
+                        T xv;
+                        uint32_t evv;
+                        xload(col, xv, evv);
+                        apply(re[j], base, o, len, xv, evv, rg[j]);
+                    }
                 }
+                range(v + kDepth + 1, rb[j], re[j]);              // stage: entry range of v+kDepth+1
             }
-            b0 = b1, e0 = e1, b1 = b2, e1 = e2, b2 = b3, e2 = e3;
-            o0r = o1r, c0 = c1, l0 = l1, o1r = o2r, c1 = c2, l1 = l2;
-            g0 = g1, g1 = g2;
-            x0 = x1;
-            ev0 = ev1;
         }
     }
     __builtin_amdgcn_wave_barrier();
