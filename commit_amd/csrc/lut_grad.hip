@@ -10,7 +10,16 @@
 //   dL/diso[i][s]    = sum over ISO voxels v of column k:   x[ISO i, k] * gy[v*nS + s]
 // (the LUT entry multiplies exactly these terms in COMMIT_A, setup_operator.py:62-218).
 //
-// Scatter-reduction with privatised LDS tables: a work-group is one wavefront
+// IC tables (the bulk of the work): an orientation-major walk.  The plan keeps
+// (built on first use) the IC entries listed by orientation, with each entry's
+// voxel, cut into chunks of at most kOmChunk entries of one orientation.  One
+// wavefront per chunk accumulates the chunk's gradient row in registers (lane
+// = sample, NJ per lane, FP64): per entry it gathers the gy row of the entry's
+// voxel (rows of the next two entries in flight) and adds coefficient x gy.
+// No LDS tables, no atomics; a second kernel sums each orientation's chunk
+// rows in chunk order, so the result is deterministic.
+//
+// EC and ISO tables (a few entries per voxel) -- scatter-reduction with privatised LDS tables: a work-group is one wavefront
 // that owns an FP64 table of 64 orientations x 64 samples (32 KB of LDS, so
 // five work-groups share a CU) for one (RF, sample chunk, orientation part)
 // and walks a slab of voxels.  Lane s only ever touches column s, and an entry
@@ -21,10 +30,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "../../include/commit_hip.h"
 #include "common.h"
 #include "plan.h"
+#include "rows.h"
 
 using commit_internal::fail;
 
@@ -36,6 +47,7 @@ using commit_internal::DevDict;
 constexpr int kOrPart = 64;         // orientations per LDS table
 constexpr int kSlabs = 64;          // voxel slabs (work-groups per table)
 constexpr int kDepth = 4;           // voxels in flight in the walk of k_lut_grad_part
+constexpr int kOmChunk = 1024;      // IC entries per wavefront of the orientation-major walk
 
 // Pad [rows][nS] float tables into the plan's [rows][nSp] layout.
 __global__ void k_pad_lut(const float *__restrict__ src, float *__restrict__ dst, int64_t rows, int nS, int nSp)
@@ -176,6 +188,168 @@ __global__ void k_lut_grad_reduce(const double *__restrict__ part, int ndirs, in
     }
 }
 
+// NJ loads of the gy row of one voxel (this lane's sample 0; tail: the lane's
+// last sample is < nS).
+template <int NJ, typename T>
+__device__ __forceinline__ void gy_row(T (&g)[NJ], const T *__restrict__ row, bool tail)
+{
+#pragma unroll
+    for (int j = 0; j < NJ - 1; j++) g[j] = row[kWave * j];
+    g[NJ - 1] = tail ? row[kWave * (NJ - 1)] : (T)0;
+}
+
+template <int NJ, typename T>
+__device__ __forceinline__ void gy_fma(double (&acc)[NJ], double c, const T (&g)[NJ])
+{
+#pragma unroll
+    for (int j = 0; j < NJ; j++) acc[j] = fma(c, (double)g[j], acc[j]);
+}
+
+// part[chunk][s] = sum over the chunk's entries (orientation o of RF i) of
+// l * x * eval * gy[v*nS + s], entries in list order.
+template <int NJ, typename T>
+__global__ __launch_bounds__(kWave) void k_lut_grad_om(DevDict d, const uint32_t *__restrict__ ent,
+                                                       const uint32_t *__restrict__ vox,
+                                                       const uint32_t *__restrict__ chunks, const T *__restrict__ xb,
+                                                       const uint32_t *__restrict__ ev, const T *__restrict__ gy,
+                                                       double *__restrict__ part)
+{
+    const int lane = threadIdx.x;
+    const uint32_t c = blockIdx.x;
+    const uint32_t k0 = chunks[3 * c + 1], k1 = chunks[3 * c + 2];
+    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    const T *gl = gy + lane;
+    double acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) acc[j] = 0.0;
+    for (uint32_t base = k0; base < k1; base += kWave) {
+        const uint32_t k = base + lane;
+        const bool ok = k < k1;
+        const uint32_t ks = ok ? k : k0;
+        const uint32_t e = ent[ks], v = vox[ks];
+        const uint32_t col = d.ic.col[e];
+        double cf = (double)xb[col];
+        cf *= (double)d.ic.len[e];
+        if (ev && ev[col] == 0u) cf = 0.0;
+        if (!ok) cf = 0.0;
+        unsigned long long m = __ballot(cf != 0.0);
+        // three gy rows rotate through the pipeline; entries keep their order
+        T ga[NJ], gb[NJ], gc[NJ];
+        int ta = take(m), tb = -1, tc = -1;
+        if (ta >= 0) gy_row<NJ>(ga, gl + (size_t)(uint32_t)__builtin_amdgcn_readlane((int)v, ta) * d.nS, tail);
+        if ((tb = take(m)) >= 0) gy_row<NJ>(gb, gl + (size_t)(uint32_t)__builtin_amdgcn_readlane((int)v, tb) * d.nS, tail);
+        while (true) {
+            if (ta < 0) break;
+            if ((tc = take(m)) >= 0)
+                gy_row<NJ>(gc, gl + (size_t)(uint32_t)__builtin_amdgcn_readlane((int)v, tc) * d.nS, tail);
+            gy_fma<NJ>(acc, readlane_f64(cf, ta), ga);
+            if (tb < 0) break;
+            if ((ta = take(m)) >= 0)
+                gy_row<NJ>(ga, gl + (size_t)(uint32_t)__builtin_amdgcn_readlane((int)v, ta) * d.nS, tail);
+            gy_fma<NJ>(acc, readlane_f64(cf, tb), gb);
+            if (tc < 0) break;
+            if ((tb = take(m)) >= 0)
+                gy_row<NJ>(gb, gl + (size_t)(uint32_t)__builtin_amdgcn_readlane((int)v, tb) * d.nS, tail);
+            gy_fma<NJ>(acc, readlane_f64(cf, tc), gc);
+        }
+    }
+    double *dst = part + (size_t)c * d.nS;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int s = lane + kWave * j;
+        if (s < d.nS) dst[s] = acc[j];
+    }
+}
+
+// out[o][s] = sum of the chunk rows of orientation o, in chunk order.
+__global__ void k_lut_grad_om_reduce(const double *__restrict__ part, const uint32_t *__restrict__ cptr, int ndirs,
+                                     int nS, double *__restrict__ out)
+{
+    const int64_t total = (int64_t)ndirs * nS;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int o = (int)(t / nS), s = (int)(t - (int64_t)o * nS);
+        double acc = 0.0;
+        for (uint32_t c = cptr[o]; c < cptr[o + 1]; c++) acc += part[(size_t)c * nS + s];
+        out[t] = acc;
+    }
+}
+
+// Orientation-major list of the IC entries, their voxels, the chunks and the
+// chunk partial rows (once per plan; the IC structure never changes).
+int om_build(commit_hip_plan *p)
+{
+    if (p->lg_cptr) return 0;                              // built (ndirs+1 pointers, never empty)
+    const DevDict &d = p->d;
+    const size_t n = (size_t)d.n;
+    std::vector<uint32_t> ptr((size_t)d.nVOX + 1);
+    std::vector<uint16_t> o(n);
+    HIP_TRY(hipMemcpy(ptr.data(), d.ic.ptr, ptr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(o.data(), d.ic.o, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> optr((size_t)d.ndirs + 1, 0u), ent(n), vox(n);
+    for (size_t k = 0; k < n; k++) optr[o[k] + 1u]++;
+    for (int q = 0; q < d.ndirs; q++) optr[q + 1] += optr[q];
+    std::vector<uint32_t> pos(optr.begin(), optr.end() - 1);
+    for (int v = 0; v < d.nVOX; v++)                       // stable: voxel order inside an orientation
+        for (uint32_t k = ptr[v]; k < ptr[v + 1]; k++) {
+            const uint32_t at = pos[o[k]]++;
+            ent[at] = k;
+            vox[at] = (uint32_t)v;
+        }
+    std::vector<uint32_t> chunks, cptr((size_t)d.ndirs + 1, 0u);
+    for (int q = 0; q < d.ndirs; q++) {
+        cptr[q] = (uint32_t)(chunks.size() / 3);
+        for (uint32_t k = optr[q]; k < optr[q + 1]; k += kOmChunk) {
+            chunks.push_back((uint32_t)q);
+            chunks.push_back(k);
+            chunks.push_back(std::min(optr[q + 1], k + (uint32_t)kOmChunk));
+        }
+    }
+    cptr[d.ndirs] = (uint32_t)(chunks.size() / 3);
+    const int nch = (int)(chunks.size() / 3);
+    int rc;
+    void *m = nullptr;
+    auto up = [&](const std::vector<uint32_t> &h, uint32_t **dst) -> int {
+        *dst = nullptr;
+        if (h.empty()) return 0;
+        void *mm = nullptr;
+        const int r = commit_internal::dev_alloc(p, h.size() * sizeof(uint32_t), &mm);
+        if (r) return r;
+        HIP_TRY(hipMemcpy(mm, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        *dst = static_cast<uint32_t *>(mm);
+        return 0;
+    };
+    if ((rc = up(ent, &p->lg_ent)) || (rc = up(vox, &p->lg_vox)) || (rc = up(chunks, &p->lg_chunk)) ||
+        (rc = up(cptr, &p->lg_cptr)))
+        return rc;
+    if (nch > 0 && (rc = commit_internal::dev_alloc(p, (size_t)nch * d.nS * sizeof(double), &m))) return rc;
+    p->lg_part = static_cast<double *>(m);
+    p->lg_nchunks = nch;
+    return 0;
+}
+
+template <typename T>
+int launch_om(commit_hip_plan *p, const T *xb, const uint32_t *ev, const T *gy, double *out, hipStream_t st)
+{
+    const DevDict &d = p->d;
+    if (p->lg_nchunks > 0) {
+        switch (d.nSp / kWave) {
+#define CASE(NJ)                                                                                                  \
+    case NJ:                                                                                                      \
+        hipLaunchKernelGGL((k_lut_grad_om<NJ, T>), dim3((unsigned)p->lg_nchunks), dim3(kWave), 0, st, d, p->lg_ent, \
+                           p->lg_vox, p->lg_chunk, xb, ev, gy, p->lg_part);                                       \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d exceeds 512", d.nS);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    const unsigned rg = (unsigned)std::min<int64_t>(((int64_t)d.ndirs * d.nS + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_lut_grad_om_reduce, dim3(rg), dim3(256), 0, st, p->lg_part, p->lg_cptr, d.ndirs, d.nS, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int lut_grad(commit_hip_plan *plan, const T *x, const T *gy, double *gwmr, double *gwmh, double *giso,
              hipStream_t st)
@@ -193,13 +367,14 @@ int lut_grad(commit_hip_plan *plan, const T *x, const T *gy, double *gwmr, doubl
         plan->lgrad_elems = need;
     }
     const unsigned rg = (unsigned)std::min<int64_t>(((int64_t)d.ndirs * d.nS + 255) / 256, 4096);
-    if (gwmr)
-        for (int i = 0; i < d.nIC; i++) {
-            hipLaunchKernelGGL(k_lut_grad_part<T>, dim3((unsigned)blocks), dim3(kWave), 0, st, d, d.ic, d.ndirs,
-                               x + (size_t)i * d.nF, d.eval + (size_t)i * d.nF, gy, nparts, nchunk, plan->lgrad);
-            hipLaunchKernelGGL(k_lut_grad_reduce, dim3(rg), dim3(256), 0, st, plan->lgrad, d.ndirs, d.nS, nparts,
-                               nchunk, gwmr + (size_t)i * d.ndirs * d.nS);
-        }
+    if (gwmr && d.nIC > 0) {
+        int rc = om_build(plan);
+        if (rc) return rc;
+        for (int i = 0; i < d.nIC; i++)
+            if ((rc = launch_om<T>(plan, x + (size_t)i * d.nF, d.eval + (size_t)i * d.nF, gy,
+                                   gwmr + (size_t)i * d.ndirs * d.nS, st)))
+                return rc;
+    }
     if (gwmh)
         for (int i = 0; i < d.nEC; i++) {
             hipLaunchKernelGGL(k_lut_grad_part<T>, dim3((unsigned)blocks), dim3(kWave), 0, st, d, d.ec, d.ndirs,

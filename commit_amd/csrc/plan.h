@@ -69,6 +69,13 @@ struct commit_hip_plan {
     float *zmulti = nullptr;   // [nIC][n][16] per-entry products of the multi-RHS A^T pass 1
     double *lgrad = nullptr;   // per-slab partial tables of the LUT gradient
     size_t lgrad_elems = 0;
+    // orientation-major walk of the IC entries for the LUT gradient (built on first use)
+    uint32_t *lg_ent = nullptr;     // [n] IC entry index, by orientation (voxel order inside one)
+    uint32_t *lg_vox = nullptr;     // [n] voxel of each listed entry
+    uint32_t *lg_chunk = nullptr;   // [lg_nchunks][3] orientation, first, end (positions in lg_ent)
+    uint32_t *lg_cptr = nullptr;    // [ndirs+1] first chunk of each orientation
+    double *lg_part = nullptr;      // [lg_nchunks][nS] partial rows, one per chunk
+    int lg_nchunks = 0;
     bool eval_ones = true;
     int multi_a_waves = 2;     // wavefronts per (voxel, 16 RHS) in the multi-RHS A x (1 or 2)
     int a_dense = 0;           // 1: A x by the dense-tile matrix-core kernel (dense.hip)
