@@ -54,6 +54,17 @@ struct AOut {
     double *rpart;        // [nVOX] partial sums of r^2, or null
 };
 
+// One compartment's entries listed by orientation (voxel order inside one
+// orientation), cut into chunks of one orientation each, for the LUT gradient.
+struct OmList {
+    uint32_t *ent = nullptr;     // [m] entry index
+    uint32_t *vox = nullptr;     // [m] voxel of each listed entry
+    uint32_t *chunk = nullptr;   // [nchunks][2] first, end (positions in ent)
+    uint32_t *cptr = nullptr;    // [ndirs+1] first chunk of each orientation; null: not built
+    double *part = nullptr;      // [nchunks][nS] partial rows, one per chunk
+    int nchunks = 0;
+};
+
 }  // namespace commit_internal
 
 struct commit_hip_plan {
@@ -67,15 +78,9 @@ struct commit_hip_plan {
     double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
     double *rpart = nullptr;   // [nVOX + kSumBlocks] residual partials
     float *zmulti = nullptr;   // [nIC][n][16] per-entry products of the multi-RHS A^T pass 1
-    double *lgrad = nullptr;   // per-slab partial tables of the LUT gradient
-    size_t lgrad_elems = 0;
-    // orientation-major walk of the IC entries for the LUT gradient (built on first use)
-    uint32_t *lg_ent = nullptr;     // [n] IC entry index, by orientation (voxel order inside one)
-    uint32_t *lg_vox = nullptr;     // [n] voxel of each listed entry
-    uint32_t *lg_chunk = nullptr;   // [lg_nchunks][3] orientation, first, end (positions in lg_ent)
-    uint32_t *lg_cptr = nullptr;    // [ndirs+1] first chunk of each orientation
-    double *lg_part = nullptr;      // [lg_nchunks][nS] partial rows, one per chunk
-    int lg_nchunks = 0;
+    // orientation-major lists of the IC, EC and ISO entries for the LUT gradient
+    // (lut_grad.hip; built on first use)
+    commit_internal::OmList lg_om[3];
     bool eval_ones = true;
     int multi_a_waves = 2;     // wavefronts per (voxel, 16 RHS) in the multi-RHS A x (1 or 2)
     int a_dense = 0;           // 1: A x by the dense-tile matrix-core kernel (dense.hip)
