@@ -55,6 +55,8 @@ SIGNATURES = {
     "commit_hip_prox_multi": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_fista_multi_sums": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_multi_residual": (_I, [_P, _P, _P, _I, _P, _P, _P, _P, _P]),
+    "commit_hip_A_multi_residual_work": (ctypes.c_int64, [_P, _P]),
+    "commit_hip_A_multi_residual": (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     "commit_hip_fista_multi_control": (_I, [_P, _P, ctypes.c_double, ctypes.c_double, _I, _P, _P, _P, _P]),
     "commit_hip_fista_multi_update": (_I, [_P, _P, _P, _P, _P, _P]),
     "commit_hip_fista_multi_qf": (_I, [_P, _P, _P, _P]),

@@ -153,6 +153,8 @@ class BatchedFista:
         self.dev = A.device
         n, m = A.nCOLS, A.nROWS
         self.n, self.m, self.R = n, m, int(R)
+        if not 1 <= self.R <= 16:
+            raise ValueError("batched fits take 1..16 columns")
         S = 1 if layout is None else layout.S
         self.S = S
         self.tol_fun, self.tol_x, self.max_iter = float(tol_fun), float(tol_x), int(max_iter)
@@ -203,7 +205,10 @@ class BatchedFista:
         self.L = L
         self.work = torch.empty(int(self.lib.commit_hip_fista_multi_work(ctypes.byref(L), ng)) + 1,
                                 dtype=torch.float64, device=dev)
-        self.rwork = torch.empty(L.nypieces * self.R + 1, dtype=torch.float64, device=dev)
+        nrw = int(self.lib.commit_hip_A_multi_residual_work(A._plan.handle, ctypes.byref(L)))
+        if nrw < 0:
+            raise RuntimeError("commit_hip_A_multi_residual_work failed")
+        self.rwork = torch.empty(nrw + 1, dtype=torch.float64, device=dev)
         # vectors
         yv = torch.as_tensor(np.asarray(y, dtype=np.float32)) if not isinstance(y, torch.Tensor) else y
         yv = yv.to(**f32).contiguous()
@@ -222,8 +227,6 @@ class BatchedFista:
         self.prev = torch.zeros((n, self.R), **f32)
         self.grad = torch.zeros((n, self.R), **f32)
         self.res = torch.zeros((m, self.R), **f32)
-        self.AX = torch.zeros((m, self.R), **f32)
-        self.zero_y = torch.zeros(m, **f32)
         nq = S * self.R
         self.sums = torch.zeros(nq * 8, dtype=torch.float64, device=dev)
         self.rsq_x = torch.zeros(nq, dtype=torch.float64, device=dev)
@@ -288,10 +291,13 @@ class BatchedFista:
                                                          reg.l1_weighted, self._P(self.sums), self._P(self.work),
                                                          self._st()), "commit_hip_fista_multi_sums")
 
-    def _resid(self, AX, y, ypc, res, rsq):
-        self._check(self.lib.commit_hip_multi_residual(ctypes.byref(self.L), self._P(AX), self._P(y), ypc,
-                                                       self._P(self.sw), self._P(res), self._P(rsq),
-                                                       self._P(self.rwork), self._st()), "commit_hip_multi_residual")
+    def _resid(self, X, y, ypc, res, rsq):
+        """res = sqrtw .* (A X - y) (res None: norms only; y None: zero) and the
+        squared norm of every fit, in one pass (commit_hip_A_multi_residual)."""
+        self._check(self.lib.commit_hip_A_multi_residual(self.A._plan.handle, ctypes.byref(self.L), self._P(X),
+                                                         self._P(y), ypc, self._P(self.sw), self._P(res),
+                                                         self._P(rsq), self._P(self.rwork), self._st()),
+                    "commit_hip_A_multi_residual")
 
     def _omega_host(self, s8, c):
         reg = self.params.reg
@@ -315,8 +321,7 @@ class BatchedFista:
         if x0 is not None:
             self.xhat.copy_(torch.as_tensor(x0, dtype=torch.float32).reshape(self.n, self.R))
         # residual and gradient at x0
-        self.A.matmat(self.xhat, out=self.AX)
-        self._resid(self.AX, self.y, self.ypc, self.res, self.rsq_h)
+        self._resid(self.xhat, self.y, self.ypc, self.res, self.rsq_h)
         self.At.matmat(self.res, out=self.grad, sqrtw=self.sw)
         # in-place prox(xhat, 1) with step 1 (col.step = 1 from the constructor)
         zero = torch.zeros_like(self.grad)
@@ -326,8 +331,7 @@ class BatchedFista:
         s_x = self.sums.cpu().numpy().reshape(-1, 8)
         rsq0 = self.rsq_h.cpu().numpy()
         # Lipschitz quotient (||A grad|| / ||grad||)^2 per fit (solvers.py:282-286)
-        self.A.matmat(self.grad, out=self.AX)
-        self._resid(self.AX, self.zero_y, 0, None, self.rsq_x)
+        self._resid(self.grad, None, 0, None, self.rsq_x)
         self._sums(self.grad)
         s_g = self.sums.cpu().numpy().reshape(-1, 8)
         a_g = self.rsq_x.cpu().numpy()
@@ -352,8 +356,7 @@ class BatchedFista:
         """One trial step of every unfinished fit (the captured sequence)."""
         self._prox(self.xhat, self.grad, self.x)
         self._sums(self.x, self.xhat, self.grad, self.prev)
-        self.A.matmat(self.x, out=self.AX)
-        self._resid(self.AX, self.y, self.ypc, None, self.rsq_x)
+        self._resid(self.x, self.y, self.ypc, None, self.rsq_x)
         self._check(self.lib.commit_hip_fista_multi_control(ctypes.byref(self.params), ctypes.byref(self.L),
                                                             self.tol_fun, self.tol_x, self.max_iter,
                                                             self._P(self.sums), self._P(self.rsq_x),
@@ -362,8 +365,7 @@ class BatchedFista:
         self._check(self.lib.commit_hip_fista_multi_update(ctypes.byref(self.L), self._P(self.x), self._P(self.xhat),
                                                            self._P(self.prev), self._P(self.col), self._st()),
                     "commit_hip_fista_multi_update")
-        self.A.matmat(self.xhat, out=self.AX)
-        self._resid(self.AX, self.y, self.ypc, self.res, self.rsq_h)
+        self._resid(self.xhat, self.y, self.ypc, self.res, self.rsq_h)
         self._check(self.lib.commit_hip_fista_multi_qf(ctypes.byref(self.L), self._P(self.rsq_h), self._P(self.col),
                                                        self._st()), "commit_hip_fista_multi_qf")
         self.At.matmat(self.res, out=self.grad, sqrtw=self.sw)

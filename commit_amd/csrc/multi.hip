@@ -30,6 +30,7 @@
 #include "../../include/commit_hip.h"
 #include "common.h"
 #include "plan.h"
+#include "rows.h"
 
 using commit_internal::fail;
 
@@ -40,6 +41,7 @@ using commit_internal::DevDict;
 using commit_internal::wave_index;
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr int kMaxNT = 20;          // nS <= 320 for the multi-RHS kernels
 constexpr int kTileR = 16;          // RHS per wavefront (MFMA N)
@@ -182,11 +184,56 @@ __device__ __forceinline__ void mm_comp_A(f4 (&acc)[NH], const Comp &cp, uint32_
     }
 }
 
+// The fused residual of the batched fits (RES): instead of A X the output
+// stage writes res = sqrtw .* (A X - y) (when res is set) and the wavefront's
+// 16 column sums of res^2, in FP64, at part[((v * MS + h) * nrg + rg) * 16 + q].
+struct ResArgs {
+    const float *y;       // [rows] or [rows][R] (ystride 1); NULL: zero
+    int ystride;
+    const float *w;       // sqrtw [rows] or NULL
+    float *res;           // [rows][R] or NULL
+    double *part;
+};
+
+// Output stage of the fused residual for one lane (column rhs of voxel v,
+// rows rb + s): res = sqrtw (A X - y) stored when ra.res is set; returns the
+// lane's sum of res^2.  YM 0: y from the staged LDS row; 1: y per column from
+// memory; 2: y = 0.  The loop issues no memory load for YM 0 and 2, so the
+// stores never wait for one another.
+template <int NH, int NT, int YM>
+__device__ __forceinline__ double res_out(const f4 (&acc)[NH], const ResArgs &ra, const float *sy, const float *sw,
+                                          size_t rb, int nS, int g, int soff, int R, int rhs)
+{
+    double ss = 0.0;
+    float *rp = ra.res ? ra.res + rb * R + rhs : nullptr;
+#pragma unroll
+    for (int m = 0; m < NH; m++) {
+        float yv[4], wv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int s = min((4 * g + k) * NT + soff + m, nS - 1);
+            yv[k] = YM == 0 ? sy[s] : YM == 1 ? ra.y[(rb + s) * R + rhs] : 0.0f;
+            wv[k] = sw[s];                      // (not staged, and unused, without sqrtw)
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int s = (4 * g + k) * NT + soff + m;
+            if (s < nS) {
+                double t = (double)acc[m][k] - (double)yv[k];
+                if (ra.w) t *= (double)wv[k];
+                if (rp) __builtin_nontemporal_store((float)t, rp + s * R);
+                ss = fma(t, t, ss);
+            }
+        }
+    }
+    return ss;
+}
+
 // One wavefront per (voxel, RHS group, sample half h < MS): tiles
 // [h*NH, h*NH + NH) of the NT = NH * MS, i.e. samples i*NT + h*NH + m.
-template <int NH, int MS>
+template <int NH, int MS, bool RES>
 __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restrict__ X, float *__restrict__ Y,
-                                                 int R, int nrg, const uint32_t *__restrict__ eval)
+                                                 int R, int nrg, const uint32_t *__restrict__ eval, ResArgs ra)
 {
     constexpr int NT = NH * MS;
     const int lane = threadIdx.x & (kWave - 1);
@@ -197,6 +244,21 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
     const int g = lane >> 4, q = lane & 15;
     const int rhs = rg * kTileR + q;
     const int soff = h * NH;
+    // RES: the voxel's rows of a shared y and of sqrtw go from memory into
+    // this wavefront's LDS now and are read by the output stage, so that
+    // stage waits on no memory.
+    __shared__ float stage[RES ? kWavesPerBlock : 1][RES ? 2 * kMaxNT * 16 : 1];
+    float *sy = nullptr, *sw = nullptr;
+    if constexpr (RES) {
+        sy = stage[threadIdx.x >> 6];
+        sw = sy + kMaxNT * 16;
+        const size_t rb = (size_t)v * d.nS;
+        for (int j = 0; j * kWave < d.nS; j++) {
+            const size_t i = rb + (size_t)min(j * kWave + lane, d.nS - 1);
+            if (ra.y && !ra.ystride) __builtin_amdgcn_global_load_lds(ra.y + i, (lds_void *)(sy + kWave * j), 4, 0, 0);
+            if (ra.w) __builtin_amdgcn_global_load_lds(ra.w + i, (lds_void *)(sw + kWave * j), 4, 0, 0);
+        }
+    }
     f4 acc[NH];
 #pragma unroll
     for (int m = 0; m < NH; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -213,15 +275,73 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
                   0, d.nF, 0, rhs < R};
         mm_comp_A<NH, NT>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], is, d.nISO, d.isol, (size_t)d.nSp, lane, soff);
     }
-    if (rhs >= R) return;
-    float *yv = Y + (size_t)v * d.nS * R + rhs;
+    if constexpr (!RES) {
+        if (rhs >= R) return;
+        float *yv = Y + (size_t)v * d.nS * R + rhs;
 #pragma unroll
-    for (int m = 0; m < NH; m++) {
+        for (int m = 0; m < NH; m++) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int s = (4 * g + k) * NT + soff + m;
-            if (s < d.nS) __builtin_nontemporal_store(acc[m][k], yv + (size_t)s * R);
+            for (int k = 0; k < 4; k++) {
+                const int s = (4 * g + k) * NT + soff + m;
+                if (s < d.nS) __builtin_nontemporal_store(acc[m][k], yv + (size_t)s * R);
+            }
         }
+    } else {
+        // (solvers.py:306-308) per element as commit_hip_multi_residual: FP64
+        // difference of the FP32 product and y, times sqrtw, squared and summed
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");      // the staged rows are in LDS
+        double ss = 0.0;
+        if (rhs < R) {
+            const size_t rb = (size_t)v * d.nS;
+            if (!ra.y)
+                ss = res_out<NH, NT, 2>(acc, ra, sy, sw, rb, d.nS, g, soff, R, rhs);
+            else if (ra.ystride)
+                ss = res_out<NH, NT, 1>(acc, ra, sy, sw, rb, d.nS, g, soff, R, rhs);
+            else
+                ss = res_out<NH, NT, 0>(acc, ra, sy, sw, rb, d.nS, g, soff, R, rhs);
+        }
+        ss = swap32_add(ss, ss);           // lanes q, q+16, q+32, q+48 hold column q
+        ss = swap16_add(ss, ss);
+        if (g == 0) ra.part[(((size_t)v * MS + h) * nrg + rg) * kTileR + q] = ss;
+    }
+}
+
+// Sums of the wavefront partials per row piece: piece p (rows [b, e) of one
+// problem) owns the voxels whose first row lies in it, [ceil(b/nS), ceil(e/nS));
+// out[p * R + r] = sum over those voxels and their sample halves, in order.
+__global__ __launch_bounds__(kBlock) void k_res_pieces(const double *__restrict__ part, const int64_t *__restrict__ pieces,
+                                                       int nS, int nVOX, int MS, int nrg, int R,
+                                                       double *__restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    const int64_t b = pieces[2 * blockIdx.x], e = pieces[2 * blockIdx.x + 1];
+    const int64_t nv = nVOX, vb0 = (b + nS - 1) / nS, ve0 = (e + nS - 1) / nS;
+    const int64_t vb = vb0 < nv ? vb0 : nv, ve = ve0 < nv ? ve0 : nv;
+    const int q = threadIdx.x & 15, j = threadIdx.x >> 4;
+    for (int rg = 0; rg < nrg; rg++) {
+        double s = 0.0;
+        for (int64_t i = vb * MS + j; i < ve * MS; i += kBlock / 16) s += part[((size_t)i * nrg + rg) * kTileR + q];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x < 16 && rg * kTileR + q < R) {
+            double t = 0.0;
+            for (int k = 0; k < kBlock / 16; k++) t += sh[k * 16 + q];
+            out[(size_t)blockIdx.x * R + rg * kTileR + q] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// rsq[s * R + r] = sum of the piece sums of problem s, in piece order
+__global__ __launch_bounds__(kBlock) void k_res_problems(const double *__restrict__ pp, const int32_t *__restrict__ pseg,
+                                                         int npieces, int S, int R, double *__restrict__ rsq)
+{
+    for (int t = threadIdx.x; t < S * R; t += kBlock) {
+        const int r = t % R, s = t / R;
+        double v = 0.0;
+        for (int p = 0; p < npieces; p++)
+            if (pseg[p] == s) v += pp[(size_t)p * R + r];
+        rsq[t] = v;
     }
 }
 
@@ -242,7 +362,6 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
 //   IC row (RF i, entry k) : z[(i*n + k)*16 + q]                        = len * dot
 //   EC row (RF i, peak k)  : X[(nIC*nF + i*nE + col)*R + rhs]            = dot
 //   ISO row (RF i, entry k): X[(nIC*nF + nEC*nE + i*nI + col)*R + rhs]   = dot
-typedef __attribute__((address_space(3))) void lds_void;
 
 struct VRows {
     uint32_t icb, nic, ecb, nec, isob, niso;   // the voxel's entry ranges
@@ -455,6 +574,45 @@ int check_multi(commit_hip_plan *p, int nrhs, int dtype)
     return 0;
 }
 
+// one or two wavefronts per (voxel, RHS group); with two, each owns half of
+// the sample tiles (NT rounded up to even; the padded LUT rows hold zeros
+// there).  The choice is a plan option (commit_hip_plan_set_option).
+template <bool RES>
+int launch_A_mm(commit_hip_plan *plan, const float *x, float *y, int nrhs, const ResArgs &ra, hipStream_t st)
+{
+    const DevDict &d = plan->d;
+    const int nrg = (nrhs + kTileR - 1) / kTileR;
+    const size_t waves = (size_t)d.nVOX * nrg * plan->multi_a_waves;
+    if (waves == 0) return 0;
+    if (waves > (size_t)INT32_MAX - kWavesPerBlock) return fail(COMMIT_HIP_EUNSUPPORTED, "too many voxels x RHS groups");
+    const unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    const uint32_t *ev = plan->eval_ones ? nullptr : d.eval;
+    if (plan->multi_a_waves == 2) {
+        switch ((nt_of(d) + 1) / 2) {
+#define CASE(NH)                                                                                            \
+    case NH:                                                                                                \
+        hipLaunchKernelGGL((k_A_mm<NH, 2, RES>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev, ra); \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+#undef CASE
+            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+        }
+    } else {
+        switch (nt_of(d)) {
+#define CASE(NT)                                                                                            \
+    case NT:                                                                                                \
+        hipLaunchKernelGGL((k_A_mm<NT, 1, RES>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev, ra); \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+            CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
+#undef CASE
+            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -465,41 +623,38 @@ int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, 
     int rc = check_multi(plan, nrhs, dtype);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(plan->device));
+    return launch_A_mm<false>(plan, static_cast<const float *>(X), static_cast<float *>(Y), nrhs, ResArgs{},
+                              static_cast<hipStream_t>(stream));
+}
+
+int64_t commit_hip_A_multi_residual_work(const commit_hip_plan *plan, const commit_hip_multi_layout *L)
+{
+    if (!plan || !L || L->nrhs < 1 || L->nypieces < 1) return -1;
+    const int64_t nrg = (L->nrhs + kTileR - 1) / kTileR;
+    return (int64_t)plan->d.nVOX * 2 * nrg * kTileR + L->nypieces * L->nrhs;
+}
+
+int commit_hip_A_multi_residual(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *X,
+                                const float *y, int y_per_column, const float *sqrtw, float *res, double *rsq,
+                                double *work, void *stream)
+{
+    if (!plan || !L || !X || !rsq || !work) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    int rc = check_multi(plan, L->nrhs, COMMIT_HIP_FP32);
+    if (rc) return rc;
+    if (L->nrhs > kTileR) return fail(COMMIT_HIP_EUNSUPPORTED, "batched fits take 1..%d columns", kTileR);
     const DevDict &d = plan->d;
-    const int nrg = (nrhs + kTileR - 1) / kTileR;
-    const size_t waves = (size_t)d.nVOX * nrg * plan->multi_a_waves;
-    if (waves == 0) return 0;
-    if (waves > (size_t)INT32_MAX - kWavesPerBlock) return fail(COMMIT_HIP_EUNSUPPORTED, "too many voxels x RHS groups");
-    const unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    const uint32_t *ev = plan->eval_ones ? nullptr : d.eval;
+    if (L->m != (int64_t)d.nVOX * d.nS) return fail(COMMIT_HIP_EINVAL, "layout rows %lld != plan rows", (long long)L->m);
+    if (L->nprob < 1 || L->nypieces < 1 || L->nypieces > INT32_MAX || !L->ypieces || !L->ypseg)
+        return fail(COMMIT_HIP_EINVAL, "layout pieces missing");
+    HIP_TRY(hipSetDevice(plan->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const float *x = static_cast<const float *>(X);
-    float *y = static_cast<float *>(Y);
-    // one or two wavefronts per (voxel, RHS group); with two, each owns half
-    // of the sample tiles (NT rounded up to even; the padded LUT rows hold
-    // zeros there).  The choice is a plan option (commit_hip_plan_set_option).
-    if (plan->multi_a_waves == 2) {
-        switch ((nt_of(d) + 1) / 2) {
-#define CASE(NH)                                                                                            \
-    case NH:                                                                                                \
-        hipLaunchKernelGGL((k_A_mm<NH, 2>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev);     \
-        break;
-            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-#undef CASE
-            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
-        }
-    } else {
-        switch (nt_of(d)) {
-#define CASE(NT)                                                                                            \
-    case NT:                                                                                                \
-        hipLaunchKernelGGL((k_A_mm<NT, 1>), dim3(grid), dim3(kBlock), 0, st, d, x, y, nrhs, nrg, ev);     \
-        break;
-            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-            CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
-#undef CASE
-            default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
-        }
-    }
+    const int R = L->nrhs, nrg = (R + kTileR - 1) / kTileR;
+    double *part = work, *pp = work + (size_t)d.nVOX * 2 * nrg * kTileR;
+    const ResArgs ra{y, y_per_column ? 1 : 0, sqrtw, res, part};
+    if ((rc = launch_A_mm<true>(plan, X, nullptr, R, ra, st))) return rc;
+    hipLaunchKernelGGL(k_res_pieces, dim3((unsigned)L->nypieces), dim3(kBlock), 0, st, part, L->ypieces, d.nS, d.nVOX,
+                       plan->multi_a_waves, nrg, R, pp);
+    hipLaunchKernelGGL(k_res_problems, dim3(1), dim3(kBlock), 0, st, pp, L->ypseg, (int)L->nypieces, L->nprob, R, rsq);
     HIP_TRY(hipGetLastError());
     return 0;
 }

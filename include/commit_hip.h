@@ -310,6 +310,18 @@ int commit_hip_fista_multi_sums(const commit_hip_prox_params *p, const commit_hi
 int commit_hip_multi_residual(const commit_hip_multi_layout *L, const float *AX, const float *y, int y_per_column,
                               const float *sqrtw, float *res, double *rsq, double *work, void *stream);
 
+/* commit_hip_A_multi and commit_hip_multi_residual in one pass over the rows:
+ * res = sqrtw .* (A X - y) (res may be NULL: norms only; y NULL: zero;
+ * sqrtw NULL: one), rsq[problem * nrhs + column] = ||res||^2, without the
+ * [rows][nrhs] product A X in memory.  res equals the two-call result bit for
+ * bit; rsq is summed per voxel first (FP64, deterministic).  A voxel counts
+ * towards the problem holding its first row, so problem row ranges must begin
+ * and end on voxel boundaries.  work: commit_hip_A_multi_residual_work doubles. */
+int64_t commit_hip_A_multi_residual_work(const commit_hip_plan *plan, const commit_hip_multi_layout *L);
+int commit_hip_A_multi_residual(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *X,
+                                const float *y, int y_per_column, const float *sqrtw, float *res, double *rsq,
+                                double *work, void *stream);
+
 /* accept / backtrack / stop each fit after a trial (sums of the trial x, rsq of A x - y) */
 int commit_hip_fista_multi_control(const commit_hip_prox_params *p, const commit_hip_multi_layout *L, double tol_fun,
                                    double tol_x, int max_iter, const double *sums, const double *rsq,

@@ -104,3 +104,51 @@ def test_graph_replay_equals_eager(stack):
     for s in range(len(da)):
         for r in range(len(da[s])):
             assert da[s][r] == db[s][r]
+
+
+@pytest.mark.parametrize("waves", [1, 2])
+@pytest.mark.parametrize("R", [4, 16])
+def test_fused_residual_equals_two_calls(stack, waves, R):
+    """commit_hip_A_multi_residual (A X and the residual in one pass) against
+    commit_hip_A_multi followed by commit_hip_multi_residual: res bitwise
+    equal, the per-fit squared norms to 1e-13 (FP64, another summation order);
+    y shared or per column, weighted or not, norms only, y = 0."""
+    import ctypes
+    from commit_amd import _lib, autotune
+    probs, C, y, gidx, gw = stack
+    A = C.operator()
+    autotune.set_option(A, _lib.OPT_MULTI_A_WAVES, waves)
+    lib = _lib.load()
+    rp = _rp(C.nIC, C.F, C.nEC, C.E, C.nISO, C.I, gidx, gw)
+    lam = np.tile(np.linspace(1e-4, 1e-2, R), (C.S, 1))
+    rng = np.random.default_rng(10 * waves + R)
+    P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    X = torch.from_numpy(rng.standard_normal((C.n, R)).astype(np.float32)).cuda()
+    AX = A.matmat(X)
+    for ypc, weighted in ((0, False), (0, True), (1, True)):
+        yv = y if ypc == 0 else rng.random((C.m, R))
+        w = rng.uniform(0.5, 1.5, C.m) if weighted else None
+        bf = batched.BatchedFista(A, yv, rp, lam, R, layout=C, sqrt_W=w, max_iter=2)
+        nq = C.S * R
+        for ydev in (bf.y, None):
+            two = torch.empty((C.m, R), dtype=torch.float32, device="cuda")
+            rsq2 = torch.empty(nq, dtype=torch.float64, device="cuda")
+            work = torch.empty(bf.L.nypieces * R + 1, dtype=torch.float64, device="cuda")
+            yz = ydev if ydev is not None else torch.zeros(C.m, dtype=torch.float32, device="cuda")
+            _lib.check(lib.commit_hip_multi_residual(ctypes.byref(bf.L), P(AX), P(yz), ypc if ydev is not None else 0,
+                                                     P(bf.sw), P(two), P(rsq2), P(work), st), "multi_residual")
+            one = torch.full((C.m, R), 7.0, dtype=torch.float32, device="cuda")
+            rsq1 = torch.empty(nq, dtype=torch.float64, device="cuda")
+            bf._resid(X, ydev, ypc if ydev is not None else 0, one, rsq1)
+            assert torch.equal(one, two), (ypc, weighted)
+            a, b = rsq1.cpu().numpy(), rsq2.cpu().numpy()
+            assert np.max(np.abs(a - b) / b) <= 1e-13, (ypc, weighted, np.max(np.abs(a - b) / b))
+            rsq0 = torch.empty(nq, dtype=torch.float64, device="cuda")
+            bf._resid(X, ydev, ypc if ydev is not None else 0, None, rsq0)      # norms only
+            assert torch.equal(rsq0, rsq1)
+            r64 = two.double().cpu().numpy()
+            for s in range(C.S):
+                rows = C.rows(s)
+                ref = (r64[rows] ** 2).sum(axis=0)
+                assert np.allclose(a[s * R:(s + 1) * R], ref, rtol=1e-6)
