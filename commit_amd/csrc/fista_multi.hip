@@ -82,36 +82,63 @@ __global__ __launch_bounds__(kBlock) void k_mprox_elem(MProx P, Seg G, int64_t n
     }
 }
 
-// Group shrinkage (proximals.pyx:103-134): one wavefront per group; lane l
-// takes column l % 16 of the members l/16, l/16 + 4, ..., so each member's row
-// of R floats is read coalesced.  Groups are disjoint (checked on the host),
-// each inside one problem (gseg).
-__global__ __launch_bounds__(kBlock) void k_mprox_group(MProx P, Seg G, float *__restrict__ x,
-                                                        const Col *__restrict__ col)
+// Group shrinkage (proximals.pyx:103-134) and group norms (proximals.pyx:80-99):
+// one block of kGroupBlock threads per group; thread t takes column t % 16 of
+// the members t/16, t/16 + 64, ... (each member's row of R floats is read
+// coalesced, four members per round in flight), the 64 partial sums of a
+// column are added in order in LDS.  Groups are disjoint (checked on the
+// host), each inside one problem (gseg).
+constexpr int kGroupBlock = 1024;
+constexpr int kGroupSubs = kGroupBlock / 16;
+
+// sum over the group's members of x[member][r]^2 for r = threadIdx.x % 16,
+// returned to every thread of that column
+__device__ __forceinline__ double group_sumsq(const MProx &P, int64_t g, int R, const float *__restrict__ x,
+                                              double (*sh)[16])
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t g = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)));
-    if (g >= P.ngroups) return;
-    const int R = G.R;
-    const int r = lane & 15, sub = lane >> 4;
-    const Col *c = col + (size_t)P.gseg[g] * R + (r < R ? r : 0);
-    const bool act = r < R && !c->done;
+    const int r = threadIdx.x & 15, sub = threadIdx.x >> 4;
+    const int rr = r < R ? r : 0;
     const int b = P.gptr[g], e = P.gptr[g + 1];
     double s = 0.0;
-    if (act)
-        for (int k = b + sub; k < e; k += 4) {
-            const double v = (double)x[(size_t)P.gidx[k] * R + r];
-            s = fma(v, v, s);
-        }
-    s += __shfl_xor(s, 16);
-    s += __shfl_xor(s, 32);
-    if (!act) return;
+    int k = b + sub;
+    for (; k + 3 * kGroupSubs < e; k += 4 * kGroupSubs) {
+        int id[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) id[j] = P.gidx[k + j * kGroupSubs];
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = x[(size_t)id[j] * R + rr];
+#pragma unroll
+        for (int j = 0; j < 4; j++) s = fma((double)v[j], (double)v[j], s);
+    }
+    for (; k < e; k += kGroupSubs) {
+        const double v = (double)x[(size_t)P.gidx[k] * R + rr];
+        s = fma(v, v, s);
+    }
+    sh[sub][r] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int j = 0; j < kGroupSubs; j++) t += sh[j][r];
+    return t;
+}
+
+__global__ __launch_bounds__(kGroupBlock) void k_mprox_group(MProx P, Seg G, float *__restrict__ x,
+                                                             const Col *__restrict__ col)
+{
+    __shared__ double sh[kGroupSubs][16];
+    const int64_t g = blockIdx.x;
+    const int R = G.R;
+    const int r = threadIdx.x & 15, sub = threadIdx.x >> 4;
+    const double s = group_sumsq(P, g, R, x, sh);
+    if (r >= R) return;
+    const Col *c = col + (size_t)P.gseg[g] * R + r;
+    if (c->done) return;
     const double lam = c->lam_group * c->step;
     if (lam == 0.0) return;                                               // (proximals.pyx:115)
     const double nrm = sqrt(s);
     const double wl = P.gw[g] * lam;
     const double f = (nrm <= wl) ? 0.0 : (nrm - wl) / nrm;
-    for (int k = b + sub; k < e; k += 4) {
+    for (int k = P.gptr[g] + sub; k < P.gptr[g + 1]; k += kGroupSubs) {
         float *p = x + (size_t)P.gidx[k] * R + r;
         *p = (f == 0.0) ? 0.0f : (float)((double)*p * f);
     }
@@ -130,18 +157,39 @@ __global__ __launch_bounds__(kBlock) void k_mnn_ic(MProx P, Seg G, float *__rest
     }
 }
 
-// Sums over the unknowns, stage 1: block p reduces piece p (rows [pb, pe) of
-// one problem) for all R columns; thread t takes column t % R and rows
-// pb + t/R, pb + t/R + L, ... (L = 256 / R); the L partials of a column are
-// then added in order.
+// Sums over the unknowns, stage 1: block (p, j) reduces part j of kSumSplit
+// of piece p (rows [pb, pe) of one problem) for all R columns; thread t takes
+// column t % R and rows b + t/R, b + t/R + L, ... (L = 256 / R, four rows per
+// round in flight); the L partials of a column are then added in order.
 //   s0 = sum (x - xhat) grad   s1 = sum (x - xhat)^2   s2 = sum (x - prev)^2
 //   s3 = sum x^2               s4..s6 = l1 terms of the three compartments
+constexpr int kSumSplit = 8;
+
 struct MSum {
     const float *x, *xhat, *grad, *prev;
     const double *w;
     int64_t l1b[3], l1e[3];
     int l1w[3];
 };
+
+__device__ __forceinline__ void msum_row(const MSum &a, int64_t i, float xf, float hf, float gf, float pf,
+                                         double (&s)[7])
+{
+    const double xi = (double)xf;
+    if (a.xhat) {
+        const double d = xi - (double)hf;
+        s[0] = fma(d, (double)gf, s[0]);
+        s[1] = fma(d, d, s[1]);
+    }
+    if (a.prev) {
+        const double d = xi - (double)pf;
+        s[2] = fma(d, d, s[2]);
+    }
+    s[3] = fma(xi, xi, s[3]);
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (i >= a.l1b[k] && i < a.l1e[k]) s[4 + k] += fabs(a.l1w[k] ? a.w[i] * xi : xi);
+}
 
 __global__ __launch_bounds__(kBlock) void k_msums_stage1(MSum a, int R, const int64_t *__restrict__ pieces,
                                                          double *__restrict__ part)
@@ -150,25 +198,28 @@ __global__ __launch_bounds__(kBlock) void k_msums_stage1(MSum a, int R, const in
     const int L = kBlock / R;
     const int tid = threadIdx.x;
     const int r = tid % R, lane0 = tid / R;
-    const int64_t b = pieces[2 * blockIdx.x], e = pieces[2 * blockIdx.x + 1];
+    const int64_t pb = pieces[2 * blockIdx.x], pe = pieces[2 * blockIdx.x + 1];
+    const int64_t per = (pe - pb + kSumSplit - 1) / kSumSplit;
+    const int64_t b = pb + per * blockIdx.y, e = b + per < pe ? b + per : pe;
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
     if (lane0 < L) {
-        for (int64_t i = b + lane0; i < e; i += L) {
-            const size_t t = (size_t)i * R + r;
-            const double xi = (double)a.x[t];
-            if (a.xhat) {
-                const double d = xi - (double)a.xhat[t];
-                s[0] = fma(d, (double)a.grad[t], s[0]);
-                s[1] = fma(d, d, s[1]);
-            }
-            if (a.prev) {
-                const double d = xi - (double)a.prev[t];
-                s[2] = fma(d, d, s[2]);
-            }
-            s[3] = fma(xi, xi, s[3]);
+        int64_t i = b + lane0;
+        for (; i + 3 * L < e; i += 4 * L) {
+            float xv[4], hv[4] = {0, 0, 0, 0}, gv[4] = {0, 0, 0, 0}, pv[4] = {0, 0, 0, 0};
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-                if (i >= a.l1b[k] && i < a.l1e[k]) s[4 + k] += fabs(a.l1w[k] ? a.w[i] * xi : xi);
+            for (int j = 0; j < 4; j++) {
+                const size_t t = (size_t)(i + j * L) * R + r;
+                xv[j] = a.x[t];
+                if (a.xhat) hv[j] = a.xhat[t], gv[j] = a.grad[t];
+                if (a.prev) pv[j] = a.prev[t];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) msum_row(a, i + j * L, xv[j], hv[j], gv[j], pv[j], s);
+        }
+        for (; i < e; i += L) {
+            const size_t t = (size_t)i * R + r;
+            msum_row(a, i, a.x[t], a.xhat ? a.xhat[t] : 0.0f, a.xhat ? a.grad[t] : 0.0f,
+                     a.prev ? a.prev[t] : 0.0f, s);
         }
     }
 #pragma unroll
@@ -178,50 +229,45 @@ __global__ __launch_bounds__(kBlock) void k_msums_stage1(MSum a, int R, const in
         const int rr = tid % R, k = tid / R;
         double v = 0.0;
         for (int l = 0; l < L; l++) v += sh[l * R + rr][k];
-        part[((size_t)blockIdx.x * R + rr) * 7 + k] = v;
+        part[(((size_t)blockIdx.x * kSumSplit + blockIdx.y) * R + rr) * 7 + k] = v;
     }
 }
 
 // gn[g*R + r] = group_w[g] * ||x_g[:, r]||   (proximals.pyx:80-99)
-__global__ __launch_bounds__(kBlock) void k_mgroup_norm(MProx P, int R, const float *__restrict__ x,
-                                                        double *__restrict__ gn)
+__global__ __launch_bounds__(kGroupBlock) void k_mgroup_norm(MProx P, int R, const float *__restrict__ x,
+                                                             double *__restrict__ gn)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t g = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)));
-    if (g >= P.ngroups) return;
-    const int r = lane & 15, sub = lane >> 4;
-    double s = 0.0;
-    if (r < R)
-        for (int k = P.gptr[g] + sub; k < P.gptr[g + 1]; k += 4) {
-            const double v = (double)x[(size_t)P.gidx[k] * R + r];
-            s = fma(v, v, s);
-        }
-    s += __shfl_xor(s, 16);
-    s += __shfl_xor(s, 32);
-    if (sub == 0 && r < R) gn[(size_t)g * R + r] = P.gw[g] * sqrt(s);
+    __shared__ double sh[kGroupSubs][16];
+    const int64_t g = blockIdx.x;
+    const double s = group_sumsq(P, g, R, x, sh);
+    if (threadIdx.x < 16 && (int)threadIdx.x < R) gn[(size_t)g * R + threadIdx.x] = P.gw[g] * sqrt(s);
 }
 
-// Stage 2: out[(s*R + r)*8 + k], one thread per (problem, column, term): the
-// pieces of problem s in piece order (k < 7), its groups in group order (k = 7).
+// Stage 2: out[(s*R + r)*8 + k], one wavefront per (problem, column, term):
+// lane l adds the parts l, l + 64, ... of problem s (pieces x splits, k < 7)
+// or its groups (k = 7) in order, then the 64 lane sums are added by a fixed
+// butterfly.
 __global__ __launch_bounds__(kBlock) void k_msums_stage2(const double *__restrict__ part,
                                                          const int32_t *__restrict__ pseg, int npieces,
                                                          const double *__restrict__ gn,
                                                          const int32_t *__restrict__ gseg, int64_t ngroups, int S,
                                                          int R, double *__restrict__ out)
 {
-    const int total = S * R * kNSum;
-    for (int t = threadIdx.x; t < total; t += kBlock) {
-        const int k = t % kNSum, sr = t / kNSum, r = sr % R, s = sr / R;
-        double v = 0.0;
-        if (k < 7) {
-            for (int p = 0; p < npieces; p++)
-                if (pseg[p] == s) v += part[((size_t)p * R + r) * 7 + k];
-        } else {
-            for (int64_t g = 0; g < ngroups; g++)
-                if (gseg[g] == s) v += gn[(size_t)g * R + r];
-        }
-        out[t] = v;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = (int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    if (t >= S * R * kNSum) return;
+    const int k = t % kNSum, sr = t / kNSum, r = sr % R, s = sr / R;
+    double v = 0.0;
+    if (k < 7) {
+        for (int q = lane; q < npieces * kSumSplit; q += kWave)
+            if (pseg[q / kSumSplit] == s) v += part[((size_t)q * R + r) * 7 + k];
+    } else {
+        for (int64_t g = lane; g < ngroups; g += kWave)
+            if (gseg[g] == s) v += gn[(size_t)g * R + r];
     }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) out[t] = v;
 }
 
 // res = sqrtw .* (AX - y) over the row pieces (y: [m] shared by the columns
@@ -437,7 +483,7 @@ int64_t commit_hip_fista_multi_work(const commit_hip_multi_layout *L, int64_t ng
 {
     if (!L) return -1;
     const int64_t R = L->nrhs;
-    return L->nxpieces * R * 7 + ngroups * R;
+    return L->nxpieces * kSumSplit * R * 7 + ngroups * R;
 }
 
 int commit_hip_prox_multi(const commit_hip_prox_params *p, const commit_hip_multi_layout *L, const float *xhat,
@@ -457,7 +503,7 @@ int commit_hip_prox_multi(const commit_hip_prox_params *p, const commit_hip_mult
     const int r0 = a.reg[0];
     if (r0 == COMMIT_HIP_REG_GROUP_LASSO || r0 == COMMIT_HIP_REG_SPARSE_GROUP_LASSO) {
         if (a.ngroups > 0)
-            hipLaunchKernelGGL(k_mprox_group, dim3(wave_grid(a.ngroups)), dim3(kBlock), 0, st, a, G, x, col);
+            hipLaunchKernelGGL(k_mprox_group, dim3((unsigned)a.ngroups), dim3(kGroupBlock), 0, st, a, G, x, col);
         if (a.nn[0] && a.end[0] > a.start[0])
             hipLaunchKernelGGL(k_mnn_ic, dim3(elem_grid((a.end[0] - a.start[0]) * L->nrhs)), dim3(kBlock), 0, st, a,
                                G, x, col);
@@ -487,11 +533,13 @@ int commit_hip_fista_multi_sums(const commit_hip_prox_params *p, const commit_hi
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int R = L->nrhs;
     double *part = work;
-    double *gn = part + (size_t)L->nxpieces * R * 7;
-    hipLaunchKernelGGL(k_msums_stage1, dim3((unsigned)L->nxpieces), dim3(kBlock), 0, st, s, R, L->xpieces, part);
-    if (a.ngroups > 0) hipLaunchKernelGGL(k_mgroup_norm, dim3(wave_grid(a.ngroups)), dim3(kBlock), 0, st, a, R, x, gn);
-    hipLaunchKernelGGL(k_msums_stage2, dim3(1), dim3(kBlock), 0, st, part, L->xpseg, (int)L->nxpieces, gn, L->gseg,
-                       a.ngroups, L->nprob, R, out);
+    double *gn = part + (size_t)L->nxpieces * kSumSplit * R * 7;
+    hipLaunchKernelGGL(k_msums_stage1, dim3((unsigned)L->nxpieces, kSumSplit), dim3(kBlock), 0, st, s, R, L->xpieces,
+                       part);
+    if (a.ngroups > 0)
+        hipLaunchKernelGGL(k_mgroup_norm, dim3((unsigned)a.ngroups), dim3(kGroupBlock), 0, st, a, R, x, gn);
+    hipLaunchKernelGGL(k_msums_stage2, dim3(wave_grid((int64_t)L->nprob * R * kNSum)), dim3(kBlock), 0, st, part,
+                       L->xpseg, (int)L->nxpieces, gn, L->gseg, a.ngroups, L->nprob, R, out);
     HIP_TRY(hipGetLastError());
     return 0;
 }
