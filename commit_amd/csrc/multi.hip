@@ -73,22 +73,29 @@ __device__ __forceinline__ void load_run(float (&a)[NT], const float *__restrict
     }
 }
 
-// ------------------------------------------------------------- Y = A X ------
-// Entries are taken in chunks of 64 (one per lane: orientation, column,
-// length).  K-step t of a chunk covers entries 4t..4t+3; lane group g owns
-// entry 4t+g.  At the start of a chunk every lane gathers the coefficients of
-// all its 16 steps (one latency for the chunk) and the next chunk's entry
-// records are requested; the LUT rows then stream two steps ahead of the
-// matrix-core work.
-struct StepIn {
-    const uint32_t *eval;       // IC: eval (null: all ones)
-    const float *X;             // X + rhs (column 0 of the compartment's RF 0 block)
-    const float *L;             // LUT of the RF, + this lane's sample offset
-    size_t xs;                  // column stride between RF blocks
-    int R, nSp, nF, i;
-    bool rhs_ok;
+// The rows of voxel v's product in one list: IC entries for each IC RF, then
+// EC peaks for each EC RF, then ISO entries for each ISO RF.
+struct VRows {
+    uint32_t icb, nic, ecb, nec, isob, niso;   // the voxel's entry ranges
+    uint32_t n1, n2, total;                    // row counts: IC, IC+EC, all
 };
 
+__device__ __forceinline__ VRows vrows(const DevDict &d, int v)
+{
+    VRows vr;
+    vr.icb = d.nIC > 0 ? d.ic.ptr[v] : 0u;
+    vr.nic = d.nIC > 0 ? d.ic.ptr[v + 1] - vr.icb : 0u;
+    vr.ecb = d.nEC > 0 ? d.ec.ptr[v] : 0u;
+    vr.nec = d.nEC > 0 ? d.ec.ptr[v + 1] - vr.ecb : 0u;
+    vr.isob = d.nISO > 0 ? d.iso.ptr[v] : 0u;
+    vr.niso = d.nISO > 0 ? d.iso.ptr[v + 1] - vr.isob : 0u;
+    vr.n1 = (uint32_t)d.nIC * vr.nic;
+    vr.n2 = vr.n1 + (uint32_t)d.nEC * vr.nec;
+    vr.total = vr.n2 + (uint32_t)d.nISO * vr.niso;
+    return vr;
+}
+
+// ------------------------------------------------------------- Y = A X ------
 template <int NT>
 __device__ __forceinline__ void a_mma(f4 (&acc)[NT], const float (&a)[NT], float b)
 {
@@ -96,91 +103,105 @@ __device__ __forceinline__ void a_mma(f4 (&acc)[NT], const float (&a)[NT], float
     for (int m = 0; m < NT; m++) acc[m] = mfma4(a[m], b, acc[m]);
 }
 
-__device__ __forceinline__ void entry_rec(const Comp &cp, uint32_t k, uint32_t e, uint32_t &o, uint32_t &col,
-                                          float &len)
+// Start of this lane's run in the LUT row coded as in entry_any.
+__device__ __forceinline__ const float *row_code(const DevDict &d, uint32_t c, size_t lane_off)
 {
-    o = 0, col = 0, len = 1.0f;
-    if (k < e) {
-        if (cp.o) o = __builtin_nontemporal_load(cp.o + k);
-        col = __builtin_nontemporal_load(cp.col + k);
-        if (cp.len) len = __builtin_nontemporal_load(cp.len + k);
-    }
+    const uint32_t tb = c >> 30, row = c & 0x3fffffffu;
+    const float *w0 = d.wmr, *w1 = d.wmh, *w2 = d.isol;
+    const float *L = tb == 0u ? w0 : tb == 1u ? w1 : w2;
+    return L + (size_t)row * d.nSp + lane_off;
 }
 
-// Coefficients of a chunk's 16 K-steps for this lane: entry 4t+g, RHS column
-// of the lane.  Straight-line code (one version with eval, one without) so
-// that the 16 gathers are in flight together; entries past the chunk and RHS
-// columns past R read column 0 and are zeroed after.
+// Entry r (< vr.total) of the voxel's list for A X: its LUT row as a code
+// (table in the top 2 bits: 0 IC, 1 EC, 2 ISO; row index below), its x
+// column and its scale (IC: length x eval; EC, ISO: 1).
 template <bool EV>
-__device__ __forceinline__ void gather16(float (&xv)[16], const StepIn &in, uint32_t col, float len, int g, int cnt,
-                                         int i)
+__device__ __forceinline__ void entry_any(const DevDict &d, const VRows &vr, uint32_t r, const uint32_t *eval,
+                                          uint32_t &code, uint32_t &xcol, float &scale)
 {
-    float lv[16];
-    uint32_t ev[16];
-    bool okv[16];
-#pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int ent = 4 * t + g;
-        const uint32_t ce = (uint32_t)__shfl((int)col, ent);
-        lv[t] = __shfl(len, ent);
-        okv[t] = ent < cnt && in.rhs_ok;
-        const uint32_t cs = okv[t] ? ce : 0u;
-        xv[t] = in.X[((size_t)cs + (size_t)i * in.xs) * (size_t)in.R];
-        if (EV) ev[t] = in.eval[cs + (size_t)i * in.nF];
+    // (members copied to scalars before the selects, so the selects stay on values)
+    const uint32_t n1 = vr.n1, n2 = vr.n2, nic = vr.nic, nec = vr.nec, niso = vr.niso;
+    const uint32_t icb = vr.icb, ecb = vr.ecb, isob = vr.isob;
+    const int rfic = d.nIC, rfec = d.nEC, rfiso = d.nISO;
+    r = min(r, vr.total - 1);
+    const bool ic = r < n1, ec = !ic && r < n2;
+    const uint32_t t = ic ? r : (ec ? r - n1 : r - n2);
+    const uint32_t m = ic ? nic : (ec ? nec : niso);
+    const int nrf = ic ? rfic : (ec ? rfec : rfiso);
+    const uint32_t i = nrf == 1 ? 0u : t / m;
+    const uint32_t k = (ic ? icb : (ec ? ecb : isob)) + (t - i * m);
+    if (ic) {
+        const uint32_t o = d.ic.o[k], f = d.ic.col[k];
+        code = i * (uint32_t)d.ndirs + o;
+        xcol = i * (uint32_t)d.nF + f;
+        scale = d.ic.len[k];
+        if (EV && eval[xcol] == 0u) scale = 0.0f;
+    } else if (ec) {
+        const uint32_t o = d.ec.o[k];
+        code = (1u << 30) | (i * (uint32_t)d.ndirs + o);
+        xcol = (uint32_t)d.nIC * (uint32_t)d.nF + i * (uint32_t)d.nE + d.ec.col[k];
+        scale = 1.0f;
+    } else {
+        code = (2u << 30) | i;
+        xcol = (uint32_t)d.nIC * (uint32_t)d.nF + (uint32_t)d.nEC * (uint32_t)d.nE + i * (uint32_t)d.nI + d.iso.col[k];
+        scale = 1.0f;
     }
-#pragma unroll
-    for (int t = 0; t < 16; t++) xv[t] = (okv[t] && (!EV || ev[t] != 0u)) ? lv[t] * xv[t] : 0.0f;
 }
 
-__device__ __forceinline__ const float *row_of(const StepIn &in, uint32_t oc, int ent, int cnt)
+// A X for one voxel (setup_operator.py:48-242): all its entries (IC for each
+// RF, EC, ISO) in chunks of 64, so a voxel pays one start-up (records,
+// coefficient gathers, first rows) per chunk instead of one per compartment.
+// K-step t of a chunk covers entries 4t..4t+3; lane group g owns entry 4t+g.
+// At a chunk start every lane gathers the coefficients of its 16 K-steps and
+// the next chunk's entry records are requested; the LUT rows stream two steps
+// ahead of the matrix-core work.  NH: the sample tiles this wavefront owns
+// (of NT = NH * MS); soff: offset of its first sample in each lane's run.
+template <int NH, int NT, bool EV>
+__device__ __forceinline__ void mm_vox_A(f4 (&acc)[NH], const DevDict &d, const VRows &vr,
+                                         const float *__restrict__ X, int R, int rc, bool rhs_ok,
+                                         const uint32_t *__restrict__ eval, int lane, int soff)
 {
-    const uint32_t oe = (uint32_t)__shfl((int)oc, ent);
-    return in.L + (size_t)(ent < cnt ? oe : 0u) * in.nSp;
-}
-
-// NH: the sample tiles this wavefront owns (of NT = NH * MS); soff: offset of
-// its first sample inside each lane's run of NT.
-template <int NH, int NT>
-__device__ __forceinline__ void mm_comp_A(f4 (&acc)[NH], const Comp &cp, uint32_t b, uint32_t e, StepIn in,
-                                          int nrf, const float *__restrict__ lut, size_t lut_rf, int lane, int soff)
-{
+    if (vr.total == 0) return;
     const int g = lane >> 4, q = lane & 15;
-    for (int i = 0; i < nrf; i++) {
-        in.i = i;
-        in.L = lut + (size_t)i * lut_rf + (size_t)q * NT + soff;
-        uint32_t o, col;
-        float len;
-        entry_rec(cp, b + lane, e, o, col, len);
-        for (uint32_t base = b; base < e; base += kWave) {
-            const int cnt = (int)min((uint32_t)kWave, e - base);
-            const int nsteps = (cnt + 3) >> 2;
-            // coefficients l * x * eval of this lane's entries     (setup_operator.py:63-96)
-            // Branch-free: every lane loads (entries past the chunk, and RHS
-            // columns past R, read column 0 of the block) so that the 16
-            // gathers are in flight together; invalid ones are zeroed after.
-            float xv[16];
-            if (in.eval)
-                gather16<true>(xv, in, col, len, g, cnt, i);
-            else
-                gather16<false>(xv, in, col, len, g, cnt, i);
-            const uint32_t oc = o;
-            if (base + kWave < e) entry_rec(cp, base + kWave + lane, e, o, col, len);
-            float a0[NH], a1[NH];
-            // (the shuffle runs on every lane; entries past the chunk read row 0)
-#define ROW(t) row_of(in, oc, 4 * (t) + g, cnt)
-            load_run<NH>(a0, ROW(0));
-            if (nsteps > 1) load_run<NH>(a1, ROW(1));
+    const size_t lane_off = (size_t)q * NT + soff;
+    uint32_t code, xcol;
+    float scale;
+    entry_any<EV>(d, vr, (uint32_t)lane, eval, code, xcol, scale);
+    for (uint32_t base = 0; base < vr.total; base += kWave) {
+        const int cnt = (int)min((uint32_t)kWave, vr.total - base);
+        const int nsteps = (cnt + 3) >> 2;
+        // coefficients scale * x of this lane's entries, all 16 gathers in flight
+        float xv[16];
+        {
+            float sv[16];
+            bool okv[16];
 #pragma unroll
-            for (int t = 0; t < 16; t += 2) {
-                a_mma<NH>(acc, a0, xv[t]);
-                if (t + 1 >= nsteps) break;
-                if (t + 2 < nsteps) load_run<NH>(a0, ROW(t + 2));
-                a_mma<NH>(acc, a1, xv[t + 1]);
-                if (t + 2 >= nsteps) break;
-                if (t + 3 < nsteps) load_run<NH>(a1, ROW(t + 3));
+            for (int t = 0; t < 16; t++) {
+                const int ent = 4 * t + g;
+                const uint32_t ce = (uint32_t)__shfl((int)xcol, ent);
+                sv[t] = __shfl(scale, ent);
+                okv[t] = ent < cnt && rhs_ok;
+                xv[t] = X[(size_t)(okv[t] ? ce : 0u) * (size_t)R + rc];
             }
-#undef ROW
+#pragma unroll
+            for (int t = 0; t < 16; t++) xv[t] = okv[t] ? sv[t] * xv[t] : 0.0f;
         }
+        const uint32_t cc = code;
+        if (base + kWave < vr.total) entry_any<EV>(d, vr, base + kWave + lane, eval, code, xcol, scale);
+        float a0[NH], a1[NH];
+#define ROW(t) row_code(d, (uint32_t)__shfl((int)cc, 4 * (t) + g), lane_off)
+        load_run<NH>(a0, ROW(0));
+        if (nsteps > 1) load_run<NH>(a1, ROW(1));
+#pragma unroll
+        for (int t = 0; t < 16; t += 2) {
+            a_mma<NH>(acc, a0, xv[t]);
+            if (t + 1 >= nsteps) break;
+            if (t + 2 < nsteps) load_run<NH>(a0, ROW(t + 2));
+            a_mma<NH>(acc, a1, xv[t + 1]);
+            if (t + 2 >= nsteps) break;
+            if (t + 3 < nsteps) load_run<NH>(a1, ROW(t + 3));
+        }
+#undef ROW
     }
 }
 
@@ -262,19 +283,12 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
     f4 acc[NH];
 #pragma unroll
     for (int m = 0; m < NH; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const size_t lut_rf = (size_t)d.ndirs * d.nSp;
     const int rc = rhs < R ? rhs : 0;                      // columns past R read column 0 (zeroed)
-    StepIn in{eval, X + rc, nullptr, (size_t)d.nF, R, d.nSp, d.nF, 0, rhs < R};
-    mm_comp_A<NH, NT>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], in, d.nIC, d.wmr, lut_rf, lane, soff);
-    if (d.nEC > 0) {
-        StepIn ie{nullptr, X + (size_t)d.nIC * d.nF * R + rc, nullptr, (size_t)d.nE, R, d.nSp, d.nF, 0, rhs < R};
-        mm_comp_A<NH, NT>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], ie, d.nEC, d.wmh, lut_rf, lane, soff);
-    }
-    if (d.nISO > 0) {
-        StepIn is{nullptr, X + ((size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE) * R + rc, nullptr, (size_t)d.nI, R,
-                  0, d.nF, 0, rhs < R};
-        mm_comp_A<NH, NT>(acc, d.iso, d.iso.ptr[v], d.iso.ptr[v + 1], is, d.nISO, d.isol, (size_t)d.nSp, lane, soff);
-    }
+    const VRows vr = vrows(d, v);
+    if (eval)
+        mm_vox_A<NH, NT, true>(acc, d, vr, X, R, rc, rhs < R, eval, lane, soff);
+    else
+        mm_vox_A<NH, NT, false>(acc, d, vr, X, R, rc, rhs < R, eval, lane, soff);
     if constexpr (!RES) {
         if (rhs >= R) return;
         float *yv = Y + (size_t)v * d.nS * R + rhs;
@@ -363,10 +377,6 @@ __global__ __launch_bounds__(kBlock) void k_res_problems(const double *__restric
 //   EC row (RF i, peak k)  : X[(nIC*nF + i*nE + col)*R + rhs]            = dot
 //   ISO row (RF i, entry k): X[(nIC*nF + nEC*nE + i*nI + col)*R + rhs]   = dot
 
-struct VRows {
-    uint32_t icb, nic, ecb, nec, isob, niso;   // the voxel's entry ranges
-    uint32_t n1, n2, total;                    // row counts: IC, IC+EC, all
-};
 
 // Row r (< total) of the voxel: its LUT row, and its output slot and scale
 // (IC: slot i*n + k of z, scale len >= 0; EC/ISO: x column, scale -1).
@@ -520,16 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
 #pragma unroll
     for (int t = 0; t < 4 * NT; t++)
         if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
-    VRows vr;
-    vr.icb = d.ic.ptr[v];
-    vr.nic = d.nIC > 0 ? d.ic.ptr[v + 1] - vr.icb : 0u;
-    vr.ecb = d.nEC > 0 ? d.ec.ptr[v] : 0u;
-    vr.nec = d.nEC > 0 ? d.ec.ptr[v + 1] - vr.ecb : 0u;
-    vr.isob = d.nISO > 0 ? d.iso.ptr[v] : 0u;
-    vr.niso = d.nISO > 0 ? d.iso.ptr[v + 1] - vr.isob : 0u;
-    vr.n1 = (uint32_t)d.nIC * vr.nic;
-    vr.n2 = vr.n1 + (uint32_t)d.nEC * vr.nec;
-    vr.total = vr.n2 + (uint32_t)d.nISO * vr.niso;
+    const VRows vr = vrows(d, v);
     mm_vox_At<NT>(yb, d, vr, (size_t)d.ndirs * d.nSp, lane, z, X + rhs, R, rhs_ok, T);
 }
 
