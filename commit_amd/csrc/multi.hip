@@ -371,68 +371,22 @@ __global__ __launch_bounds__(kBlock) void k_res_problems(const double *__restric
 // and the 16 lanes of one operand read hit 16 consecutive slots (no bank
 // conflicts).  A tile's operands are read into registers, the next tile is
 // requested into the same LDS, and the matrix-core work of the current tile
-// runs under that load.
+// runs under that load.  The row records (LUT row, output slot, scale) come
+// from registers: each lane holds the records of one row of a 64-row window,
+// the next window loads four tiles ahead, and a tile's row pointers and slots
+// are shuffled out of the window, so issuing the next tile's copy waits on no
+// record load.
 // yb: this lane's B operand, Y_v[g*4NT + t][rhs] for t < 4NT.
 //   IC row (RF i, entry k) : z[(i*n + k)*16 + q]                        = len * dot
 //   EC row (RF i, peak k)  : X[(nIC*nF + i*nE + col)*R + rhs]            = dot
 //   ISO row (RF i, entry k): X[(nIC*nF + nEC*nE + i*nI + col)*R + rhs]   = dot
 
 
-// Row r (< total) of the voxel: its LUT row, and its output slot and scale
-// (IC: slot i*n + k of z, scale len >= 0; EC/ISO: x column, scale -1).
-struct RowRef {
-    const float *row;
-    uint32_t slot;
-    float scale;
-};
-
-__device__ __forceinline__ RowRef row_ref(const DevDict &d, const VRows &vr, uint32_t r, size_t lut_rf)
-{
-    if (r < vr.n1) {
-        const uint32_t i = d.nIC == 1 ? 0u : r / vr.nic, k = vr.icb + (r - i * vr.nic);
-        return RowRef{d.wmr + (size_t)i * lut_rf + (size_t)d.ic.o[k] * d.nSp, i * (uint32_t)d.n + k, d.ic.len[k]};
-    }
-    if (r < vr.n2) {
-        const uint32_t t = r - vr.n1, i = vr.nec == 1 ? t : t / vr.nec, k = vr.ecb + (t - i * vr.nec);
-        return RowRef{d.wmh + (size_t)i * lut_rf + (size_t)d.ec.o[k] * d.nSp,
-                      (uint32_t)d.nIC * (uint32_t)d.nF + i * (uint32_t)d.nE + d.ec.col[k], -1.0f};
-    }
-    const uint32_t t = r - vr.n2, i = vr.niso == 1 ? t : t / vr.niso, k = vr.isob + (t - i * vr.niso);
-    return RowRef{d.isol + (size_t)i * d.nSp,
-                  (uint32_t)d.nIC * (uint32_t)d.nF + (uint32_t)d.nEC * (uint32_t)d.nE + i * (uint32_t)d.nI +
-                      d.iso.col[k],
-                  -1.0f};
-}
-
-// Request the LUT rows of tile e0 (rows past the end repeat the last row).
-template <int NT>
-__device__ __forceinline__ void at_request(const DevDict &d, const VRows &vr, uint32_t e0, size_t lut_rf, int lane,
-                                           float *T)
-{
-    const uint32_t r = min(e0 + (uint32_t)(lane & 15), vr.total - 1);   // row lane%16 of the tile
-    const float *src = row_ref(d, vr, r, lut_rf).row + 4 * (lane >> 4);
-#pragma unroll
-    for (int u = 0; u < NT; u++)                                  // chunks 4u .. 4u+3 of every row
-        __builtin_amdgcn_global_load_lds(src + 16 * u, (lds_void *)(T + 256 * u), 16, 0, 0);
-}
-
 // Output slot and scale of the 4 D rows (e0+4g+k) a tile produces for this lane.
 struct Aux {
     uint32_t slot[4];
     float scale[4];
 };
-
-__device__ __forceinline__ Aux at_aux(const DevDict &d, const VRows &vr, uint32_t e0, int g, size_t lut_rf)
-{
-    Aux a;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const RowRef rr = row_ref(d, vr, min(e0 + 4 * g + k, vr.total - 1), lut_rf);
-        a.slot[k] = rr.slot;
-        a.scale[k] = rr.scale;
-    }
-    return a;
-}
 
 // Write the D tile of rows e0.. (deferred by one tile, so that the wait for
 // the next LUT tile does not also wait for these stores).
@@ -450,6 +404,76 @@ __device__ __forceinline__ void at_store(const f4 &acc, const Aux &a, uint32_t e
     }
 }
 
+// Record of row r (< vr.total) of the voxel's A^T list: LUT row code (table
+// in the top 2 bits: 0 IC, 1 EC, 2 ISO), output slot and scale (IC: slot
+// i*n + k of z, scale len; EC/ISO: x column, scale -1).
+__device__ __forceinline__ void at_rec(const DevDict &d, const VRows &vr, uint32_t r, uint32_t &code, uint32_t &slot,
+                                       float &scale)
+{
+    const uint32_t n1 = vr.n1, n2 = vr.n2, nic = vr.nic, nec = vr.nec, niso = vr.niso;
+    const uint32_t icb = vr.icb, ecb = vr.ecb, isob = vr.isob;
+    const int rfic = d.nIC, rfec = d.nEC, rfiso = d.nISO;
+    r = min(r, vr.total - 1);
+    const bool ic = r < n1, ec = !ic && r < n2;
+    const uint32_t t = ic ? r : (ec ? r - n1 : r - n2);
+    const uint32_t m = ic ? nic : (ec ? nec : niso);
+    const int nrf = ic ? rfic : (ec ? rfec : rfiso);
+    const uint32_t i = nrf == 1 ? 0u : t / m;
+    const uint32_t k = (ic ? icb : (ec ? ecb : isob)) + (t - i * m);
+    if (ic) {
+        code = i * (uint32_t)d.ndirs + d.ic.o[k];
+        slot = i * (uint32_t)d.n + k;
+        scale = d.ic.len[k];
+    } else if (ec) {
+        code = (1u << 30) | (i * (uint32_t)d.ndirs + d.ec.o[k]);
+        slot = (uint32_t)d.nIC * (uint32_t)d.nF + i * (uint32_t)d.nE + d.ec.col[k];
+        scale = -1.0f;
+    } else {
+        code = (2u << 30) | i;
+        slot = (uint32_t)d.nIC * (uint32_t)d.nF + (uint32_t)d.nEC * (uint32_t)d.nE + i * (uint32_t)d.nI + d.iso.col[k];
+        scale = -1.0f;
+    }
+}
+
+// Records of 64 consecutive rows, one per lane.
+struct RecWin {
+    uint32_t code, slot;
+    float scale;
+};
+
+__device__ __forceinline__ RecWin rec_win(const DevDict &d, const VRows &vr, uint32_t b, int lane)
+{
+    RecWin w{0u, 0u, 0.0f};
+    if (b < vr.total) at_rec(d, vr, b + (uint32_t)lane, w.code, w.slot, w.scale);
+    return w;
+}
+
+// Request the LUT rows of tile e0 (rows past the end repeat the last row);
+// W holds the records of rows [wb, wb + 64) and e0 lies in that window.
+template <int NT>
+__device__ __forceinline__ void at_request_w(const DevDict &d, const VRows &vr, uint32_t e0, const RecWin &W,
+                                             uint32_t wb, int lane, float *T)
+{
+    const uint32_t r = min(e0 + (uint32_t)(lane & 15), vr.total - 1);   // row lane%16 of the tile
+    const uint32_t c = (uint32_t)__shfl((int)W.code, (int)(r - wb));
+    const float *src = row_code(d, c, 0) + 4 * (lane >> 4);
+#pragma unroll
+    for (int u = 0; u < NT; u++)                                  // chunks 4u .. 4u+3 of every row
+        __builtin_amdgcn_global_load_lds(src + 16 * u, (lds_void *)(T + 256 * u), 16, 0, 0);
+}
+
+__device__ __forceinline__ Aux at_aux_w(const VRows &vr, uint32_t e0, int g, const RecWin &W, uint32_t wb)
+{
+    Aux a;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t r = min(e0 + 4 * g + k, vr.total - 1);
+        a.slot[k] = (uint32_t)__shfl((int)W.slot, (int)(r - wb));
+        a.scale[k] = __shfl(W.scale, (int)(r - wb));
+    }
+    return a;
+}
+
 template <int NT>
 __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d, const VRows &vr,
                                           size_t lut_rf, int lane, float *__restrict__ z, float *__restrict__ Xr,
@@ -457,8 +481,11 @@ __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDi
 {
     const int g = lane >> 4, q = lane & 15;
     if (vr.total == 0) return;
-    at_request<NT>(d, vr, 0, lut_rf, lane, T);
-    Aux aux = at_aux(d, vr, 0, g, lut_rf), auxp = aux;
+    // record windows: W0 rows [wb, wb+64), W1 the next 64 (loaded a window ahead)
+    uint32_t wb = 0;
+    RecWin W0 = rec_win(d, vr, 0, lane), W1 = rec_win(d, vr, kWave, lane);
+    at_request_w<NT>(d, vr, 0, W0, wb, lane, T);
+    Aux aux = at_aux_w(vr, 0, g, W0, wb), auxp = aux;
     f4 pend = f4{0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t pe0 = vr.total;                             // total: nothing pending
     for (uint32_t e0 = 0; e0 < vr.total; e0 += 16) {
@@ -472,9 +499,15 @@ __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDi
         __builtin_amdgcn_wave_barrier();
         if (pe0 < vr.total) at_store(pend, auxp, pe0, vr.total, g, q, z, Xr, R, rhs_ok);
         Aux auxn = aux;
-        if (e0 + 16 < vr.total) {
-            at_request<NT>(d, vr, e0 + 16, lut_rf, lane, T);
-            auxn = at_aux(d, vr, e0 + 16, g, lut_rf);
+        const uint32_t e1 = e0 + 16;
+        if (e1 < vr.total) {
+            if (e1 == wb + kWave) {                          // next tile opens the next window
+                W0 = W1;
+                wb += kWave;
+                W1 = rec_win(d, vr, wb + kWave, lane);
+            }
+            at_request_w<NT>(d, vr, e1, W0, wb, lane, T);
+            auxn = at_aux_w(vr, e1, g, W0, wb);
         }
         f4 acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
 #pragma unroll
