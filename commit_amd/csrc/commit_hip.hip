@@ -131,7 +131,7 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (
 template <typename T>
 __device__ __forceinline__ void flush8(const double *ring, int tb, int cnt, int lane, double l,
                                        uint32_t colk, bool ic, uint32_t m, double *__restrict__ z, T *__restrict__ out,
-                                       size_t xs, int i, uint32_t base)
+                                       size_t xs, int i)
 {
     const int te = tb + (lane >> 3);
     const double2 *row = reinterpret_cast<const double2 *>(ring + (size_t)(te % kRing) * kRingStride + 8 * (lane & 7));
@@ -188,7 +188,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
         ring[(size_t)(t % kRing) * kRingStride + lane] = dot_row<NJ>(CUR, yv);                            \
         if ((t & 7) == 7) {                                                                               \
             __builtin_amdgcn_wave_barrier();                                                              \
-            flush8(ring, t - 7, cnt, lane, l, colk, ic, cp.m, z, out, xs, i, base);                       \
+            flush8(ring, t - 7, cnt, lane, l, colk, ic, cp.m, z, out, xs, i);                             \
             __builtin_amdgcn_wave_barrier();                                                              \
         }                                                                                                 \
         if (++t >= cnt) break;                                                                            \
@@ -201,7 +201,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
 #undef AT_STAGE
             if (cnt & 7) {                        // last partial group of 8
                 __builtin_amdgcn_wave_barrier();
-                flush8(ring, cnt & ~7, cnt, lane, l, colk, ic, cp.m, z, out, xs, i, base);
+                flush8(ring, cnt & ~7, cnt, lane, l, colk, ic, cp.m, z, out, xs, i);
                 __builtin_amdgcn_wave_barrier();
             }
         }

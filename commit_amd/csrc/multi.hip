@@ -475,9 +475,8 @@ __device__ __forceinline__ Aux at_aux_w(const VRows &vr, uint32_t e0, int g, con
 }
 
 template <int NT>
-__device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d, const VRows &vr,
-                                          size_t lut_rf, int lane, float *__restrict__ z, float *__restrict__ Xr,
-                                          int R, bool rhs_ok, float *T)
+__device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d, const VRows &vr, int lane,
+                                          float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok, float *T)
 {
     const int g = lane >> 4, q = lane & 15;
     if (vr.total == 0) return;
@@ -564,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
     for (int t = 0; t < 4 * NT; t++)
         if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
     const VRows vr = vrows(d, v);
-    mm_vox_At<NT>(yb, d, vr, (size_t)d.ndirs * d.nSp, lane, z, X + rhs, R, rhs_ok, T);
+    mm_vox_At<NT>(yb, d, vr, lane, z, X + rhs, R, rhs_ok, T);
 }
 
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
