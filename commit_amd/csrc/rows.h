@@ -101,18 +101,37 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
 
 // Output stage of A for voxel v: y row (or the residual sqrtw .* (A x - yobs))
 // and the voxel's partial of ||r||^2.                  (solvers.py:306-308)
-template <int NJ, typename T>
+// RES false: a plain product (out.yobs, out.sqrtw and out.rpart are null).
+template <int NJ, typename T, bool RES = true>
 __device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &d, int v, const AOut<T> &out, int lane)
 {
     const size_t row = (size_t)v * d.nS;
+    if constexpr (!RES) {
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int s = lane + kWave * j;
+            if (s < d.nS) __builtin_nontemporal_store((T)acc[j], out.y + row + s);
+        }
+        return;
+    }
     double sq = 0.0;
+    // the observations and weights are all requested before the first store:
+    // a load issued after a store would wait for that store as well
+    T yo[NJ], wv[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int s = lane + kWave * j;
+        const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
+        yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
+        wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
+    }
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int s = lane + kWave * j;
         if (s < d.nS) {
             double r = acc[j];
-            if (out.yobs) r -= (double)stream_load(out.yobs + row + s);
-            if (out.sqrtw) r *= (double)stream_load(out.sqrtw + row + s);
+            if (out.yobs) r -= (double)yo[j];
+            if (out.sqrtw) r *= (double)wv[j];
             __builtin_nontemporal_store((T)r, out.y + row + s);
             sq = fma(r, r, sq);
         }

@@ -222,9 +222,11 @@ __device__ __forceinline__ const float *readlane_ptr(const float *p, int l)
     return reinterpret_cast<const float *>(((uint64_t)hi << 32) | lo);
 }
 
-template <int NJ, int V, int W, typename T>
-__global__ __launch_bounds__(W * kWave, 4) void k_A_slab(DevDict d, const T *__restrict__ x, const T *__restrict__ xic,
-                                                     AOut<T> out, SlabArgs sa)
+// RES: the residual output stage (commit_hip_A_residual); a separate
+// instantiation, so the plain product keeps its own register allocation.
+template <int NJ, int V, int W, typename T, bool RES>
+__device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x, const T *__restrict__ xic,
+                                       const AOut<T> &out, const SlabArgs &sa)
 {
     __shared__ __align__(16) float L[SlabLds<W>::floats];
     const int lane = threadIdx.x & (kWave - 1);
@@ -291,8 +293,22 @@ __global__ __launch_bounds__(W * kWave, 4) void k_A_slab(DevDict d, const T *__r
                            x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
                            (size_t)d.nSp, 0, lane, tail);
         }
-        a_store<NJ, T>(acc[u], d, v, out, lane);
+        a_store<NJ, T, RES>(acc[u], d, v, out, lane);
     }
+}
+
+template <int NJ, int V, int W, typename T>
+__global__ __launch_bounds__(W * kWave, 4) void k_A_slab(DevDict d, const T *__restrict__ x, const T *__restrict__ xic,
+                                                     AOut<T> out, SlabArgs sa)
+{
+    a_slab<NJ, V, W, T, false>(d, x, xic, out, sa);
+}
+
+template <int NJ, int V, int W, typename T>
+__global__ __launch_bounds__(W * kWave, 4) void k_A_slab_res(DevDict d, const T *__restrict__ x,
+                                                         const T *__restrict__ xic, AOut<T> out, SlabArgs sa)
+{
+    a_slab<NJ, V, W, T, true>(d, x, xic, out, sa);
 }
 
 template <int NJ>
@@ -634,7 +650,10 @@ int launch_A_slab(commit_hip_plan *p, const T *x, const T *xic, const AOut<T> &o
 {
     const DevDict &d = p->d;
     if (!p->slab || !p->slab_vlist || d.nSp / kWave > kSlabMaxNJ) return COMMIT_HIP_EUNSUPPORTED;
-    SLAB_SWITCH(k_A_slab, kSlabVA, p->slab_vlist, p->slab_blocks, d, x, xic, out);
+    if (out.rpart || out.yobs || out.sqrtw)
+        SLAB_SWITCH(k_A_slab_res, kSlabVA, p->slab_vlist, p->slab_blocks, d, x, xic, out);
+    else
+        SLAB_SWITCH(k_A_slab, kSlabVA, p->slab_vlist, p->slab_blocks, d, x, xic, out);
     return 0;
 }
 
