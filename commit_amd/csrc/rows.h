@@ -143,6 +143,47 @@ __device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &
     }
 }
 
+// The residual output stage split in two, so that a wavefront with several
+// voxels can request the next voxel's observations and weights before it
+// stores the current one: res_load, then res_store.
+template <int NJ, typename T>
+__device__ __forceinline__ void res_load(T (&yo)[NJ], T (&wv)[NJ], const DevDict &d, int v, const AOut<T> &out,
+                                         int lane)
+{
+    const size_t row = (size_t)v * d.nS;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int s = lane + kWave * j;
+        const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
+        yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
+        wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
+    }
+}
+
+template <int NJ, typename T>
+__device__ __forceinline__ void res_store(const double (&acc)[NJ], const T (&yo)[NJ], const T (&wv)[NJ],
+                                          const DevDict &d, int v, const AOut<T> &out, int lane)
+{
+    const size_t row = (size_t)v * d.nS;
+    double sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int s = lane + kWave * j;
+        if (s < d.nS) {
+            double r = acc[j];
+            if (out.yobs) r -= (double)yo[j];
+            if (out.sqrtw) r *= (double)wv[j];
+            __builtin_nontemporal_store((T)r, out.y + row + s);
+            sq = fma(r, r, sq);
+        }
+    }
+    if (out.rpart) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
+        if (lane == 0) out.rpart[v] = sq;
+    }
+}
+
 // ---- cross-lane moves of doubles on the VALU (no LDS traffic) --------------
 // DPP controls: quad_perm [1,0,3,2] and [2,3,0,1], row_mirror (lane i <- 15-i
 // inside each row of 16), row_half_mirror (lane i <- 7-i inside each 8).

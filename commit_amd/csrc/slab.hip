@@ -293,7 +293,23 @@ __device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x
                            x + (size_t)d.nIC * d.nF + (size_t)d.nEC * d.nE, (size_t)d.nI, d.nISO, d.isol,
                            (size_t)d.nSp, 0, lane, tail);
         }
-        a_store<NJ, T, RES>(acc[u], d, v, out, lane);
+        if constexpr (!RES) a_store<NJ, T, false>(acc[u], d, v, out, lane);
+    }
+    if constexpr (RES) {
+        // residual: stores after every voxel's items, so no row fetch above
+        // waits on a store, and the next voxel's observations and weights are
+        // requested before this voxel's stores
+        T yo[NJ] = {}, wv[NJ] = {};
+        if (vox[0] >= 0) res_load<NJ, T>(yo, wv, d, vox[0], out, lane);
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+            T yn[NJ] = {}, wn[NJ] = {};
+            const int vn = vox[u + 1 < V ? u + 1 : 0];
+            if (u + 1 < V && vn >= 0) res_load<NJ, T>(yn, wn, d, vn, out, lane);
+            if (vox[u] >= 0) res_store<NJ, T>(acc[u], yo, wv, d, vox[u], out, lane);
+#pragma unroll
+            for (int j = 0; j < NJ; j++) yo[j] = yn[j], wv[j] = wn[j];
+        }
     }
 }
 
