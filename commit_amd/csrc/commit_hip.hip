@@ -752,7 +752,7 @@ int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, 
 int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,
                           void *res, double *rsq, void *stream)
 {
-    if (!plan || !x || !res || !rsq) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (!plan || !x || !rsq) return fail(COMMIT_HIP_EINVAL, "NULL argument");
     if (plan->nolut && plan->nS_ref > 1)
         return fail(COMMIT_HIP_EUNSUPPORTED, "residual of a LUT-free operator with nS > 1");
     HIP_TRY(hipSetDevice(plan->device));

@@ -172,7 +172,7 @@ __device__ __forceinline__ void dense_store(const DevDict &d, const T *__restric
             const size_t row = (size_t)v * d.nS + s;
             if (out.yobs) yv -= (double)out.yobs[row];        // (solvers.py:308)
             if (out.sqrtw) yv *= (double)out.sqrtw[row];      // (solvers.py:306)
-            __builtin_nontemporal_store((T)yv, out.y + row);
+            if (out.y) __builtin_nontemporal_store((T)yv, out.y + row);
             sq[r] = fma(yv, yv, sq[r]);
         }
     }

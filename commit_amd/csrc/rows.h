@@ -132,7 +132,7 @@ __device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &
             double r = acc[j];
             if (out.yobs) r -= (double)yo[j];
             if (out.sqrtw) r *= (double)wv[j];
-            __builtin_nontemporal_store((T)r, out.y + row + s);
+            if (out.y) __builtin_nontemporal_store((T)r, out.y + row + s);
             sq = fma(r, r, sq);
         }
     }
@@ -173,7 +173,7 @@ __device__ __forceinline__ void res_store(const double (&acc)[NJ], const T (&yo)
             double r = acc[j];
             if (out.yobs) r -= (double)yo[j];
             if (out.sqrtw) r *= (double)wv[j];
-            __builtin_nontemporal_store((T)r, out.y + row + s);
+            if (out.y) __builtin_nontemporal_store((T)r, out.y + row + s);
             sq = fma(r, r, sq);
         }
     }

@@ -302,9 +302,9 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
     rsq = torch.empty(1, dtype=torch.float64, device=device)
     st = _stream(yd)
 
-    def residual(v, with_y=True):
-        _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yd) if with_y else None, P(sw), P(res), P(rsq), st),
-                   "commit_hip_A_residual")
+    def residual(v, with_y=True, store=True):
+        _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yd) if with_y else None, P(sw), P(res) if store else None,
+                                             P(rsq), st), "commit_hip_A_residual")
         return float(rsq.item())
 
     def residual64(v, yv, swv, out):
@@ -354,7 +354,7 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
         s = reg.sums(x, xhat, grad, prev_x)
         reg_x = reg.omega_from(s)
         q = qfval + s[0] + 0.5 / step * sqrt(s[1]) ** 2 + reg_x
-        rn = sqrt(residual(x))
+        rn = sqrt(residual(x, store=False))                   # the trial point needs only the norm
         return reg_x, q, rn, 0.5 * rn ** 2 + reg_x, s
 
     while True:

@@ -138,8 +138,9 @@ int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream);
  * res = sqrtw .* (A x - yobs) and *rsq = ||res||^2 (rsq: one device double).
  * yobs may be NULL (res = A x, *rsq = ||A x||^2, the Lipschitz quotient of
  * solvers.py:282-286); sqrtw may be NULL (no confidence weights,
- * solvers.py:238-239).  Fuses the reference's A.dot(x) - y and norm
- * (solvers.py:265-270, 308-311).
+ * solvers.py:238-239); res may be NULL (the norm only, as a FISTA trial point
+ * needs).  Fuses the reference's A.dot(x) - y and norm (solvers.py:265-270,
+ * 308-311).
  */
 int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,
                           void *res, double *rsq, void *stream);

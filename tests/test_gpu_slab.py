@@ -140,3 +140,33 @@ def test_slab_many_ec_iso_items(mode):
     rng = np.random.default_rng(mode)
     x, y = rng.standard_normal(A.shape[1]), rng.standard_normal(A.shape[0])
     assert rel(A.dot(x), O.dot(x)) <= TOL and rel(A.T.dot(y), O.T.dot(y)) <= TOL
+
+
+@pytest.mark.parametrize("option,value", [(_lib.OPT_SLAB, 0), (_lib.OPT_SLAB, 2), (_lib.OPT_A_DENSE, 1)])
+def test_residual_norm_only(option, value):
+    """commit_hip_A_residual with res NULL (a FISTA trial point needs only the
+    norm): the squared norm is bitwise the one of the call that stores res,
+    and res is left untouched; FP64 and FP32 storage."""
+    D, K = _dictionary(23, nVOX=120, n=6000, nS=288, ndirs=60, nIC=1, nEC=1, nISO=2)
+    A = LinearOperator(D, K, None)
+    autotune.set_option(A, option, value)
+    lib = _lib.load()
+    P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(4)
+    for prec, dt in ((_lib.FP64, torch.float64), (_lib.FP32, torch.float32)):
+        A.set_precision(prec)
+        x = torch.from_numpy(rng.standard_normal(A.shape[1])).to("cuda", dt)
+        y = torch.from_numpy(rng.standard_normal(A.shape[0])).to("cuda", dt)
+        w = torch.from_numpy(rng.uniform(0.5, 1.5, A.shape[0])).to("cuda", dt)
+        for yo, wo in ((y, w), (y, None), (None, w)):
+            res = torch.empty(A.shape[0], dtype=dt, device="cuda")
+            r1 = torch.empty(1, dtype=torch.float64, device="cuda")
+            r2 = torch.empty(1, dtype=torch.float64, device="cuda")
+            _lib.check(lib.commit_hip_A_residual(A._plan.handle, P(x), P(yo), P(wo), P(res), P(r1), st), "res")
+            keep = res.clone()
+            _lib.check(lib.commit_hip_A_residual(A._plan.handle, P(x), P(yo), P(wo), None, P(r2), st), "norm")
+            assert torch.equal(r1, r2) and torch.equal(res, keep)
+            ref = (res.double() ** 2).sum().item()           # (FP32: from the rounded res)
+            assert abs(r1.item() - ref) <= (1e-12 if prec == _lib.FP64 else 1e-6) * ref
+    A.set_precision(_lib.FP64)
