@@ -346,17 +346,22 @@ __global__ __launch_bounds__(kBlock) void k_res_pieces(const double *__restrict_
     }
 }
 
-// rsq[s * R + r] = sum of the piece sums of problem s, in piece order
+// rsq[s * R + r] = sum of the piece sums of problem s: one wavefront per
+// (problem, column), lane l adding pieces l, l + 64, ... in order, then a
+// fixed butterfly over the 64 lanes
 __global__ __launch_bounds__(kBlock) void k_res_problems(const double *__restrict__ pp, const int32_t *__restrict__ pseg,
                                                          int npieces, int S, int R, double *__restrict__ rsq)
 {
-    for (int t = threadIdx.x; t < S * R; t += kBlock) {
-        const int r = t % R, s = t / R;
-        double v = 0.0;
-        for (int p = 0; p < npieces; p++)
-            if (pseg[p] == s) v += pp[(size_t)p * R + r];
-        rsq[t] = v;
-    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = (int)(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    if (t >= S * R) return;
+    const int r = t % R, s = t / R;
+    double v = 0.0;
+    for (int p = lane; p < npieces; p += kWave)
+        if (pseg[p] == s) v += pp[(size_t)p * R + r];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) rsq[t] = v;
 }
 
 // ------------------------------------------------------------ X = A^T Y -----
@@ -687,7 +692,8 @@ int commit_hip_A_multi_residual(commit_hip_plan *plan, const commit_hip_multi_la
     if ((rc = launch_A_mm<true>(plan, X, nullptr, R, ra, st))) return rc;
     hipLaunchKernelGGL(k_res_pieces, dim3((unsigned)L->nypieces), dim3(kBlock), 0, st, part, L->ypieces, d.nS, d.nVOX,
                        plan->multi_a_waves, nrg, R, pp);
-    hipLaunchKernelGGL(k_res_problems, dim3(1), dim3(kBlock), 0, st, pp, L->ypseg, (int)L->nypieces, L->nprob, R, rsq);
+    hipLaunchKernelGGL(k_res_problems, dim3((unsigned)((L->nprob * R + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(kBlock), 0, st, pp, L->ypseg, (int)L->nypieces, L->nprob, R, rsq);
     HIP_TRY(hipGetLastError());
     return 0;
 }
