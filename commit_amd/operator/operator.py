@@ -162,14 +162,23 @@ class LinearOperator:
         self._plan.set_eval(ICeval)
         self.DICTIONARY["IC"]["eval"] = np.ascontiguousarray(ICeval, dtype=np.uint32)
 
+    def _check_device(self, t, what):
+        if t.device.index != self._plan.device:
+            raise RuntimeError(f"{what}: tensor on {t.device}, the plan lives on cuda:{self._plan.device}")
+
     def matvec(self, v_in, out=None):
         """Device product: v_in is a contiguous GPU tensor of the plan's dtype."""
         if v_in.numel() != self.shape[1]:
             raise RuntimeError("A.dot(): dimensions do not match")
         if v_in.dtype != self._plan.dtype or not v_in.is_cuda or not v_in.is_contiguous():
             raise RuntimeError(f"A.dot(): expected a contiguous CUDA tensor of {self._plan.dtype}")
+        self._check_device(v_in, "A.dot()")
         if out is None:
             out = torch.empty(self.shape[0], dtype=self._plan.dtype, device=v_in.device)
+        elif out.numel() != self.shape[0] or out.dtype != self._plan.dtype or not out.is_contiguous():
+            raise RuntimeError(f"A.dot(): out must be a contiguous {self._plan.dtype} tensor of {self.shape[0]} entries")
+        else:
+            self._check_device(out, "A.dot()")
         stream = torch.cuda.current_stream(v_in.device).cuda_stream
         self._plan.run(self.adjoint, v_in, out, stream)
         return out
@@ -184,16 +193,20 @@ class LinearOperator:
             raise RuntimeError("A.matmat(): dimensions do not match")
         if V.dtype != torch.float32 or not V.is_cuda or not V.is_contiguous():
             raise RuntimeError("A.matmat(): expected a contiguous CUDA float32 tensor")
+        self._check_device(V, "A.matmat()")
         R = int(V.shape[1])
         if out is None:
             out = torch.empty((self.shape[0], R), dtype=torch.float32, device=V.device)
         elif out.shape != (self.shape[0], R) or out.dtype != torch.float32 or not out.is_contiguous():
             raise RuntimeError("A.matmat(): bad output tensor")
+        else:
+            self._check_device(out, "A.matmat()")
         if sqrtw is not None:
             if not self.adjoint:
                 raise RuntimeError("A.matmat(): row weights apply to the adjoint only")
             if sqrtw.numel() != self.nROWS or sqrtw.dtype != torch.float32 or not sqrtw.is_contiguous():
                 raise RuntimeError("A.matmat(): sqrtw must be contiguous float32 [nROWS]")
+            self._check_device(sqrtw, "A.matmat()")
         stream = torch.cuda.current_stream(V.device).cuda_stream
         self._plan.run_multi(self.adjoint, V, out, R, sqrtw, stream)
         return out

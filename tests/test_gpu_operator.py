@@ -102,6 +102,16 @@ def test_size_check_and_dtype():
     with pytest.raises(RuntimeError, match="dimensions do not match"):
         A.T.dot(np.zeros(A.shape[1]))
     assert A.T.shape == (A.shape[1], A.shape[0])
+    # a device output buffer of the wrong size, type or layout is refused
+    # before any launch
+    xd = torch.zeros(A.shape[1], dtype=torch.float64, device="cuda")
+    for bad in (torch.empty(A.shape[0] - 1, dtype=torch.float64, device="cuda"),
+                torch.empty(A.shape[0], dtype=torch.float32, device="cuda"),
+                torch.empty((A.shape[0], 2), dtype=torch.float64, device="cuda")[:, 0]):
+        with pytest.raises(RuntimeError, match="out must be"):
+            A.matvec(xd, out=bad)
+    ok = torch.full((A.shape[0],), 7.0, dtype=torch.float64, device="cuda")
+    assert A.matvec(xd, out=ok) is ok and not ok.any()
 
 
 def test_device_path_and_determinism(small):
