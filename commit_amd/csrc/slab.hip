@@ -22,10 +22,11 @@
 //
 //   A x   : per entry, NJ FP64 FMAs per lane from an LDS row; zero
 //           coefficients skipped with a ballot (setup_operator.py:88).
-//   A^T y : per entry, a per-lane partial dot product; 8 entries are reduced
-//           together over the 64 lanes with VALU cross-lane moves
-//           (permlane32/16 swaps and DPP), no LDS traffic.  The IC product
-//           goes to its fibre-major scratch slot, summed per fibre by
+//   A^T y : per distinct (voxel, orientation), a per-lane partial dot
+//           product; 8 of them are reduced together over the 64 lanes with
+//           VALU cross-lane moves (permlane32/16 swaps and DPP).  Entries of
+//           equal orientation in a voxel share the dot.  The IC product
+//           l * dot goes to its fibre-major scratch slot, summed per fibre by
 //           pass 2 (k_At_fibre).
 #include <hip/hip_runtime.h>
 
@@ -337,20 +338,31 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (
 }
 
 // A^T y pass 1: up to 64 IC entries of one voxel and one slab, lane l holding
-// record r of entry b+l, cnt of them valid; z[zpos] = l * dot.  The row
-// offsets go to the wavefront's broadcast table.  Groups of 8 entries:
-// per-lane partial dots, then reduce8 (entry t0+g ends in lanes 8g..8g+7;
-// lane 8g stores it).  Entries past cnt in the last group read valid rows and
-// are not stored.                                     (setup_operator.py:384-428)
+// record r of entry b+l, cnt of them valid; z[zpos] = l * dot.  A voxel's
+// entries are sorted by orientation, so entries of equal orientation are
+// adjacent and share one dot product LUT[o].y[v]: only the first entry of each
+// run (a head) computes it.  The heads' row offsets go, in order, to slots
+// 0..nu-1 of the wavefront's broadcast table (the other lanes fill the slots
+// past nu with valid rows).  Groups of 8 heads: per-lane partial dots, then
+// reduce8 (head t0+g ends in lanes 8g..8g+7; lane 8g puts it in the table's
+// c[t0+g]).  Slots past nu in the last group read valid rows and are not
+// kept.  Each entry then reads its run's dot from the table and stores
+// l * dot.                                            (setup_operator.py:384-428)
 template <int NJ>
 __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec &r, int cnt, double *__restrict__ zi,
                                               const float *Lb, Bcast *bt, int o0, int nSp, int lane)
 {
-    bt->off[lane] = ((int)r.o - o0) * nSp;
+    const uint32_t op = (uint32_t)__shfl((int)r.o, lane > 0 ? lane - 1 : 0);
+    const bool head = lane < cnt && (lane == 0 || r.o != op);
+    const unsigned long long m = __ballot(head);
+    const int nu = __popcll(m);
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const int run = head ? below : below - 1;                 // this entry's head, counted in head order
+    bt->off[head ? below : nu + (lane - below)] = ((int)r.o - o0) * nSp;
     int4 oa = *reinterpret_cast<const int4 *>(&bt->off[0]);
     int4 ob = *reinterpret_cast<const int4 *>(&bt->off[4]);
     int t0 = 0;
-    for (; t0 + 4 < cnt; t0 += 8) {
+    for (; t0 + 4 < nu; t0 += 8) {
         const int of[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
         double p[8];
 #pragma unroll
@@ -364,11 +376,9 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
         ob = *reinterpret_cast<const int4 *>(&bt->off[tn + 4]);
         const double sum = reduce8(p, lane);
         const int te = t0 + (lane >> 3);
-        const float le = __shfl(r.len, te);
-        const uint32_t ze = (uint32_t)__shfl((int)r.col, te);
-        if ((lane & 7) == 0 && te < cnt) zi[ze] = (double)le * sum;
+        if ((lane & 7) == 0 && te < nu) bt->c[te] = sum;
     }
-    if (t0 < cnt) {                          // a tail of at most 4 entries: reduce4 (entry t0+j in row j)
+    if (t0 < nu) {                           // a tail of at most 4 heads: reduce4 (head t0+j in row j)
         const int of[4] = {oa.x, oa.y, oa.z, oa.w};
         double p[4];
 #pragma unroll
@@ -379,10 +389,9 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
         }
         const double sum = reduce4(p);
         const int te = t0 + (lane >> 4);
-        const float le = __shfl(r.len, te);
-        const uint32_t ze = (uint32_t)__shfl((int)r.col, te);
-        if ((lane & 15) == 0 && te < cnt) zi[ze] = (double)le * sum;
+        if ((lane & 15) == 0 && te < nu) bt->c[te] = sum;
     }
+    if (lane < cnt) zi[r.col] = (double)r.len * bt->c[run];
 }
 
 // EC / ISO entries [b, e) of one voxel, rows from global memory:
