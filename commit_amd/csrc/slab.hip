@@ -20,8 +20,10 @@
 // slab walk: the items of all of a wavefront's voxels are decoded lane-parallel
 // and each voxel's rows are fetched together.
 //
-//   A x   : per entry, NJ FP64 FMAs per lane from an LDS row; zero
-//           coefficients skipped with a ballot (setup_operator.py:88).
+//   A x   : per run of entries of equal orientation in a voxel (their
+//           coefficients added first), NJ FP64 FMAs per lane from an LDS
+//           row; zero coefficients skipped with a ballot
+//           (setup_operator.py:88).
 //   A^T y : per distinct (voxel, orientation), a per-lane partial dot
 //           product; 8 of them are reduced together over the 64 lanes with
 //           VALU cross-lane moves (permlane32/16 swaps and DPP).  Entries of
@@ -105,6 +107,10 @@ __device__ __forceinline__ Rec rec_load(const Comp &cp, const uint32_t *__restri
     return Rec{stream_load(cp.o + ks), stream_load(colv + ks), stream_load(cp.len + ks)};
 }
 
+// Nonzero coefficients in a chunk from which A x looks for runs of one
+// orientation to merge.
+constexpr int kMergeMin = 12;
+
 // A x: apply up to 64 entries of one voxel and one slab, rows in LDS (Lb: this
 // lane's sample 0 of the slab's first orientation o0); lane l holds entry b+l
 // (record r, x value xv), valid when ok.  The nonzero coefficients and their
@@ -121,8 +127,33 @@ __device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T 
     double c = (double)xv;
     c *= (double)r.len;
     if (!ok) c = 0.0;
+    // A voxel's entries are sorted by orientation, so entries sharing a LUT
+    // row are adjacent: the first of each run of nonzero coefficients of one
+    // orientation adds the run's coefficients in entry order and applies the
+    // row once; the others drop out (one step per entry of the longest run).
+    // Chunks with few nonzero coefficients (a sparse x) rarely hold such runs
+    // and skip the search.
+    const unsigned long long m0 = __ballot(c != 0.0);
+    if (m0 == 0) return;
+    unsigned long long cm = 0;
+    if (__popcll(m0) >= kMergeMin) {
+        const uint32_t on = (uint32_t)__shfl((int)r.o, lane < kWave - 1 ? lane + 1 : lane);
+        cm = __ballot(lane < kWave - 1 && ((m0 >> lane) & 1ull) && ((m0 >> (lane + 1)) & 1ull) && on == r.o);
+    }
+    if (cm != 0) {
+        const bool cont = (cm >> lane) & 1ull;                        // entry b+l+1 is in my run
+        const bool head = lane == 0 || !((cm >> (lane - 1)) & 1ull);
+        bool alive = head && cont;
+        double s = c;
+        for (int k = 1; __ballot(alive) != 0; k++) {
+            const double ck = __shfl(c, lane + k < kWave ? lane + k : lane);
+            if (alive) s += ck;
+            alive = alive && lane + k < kWave && ((cm >> (lane + k)) & 1ull);
+        }
+        c = head ? s : 0.0;
+    }
     const bool nzl = c != 0.0;
-    const unsigned long long m = __ballot(nzl);
+    const unsigned long long m = cm != 0 ? __ballot(nzl) : m0;
     const int nz = __popcll(m);
     if (nz == 0) return;
     const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
