@@ -78,6 +78,7 @@ __device__ __forceinline__ void load_run(float (&a)[NT], const float *__restrict
 struct VRows {
     uint32_t icb, nic, ecb, nec, isob, niso;   // the voxel's entry ranges
     uint32_t n1, n2, total;                    // row counts: IC, IC+EC, all
+    const uint32_t *uhead;                     // A^T: IC rows are heads uhead[icb..icb+nic)
 };
 
 __device__ __forceinline__ VRows vrows(const DevDict &d, int v)
@@ -92,6 +93,24 @@ __device__ __forceinline__ VRows vrows(const DevDict &d, int v)
     vr.n1 = (uint32_t)d.nIC * vr.nic;
     vr.n2 = vr.n1 + (uint32_t)d.nEC * vr.nec;
     vr.total = vr.n2 + (uint32_t)d.nISO * vr.niso;
+    vr.uhead = nullptr;
+    return vr;
+}
+
+// The A^T list of voxel v: its IC rows are the heads of its runs of equal
+// orientation (mm_uhead[uptr[v]..uptr[v+1])), then EC and ISO rows as in A X.
+__device__ __forceinline__ VRows vrows_at(const DevDict &d, int v, const uint32_t *__restrict__ uptr,
+                                          const uint32_t *__restrict__ uhead)
+{
+    VRows vr = vrows(d, v);
+    if (d.nIC > 0) {
+        vr.icb = uptr[v];
+        vr.nic = uptr[v + 1] - vr.icb;
+        vr.n1 = (uint32_t)d.nIC * vr.nic;
+        vr.n2 = vr.n1 + (uint32_t)d.nEC * vr.nec;
+        vr.total = vr.n2 + (uint32_t)d.nISO * vr.niso;
+    }
+    vr.uhead = uhead;
     return vr;
 }
 
@@ -402,7 +421,7 @@ __device__ __forceinline__ void at_store(const f4 &acc, const Aux &a, uint32_t e
     for (int k = 0; k < 4; k++) {                        // D row 4g+k = row e0+4g+k, column q
         if (e0 + 4 * g + k < total) {
             if (a.scale[k] >= 0.0f)
-                z[(size_t)a.slot[k] * kTileR + q] = a.scale[k] * acc[k];    // (setup_operator.py:412-428)
+                z[(size_t)a.slot[k] * kTileR + q] = acc[k];                 // (setup_operator.py:412-428)
             else if (rhs_ok)
                 Xr[(size_t)a.slot[k] * R] = acc[k];                         // (setup_operator.py:444-563)
         }
@@ -410,8 +429,8 @@ __device__ __forceinline__ void at_store(const f4 &acc, const Aux &a, uint32_t e
 }
 
 // Record of row r (< vr.total) of the voxel's A^T list: LUT row code (table
-// in the top 2 bits: 0 IC, 1 EC, 2 ISO), output slot and scale (IC: slot
-// i*n + k of z, scale len; EC/ISO: x column, scale -1).
+// in the top 2 bits: 0 IC, 1 EC, 2 ISO), output slot and a sign (IC: slot
+// i*n + h of z for head entry h, 1; EC/ISO: x column, -1).
 __device__ __forceinline__ void at_rec(const DevDict &d, const VRows &vr, uint32_t r, uint32_t &code, uint32_t &slot,
                                        float &scale)
 {
@@ -426,9 +445,10 @@ __device__ __forceinline__ void at_rec(const DevDict &d, const VRows &vr, uint32
     const uint32_t i = nrf == 1 ? 0u : t / m;
     const uint32_t k = (ic ? icb : (ec ? ecb : isob)) + (t - i * m);
     if (ic) {
-        code = i * (uint32_t)d.ndirs + d.ic.o[k];
-        slot = i * (uint32_t)d.n + k;
-        scale = d.ic.len[k];
+        const uint32_t h = vr.uhead[k];
+        code = i * (uint32_t)d.ndirs + d.ic.o[h];
+        slot = i * (uint32_t)d.n + h;
+        scale = 1.0f;
     } else if (ec) {
         code = (1u << 30) | (i * (uint32_t)d.ndirs + d.ec.o[k]);
         slot = (uint32_t)d.nIC * (uint32_t)d.nF + i * (uint32_t)d.nE + d.ec.col[k];
@@ -532,7 +552,9 @@ __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDi
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__restrict__ Y,
                                                   const float *__restrict__ sqrtw, float *__restrict__ z,
-                                                  float *__restrict__ X, int R, int rg)
+                                                  float *__restrict__ X, int R, int rg,
+                                                  const uint32_t *__restrict__ uptr,
+                                                  const uint32_t *__restrict__ uhead)
 {
     __shared__ __align__(16) float tiles[kWavesPerBlock][256 * NT];   // 16 rows x 16NT floats per wave
     const int lane = threadIdx.x & (kWave - 1);
@@ -567,15 +589,17 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
 #pragma unroll
     for (int t = 0; t < 4 * NT; t++)
         if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
-    const VRows vr = vrows(d, v);
+    const VRows vr = vrows_at(d, v, uptr, uhead);
     mm_vox_At<NT>(yb, d, vr, lane, z, X + rhs, R, rhs_ok, T);
 }
 
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
-// voxel order) of z, in FP64.  16 lanes per (fibre, RF): one 64-byte row of z
-// per entry.
+// voxel order) of len * z[head], in FP64.  16 lanes per (fibre, RF): one
+// 64-byte row of z per entry; heads and lengths are read in fibre order.
 __global__ __launch_bounds__(kBlock) void k_At_fibre_mm(DevDict d, const float *__restrict__ z,
-                                                        float *__restrict__ X, int R, int rg)
+                                                        float *__restrict__ X, int R, int rg,
+                                                        const uint32_t *__restrict__ hpos,
+                                                        const float *__restrict__ lenF)
 {
     const size_t total = (size_t)d.nF * d.nIC * kTileR;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -588,16 +612,122 @@ __global__ __launch_bounds__(kBlock) void k_At_fibre_mm(DevDict d, const float *
         double acc = 0.0;
         uint32_t k = kb;
         for (; k + 8 <= ke; k += 8) {
-            float v[8];
+            float v[8], l[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = zi[(size_t)d.fib_perm[k + u] * kTileR];
+            for (int u = 0; u < 8; u++) {
+                v[u] = zi[(size_t)hpos[k + u] * kTileR];
+                l[u] = lenF[k + u];
+            }
 #pragma unroll
-            for (int u = 0; u < 8; u++) acc += (double)v[u];
+            for (int u = 0; u < 8; u++) acc += (double)l[u] * (double)v[u];
         }
-        for (; k < ke; k++) acc += (double)zi[(size_t)d.fib_perm[k] * kTileR];
+        for (; k < ke; k++) acc += (double)lenF[k] * (double)zi[(size_t)hpos[k] * kTileR];
         const int rhs = rg * kTileR + q;
         if (rhs < R) X[gcol * R + rhs] = d.eval[gcol] ? (float)acc : 0.0f;
     }
+}
+
+// Run heads of the multi-RHS A^T (built once per plan).  Entry k of voxel v
+// heads a run when it is the voxel's first or its orientation differs from
+// entry k-1's (a voxel's entries are sorted by orientation).
+__global__ __launch_bounds__(kBlock) void k_mm_heads(const uint32_t *__restrict__ ptr, const uint16_t *__restrict__ o,
+                                                     int nVOX, uint32_t *__restrict__ headof,
+                                                     uint32_t *__restrict__ cnt)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nVOX) return;
+    const uint32_t b = ptr[v], e = ptr[v + 1];
+    uint32_t h = b, c = 0;
+    for (uint32_t k = b; k < e; k++) {
+        if (k == b || o[k] != o[k - 1]) {
+            h = k;
+            c++;
+        }
+        headof[k] = h;
+    }
+    cnt[v] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mm_uhead(const uint32_t *__restrict__ ptr, const uint16_t *__restrict__ o,
+                                                     const uint32_t *__restrict__ uptr, int nVOX,
+                                                     uint32_t *__restrict__ uhead)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nVOX) return;
+    const uint32_t b = ptr[v], e = ptr[v + 1];
+    uint32_t j = uptr[v];
+    for (uint32_t k = b; k < e; k++)
+        if (k == b || o[k] != o[k - 1]) uhead[j++] = k;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mm_fibre_order(const uint32_t *__restrict__ fib_perm,
+                                                           const uint32_t *__restrict__ headof,
+                                                           const float *__restrict__ len, uint32_t n,
+                                                           uint32_t *__restrict__ hpos, float *__restrict__ lenF)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t e = fib_perm[k];
+        hpos[k] = headof[e];
+        lenF[k] = len[e];
+    }
+}
+
+// Builds the head lists on the null stream and waits for them.
+int build_mm_heads(commit_hip_plan *p)
+{
+    const DevDict &d = p->d;
+    const int nV = d.nVOX;
+    const uint32_t n = (uint32_t)d.n;
+    void *a = nullptr, *b = nullptr, *c = nullptr, *e = nullptr;
+    int rc;
+    if ((rc = commit_internal::dev_alloc(p, (size_t)(nV + 1) * sizeof(uint32_t), &a))) return rc;
+    if ((rc = commit_internal::dev_alloc(p, (size_t)n * sizeof(uint32_t), &b))) return rc;
+    if ((rc = commit_internal::dev_alloc(p, (size_t)n * sizeof(float), &c))) return rc;
+    p->mm_uptr = static_cast<uint32_t *>(a);
+    p->mm_hpos = static_cast<uint32_t *>(b);
+    p->mm_lenF = static_cast<float *>(c);
+    uint32_t *headof = nullptr, *cnt = nullptr;
+    if (hipMalloc((void **)&headof, (size_t)n * sizeof(uint32_t)) != hipSuccess)
+        return fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed", (size_t)n * sizeof(uint32_t));
+    if (hipMalloc((void **)&cnt, (size_t)(nV + 1) * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipFree(headof);
+        return fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed", (size_t)(nV + 1) * sizeof(uint32_t));
+    }
+    std::vector<uint32_t> h((size_t)nV + 1, 0u);
+    const unsigned gv = (unsigned)((nV + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_mm_heads, dim3(gv), dim3(kBlock), 0, nullptr, d.ic.ptr, d.ic.o, nV, headof, cnt);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpy(h.data(), cnt, (size_t)nV * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) {
+        uint64_t run = 0;                                  // exclusive scan: first head of each voxel
+        for (int v = 0; v < nV; v++) {
+            const uint32_t cv = h[v];
+            h[v] = (uint32_t)run;
+            run += cv;
+        }
+        h[nV] = (uint32_t)run;
+        err = hipMemcpy(p->mm_uptr, h.data(), (size_t)(nV + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
+    if (err == hipSuccess) {
+        if ((rc = commit_internal::dev_alloc(p, (size_t)std::max<uint32_t>(h[nV], 1u) * sizeof(uint32_t), &e))) {
+            (void)hipFree(headof);
+            (void)hipFree(cnt);
+            return rc;
+        }
+        p->mm_uhead = static_cast<uint32_t *>(e);
+        hipLaunchKernelGGL(k_mm_uhead, dim3(gv), dim3(kBlock), 0, nullptr, d.ic.ptr, d.ic.o, p->mm_uptr, nV,
+                           p->mm_uhead);
+        const unsigned gn = (unsigned)std::min<uint32_t>((n + kBlock - 1) / kBlock, 1u << 16);
+        if (n > 0)
+            hipLaunchKernelGGL(k_mm_fibre_order, dim3(gn), dim3(kBlock), 0, nullptr, d.fib_perm, headof, d.ic.len, n,
+                               p->mm_hpos, p->mm_lenF);
+        err = hipGetLastError();
+        if (err == hipSuccess) err = hipStreamSynchronize(nullptr);
+    }
+    (void)hipFree(headof);
+    (void)hipFree(cnt);
+    if (err != hipSuccess) return fail(COMMIT_HIP_EDEVICE, "run heads: %s", hipGetErrorString(err));
+    return 0;
 }
 
 int nt_of(const DevDict &d) { return (d.nS + 15) / 16; }
@@ -710,6 +840,7 @@ int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw,
         void *zp = nullptr;
         if ((rc = commit_internal::dev_alloc(plan, (size_t)d.n * d.nIC * kTileR * sizeof(float), &zp))) return rc;
         plan->zmulti = static_cast<float *>(zp);
+        if ((rc = build_mm_heads(plan))) return rc;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     const float *y = static_cast<const float *>(Y);
@@ -724,7 +855,8 @@ int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw,
             switch (nt_of(d)) {
 #define CASE(NT)                                                                                            \
     case NT:                                                                                                \
-        hipLaunchKernelGGL(k_At_mm<NT>, dim3(grid), dim3(kBlock), 0, st, d, y, w, plan->zmulti, x, nrhs, rg); \
+        hipLaunchKernelGGL(k_At_mm<NT>, dim3(grid), dim3(kBlock), 0, st, d, y, w, plan->zmulti, x, nrhs, rg,   \
+                           plan->mm_uptr, plan->mm_uhead);                                                \
         break;
                 CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
                 CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
@@ -734,7 +866,8 @@ int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw,
             HIP_TRY(hipGetLastError());
         }
         if (total > 0) {
-            hipLaunchKernelGGL(k_At_fibre_mm, dim3(g2), dim3(kBlock), 0, st, d, plan->zmulti, x, nrhs, rg);
+            hipLaunchKernelGGL(k_At_fibre_mm, dim3(g2), dim3(kBlock), 0, st, d, plan->zmulti, x, nrhs, rg, plan->mm_hpos,
+                               plan->mm_lenF);
             HIP_TRY(hipGetLastError());
         }
     }

@@ -78,6 +78,13 @@ struct commit_hip_plan {
     double *xic = nullptr;     // [nF*nIC] x*eval, when eval is not all ones
     double *rpart = nullptr;   // [nVOX + kSumBlocks] residual partials
     float *zmulti = nullptr;   // [nIC][n][16] per-entry products of the multi-RHS A^T pass 1
+    // multi-RHS A^T: a voxel's entries of equal orientation share one row
+    // product, computed for the first entry of each run (its head); built
+    // with zmulti
+    uint32_t *mm_uptr = nullptr;   // [nVOX+1] first head of each voxel in mm_uhead
+    uint32_t *mm_uhead = nullptr;  // [heads] entry index of each head, voxel-major
+    uint32_t *mm_hpos = nullptr;   // [n] head of each entry, in fibre order (as fib_perm)
+    float *mm_lenF = nullptr;      // [n] length of each entry, in fibre order
     // orientation-major lists of the IC, EC and ISO entries for the LUT gradient
     // (lut_grad.hip; built on first use)
     commit_internal::OmList lg_om[3];
