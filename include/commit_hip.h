@@ -166,7 +166,10 @@ int commit_hip_At_pass(commit_hip_plan *plan, const void *y, const void *sqrtw, 
  * dtype must be COMMIT_HIP_FP32 (float32 X, Y, sqrtw; FP32 matrix-core
  * products, FP64 per-fibre sums in A^T).  Needs nS <= 320 and a LUT-based
  * plan.  Outputs are fully overwritten.  A^T allocates its per-entry scratch
- * (16 floats per IC segment and RF) on first use and keeps it in the plan.
+ * (16 floats per IC segment and RF) and builds its run-head lists (12 bytes per
+ * IC segment; entries of equal orientation in a voxel share one row product)
+ * on first use, synchronously, and keeps them in the plan: make the first call
+ * outside graph capture.
  */
 int commit_hip_A_multi(commit_hip_plan *plan, const void *X, void *Y, int nrhs, int dtype, void *stream);
 int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw, void *X, int nrhs, int dtype,
