@@ -52,3 +52,36 @@ def load_golden(name="operator_small"):
     K = {"wmr": g["wmr"], "wmh": g["wmh"], "iso": g["iso"]}
     assert D["IC"]["eval"].size == int(g["nF"]) * nIC
     return D, K, {k: g[k] for k in ("x", "y", "Ax", "Aty", "gwmr", "gwmh", "giso")}
+
+
+def runs_dictionary(seed, nS=288, nIC=1, nEC=1, nISO=2):
+    """Voxels whose entries form runs of one orientation of every length from
+    1 to 200: runs that cross 64-entry chunks and groups of 8, voxels of a
+    single run, and runs of one orientation that straddle slab boundaries
+    (orientations 99/100/101 with 100-orientation slabs)."""
+    rng = np.random.default_rng(seed)
+    nVOX, ndirs = 40, 500
+    vox, ori = [], []
+    for v in range(nVOX):
+        if v == 5:
+            continue                                  # an empty voxel
+        if v % 7 == 0:
+            lens = [int(rng.integers(60, 200))]       # a single long run
+        else:
+            lens = list(rng.integers(1, 12, size=int(rng.integers(3, 25))))
+            lens[int(rng.integers(0, len(lens)))] = int(rng.integers(65, 140))
+        dirs = rng.choice(ndirs, size=len(lens), replace=False)
+        dirs[: min(3, len(dirs))] = [99, 100, 101][: min(3, len(dirs))]
+        for L, o in zip(lens, dirs):
+            vox += [v] * int(L)
+            ori += [int(o)] * int(L)
+    n = len(vox)
+    nF = n // 6
+    D, K = random_dictionary(seed, nVOX=nVOX, nF=nF, n=n, nS=nS, ndirs=ndirs, nIC=nIC, nEC=nEC, nISO=nISO,
+                             empty_voxels=(5,))
+    perm = rng.permutation(n)                         # entries of a run from many fibres, any order
+    D["IC"]["vox"] = np.asarray(vox, np.uint32)
+    D["IC"]["dir"] = np.asarray(ori, np.uint16)
+    D["IC"]["str"] = D["IC"]["str"][perm]
+    return D, K
+

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from _problems import random_dictionary, rel
+from _problems import random_dictionary, rel, runs_dictionary
 from commit_amd import synth
 from commit_amd.operator.operator import LinearOperator
 from oracle import oracle
@@ -159,3 +159,28 @@ def test_autotune_sets_and_caches(small, tmp_path):
     assert rep2 == rep
     assert autotune.get_option(B, _lib.OPT_XCD_REMAP) == rep["xcd_remap"]["choice"]
     assert autotune.get_option(B, _lib.OPT_SLAB) == rep["slab"]["choice"]
+
+
+@pytest.mark.parametrize("rf", [(1, 1, 2), (2, 1, 0)])
+def test_multi_runs_of_one_orientation(rf):
+    """16-RHS A^T Y computes one row product per run of equal orientation in a
+    voxel (runs of every length up to 200, across 16-row tiles and 64-row
+    record windows) and pass 2 scales it by each entry's length: per column
+    against the dense oracle, with an eval mask."""
+    nIC, nEC, nISO = rf
+    D, K = runs_dictionary(23, nS=288, nIC=nIC, nEC=nEC, nISO=nISO)
+    ev = np.ones(D["IC"]["nSTR"] * nIC, np.uint32)
+    ev[::5] = 0
+    D["IC"]["eval"] = ev
+    A = LinearOperator(D, K, None)
+    rng = np.random.default_rng(5)
+    R = 16
+    X = rng.standard_normal((A.shape[1], R)).astype(np.float32)
+    Y = rng.standard_normal((A.shape[0], R)).astype(np.float32)
+    dense = oracle.assemble_A(D, K)
+    Xa = A.T.matmat(_dev32(Y)).cpu().numpy()
+    Ya = A.matmat(_dev32(X)).cpu().numpy()
+    ex = _cols_rel(Xa, dense.T @ Y.astype(np.float64))
+    ey = _cols_rel(Ya, dense @ X.astype(np.float64))
+    print(f"rf={rf}: A X {ey:.2e}  A^T Y {ex:.2e}", flush=True)
+    assert ey <= TOL32 and ex <= TOL32

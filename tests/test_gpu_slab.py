@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from _problems import random_dictionary, rel
+from _problems import random_dictionary, rel, runs_dictionary
 from commit_amd import _lib, autotune
 from commit_amd.operator.operator import LinearOperator
 from oracle import oracle
@@ -170,3 +170,26 @@ def test_residual_norm_only(option, value):
             ref = (res.double() ** 2).sum().item()           # (FP32: from the rounded res)
             assert abs(r1.item() - ref) <= (1e-12 if prec == _lib.FP64 else 1e-6) * ref
     A.set_precision(_lib.FP64)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("zero_frac", [0.0, 0.3, 0.85])
+def test_slab_runs_of_one_orientation(mode, zero_frac):
+    """Entries of equal orientation in a voxel share one dot product (A^T) and
+    one row application (A x merges the run's nonzero coefficients): every run
+    length, zeros inside runs, against the oracle at the FP64 bar."""
+    D, K = runs_dictionary(11 + mode)
+    T = oracle.set_threads(D, 4)
+    A = LinearOperator(D, K, T)
+    autotune.set_option(A, _lib.OPT_SLAB, mode)
+    O = oracle.OracleOperator(D, K, T)
+    rng = np.random.default_rng(mode)
+    x = rng.uniform(0.1, 1.0, A.shape[1])
+    x[rng.random(x.size) < zero_frac] = 0.0
+    y = rng.standard_normal(A.shape[0])
+    w = rng.uniform(0.5, 1.5, A.shape[0])
+    Ax, Aty, res, rsq, g = _products(A, x, y, w)
+    r_o = w * (O.dot(x) - y)
+    e = (rel(Ax, O.dot(x)), rel(Aty, O.T.dot(y)), rel(res, r_o), rel(g, O.T.dot(w * r_o)))
+    print(f"mode={mode} zeros={zero_frac}: " + " ".join(f"{v:.1e}" for v in e), flush=True)
+    assert max(e) <= TOL
