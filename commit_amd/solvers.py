@@ -126,9 +126,10 @@ class DeviceRegularisation:
         self.work = torch.empty(_lib.sums_work(self.ngroups), dtype=torch.float64, device=device)
         self.out = torch.empty(8, dtype=torch.float64, device=device)
 
-    def sums(self, x, xhat=None, grad=None, prev=None):
+    def sums(self, x, xhat=None, grad=None, prev=None, out=None):
         """The eight deterministic sums of commit_hip_fista_sums (FP64
-        accumulation for float64 and float32 vectors alike), on the host."""
+        accumulation for float64 and float32 vectors alike), on the host; with
+        out (a float64 device tensor of 8), written there without a host sync."""
         lib = _lib.load()
         P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
         p = self.params
@@ -136,9 +137,10 @@ class DeviceRegularisation:
         rc = fn(self.n, P(x), P(xhat), P(grad), P(prev), self.l1_range, self.l1_weighted,
                                        ctypes.c_void_p(self.w.data_ptr()), self.ngroups,
                                        ctypes.c_void_p(p.group_ptr or 0), ctypes.c_void_p(p.group_idx or 0),
-                                       ctypes.c_void_p(p.group_w or 0), P(self.out), P(self.work), _stream(x))
+                                       ctypes.c_void_p(p.group_w or 0), P(self.out if out is None else out),
+                                       P(self.work), _stream(x))
         _lib.check(rc, "commit_hip_fista_sums")
-        return self.out.cpu().numpy()
+        return self.out.cpu().numpy() if out is None else None
 
     def omega_from(self, s):
         """Omega(x) from the sums (solvers.py:202 and the per-compartment omegas)."""
@@ -300,12 +302,15 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
     x = torch.empty(n, **vec)
     tmp = torch.empty(n, **vec)
     rsq = torch.empty(1, dtype=torch.float64, device=device)
+    # the loop's scalars, read back once per trial step: the eight sums,
+    # ||r(x)||^2 of the trial point, ||r(xhat)||^2 after a momentum update
+    sc = torch.empty(10, dtype=torch.float64, device=device)
     st = _stream(yd)
 
-    def residual(v, with_y=True, store=True):
+    def residual(v, with_y=True, store=True, out=None):
         _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yd) if with_y else None, P(sw), P(res) if store else None,
-                                             P(rsq), st), "commit_hip_A_residual")
-        return float(rsq.item())
+                                             P(rsq if out is None else out), st), "commit_hip_A_residual")
+        return float(rsq.item()) if out is None else None
 
     def residual64(v, yv, swv, out):
         """||sqrtw (A v - y)||^2 with FP64 vectors on the same plan."""
@@ -348,13 +353,22 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
         print("------|--------------------------------|-----------------------------------------------|------------------------------")
     it = 1
 
+    qf_pending = False                                        # ||r(xhat)||^2 waits in sc[9]
+
     def trial(step):
         # x = prox(xhat - step*grad, step); q, objective   (solvers.py:299-312)
+        nonlocal qfval, qf_pending
         reg.prox(x, step, xin=xhat, grad=grad, step=step)
-        s = reg.sums(x, xhat, grad, prev_x)
+        reg.sums(x, xhat, grad, prev_x, out=sc[:8])
+        residual(x, store=False, out=sc[8:9])                 # the trial point needs only the norm
+        v = sc.cpu().numpy()                                  # one host sync per trial step
+        if qf_pending:
+            qfval = 0.5 * sqrt(v[9]) ** 2
+            qf_pending = False
+        s = v[:8]
         reg_x = reg.omega_from(s)
         q = qfval + s[0] + 0.5 / step * sqrt(s[1]) ** 2 + reg_x
-        rn = sqrt(residual(x, store=False))                   # the trial point needs only the norm
+        rn = sqrt(v[8])
         return reg_x, q, rn, 0.5 * rn ** 2 + reg_x, s
 
     while True:
@@ -391,13 +405,13 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
         t = 0.5 * (1 + sqrt(1 + 4 * told ** 2))
         _lib.check(axpby(n, 1.0, P(x), -1.0, P(prev_x), P(tmp), st), "commit_hip_axpby")
         _lib.check(axpby(n, 1.0, P(x), (told - 1) / t, P(tmp), P(xhat), st), "commit_hip_axpby")
-        qf = residual(xhat)
+        residual(xhat, out=sc[9:10])                          # read with the next trial's scalars
         gradient()
         it += 1
         prev_obj = curr_obj
         prev_x.copy_(x)
         told = t
-        qfval = 0.5 * sqrt(qf) ** 2
+        qf_pending = True
     if verbose == 4:
         print("< Stopping criterion: %s >" % criterion, flush=True)
     details = {"residual": 0.5 * res_norm ** 2, "regterm": reg_term_x, "cost_function": curr_obj,
