@@ -672,7 +672,8 @@ __global__ __launch_bounds__(kBlock) void k_mm_fibre_order(const uint32_t *__res
     }
 }
 
-// Builds the head lists on the null stream and waits for them.
+// Builds the head lists on the null stream and waits for them; mm_uhead is set
+// last, so a failed build is redone by the next call.
 int build_mm_heads(commit_hip_plan *p)
 {
     const DevDict &d = p->d;
@@ -695,7 +696,7 @@ int build_mm_heads(commit_hip_plan *p)
     }
     std::vector<uint32_t> h((size_t)nV + 1, 0u);
     const unsigned gv = (unsigned)((nV + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_mm_heads, dim3(gv), dim3(kBlock), 0, nullptr, d.ic.ptr, d.ic.o, nV, headof, cnt);
+    if (gv > 0) hipLaunchKernelGGL(k_mm_heads, dim3(gv), dim3(kBlock), 0, nullptr, d.ic.ptr, d.ic.o, nV, headof, cnt);
     hipError_t err = hipGetLastError();
     if (err == hipSuccess) err = hipMemcpy(h.data(), cnt, (size_t)nV * sizeof(uint32_t), hipMemcpyDeviceToHost);
     if (err == hipSuccess) {
@@ -714,9 +715,9 @@ int build_mm_heads(commit_hip_plan *p)
             (void)hipFree(cnt);
             return rc;
         }
-        p->mm_uhead = static_cast<uint32_t *>(e);
-        hipLaunchKernelGGL(k_mm_uhead, dim3(gv), dim3(kBlock), 0, nullptr, d.ic.ptr, d.ic.o, p->mm_uptr, nV,
-                           p->mm_uhead);
+        if (gv > 0)
+            hipLaunchKernelGGL(k_mm_uhead, dim3(gv), dim3(kBlock), 0, nullptr, d.ic.ptr, d.ic.o, p->mm_uptr, nV,
+                               static_cast<uint32_t *>(e));
         const unsigned gn = (unsigned)std::min<uint32_t>((n + kBlock - 1) / kBlock, 1u << 16);
         if (n > 0)
             hipLaunchKernelGGL(k_mm_fibre_order, dim3(gn), dim3(kBlock), 0, nullptr, d.fib_perm, headof, d.ic.len, n,
@@ -727,6 +728,7 @@ int build_mm_heads(commit_hip_plan *p)
     (void)hipFree(headof);
     (void)hipFree(cnt);
     if (err != hipSuccess) return fail(COMMIT_HIP_EDEVICE, "run heads: %s", hipGetErrorString(err));
+    p->mm_uhead = static_cast<uint32_t *>(e);
     return 0;
 }
 
@@ -840,8 +842,8 @@ int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw,
         void *zp = nullptr;
         if ((rc = commit_internal::dev_alloc(plan, (size_t)d.n * d.nIC * kTileR * sizeof(float), &zp))) return rc;
         plan->zmulti = static_cast<float *>(zp);
-        if ((rc = build_mm_heads(plan))) return rc;
     }
+    if (!plan->mm_uhead && d.n > 0 && d.nIC > 0 && (rc = build_mm_heads(plan))) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const float *y = static_cast<const float *>(Y);
     const float *w = static_cast<const float *>(sqrtw);
