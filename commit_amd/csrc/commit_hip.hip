@@ -572,7 +572,7 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
     }
     {
         void *rp = nullptr;
-        if ((rc = dev_alloc(p, ((size_t)nVOX + kSumBlocks) * sizeof(double), &rp))) goto done;
+        if ((rc = dev_alloc(p, ((size_t)nVOX + kSumBlocks + kCombBlocks) * sizeof(double), &rp))) goto done;
         p->rpart = static_cast<double *>(rp);
     }
     if (nF > 0 && nIC > 0) {
@@ -715,9 +715,53 @@ int do_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const 
     return 0;
 }
 
+// r = a r1 + b r2 over n rows, with per-block sums of r^2 (FP64, from the
+// unrounded values): block k owns a fixed contiguous range, reduced in a
+// fixed order, so the norm is repeatable.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_combine(size_t n, double a, const T *__restrict__ r1, double b,
+                                                    const T *__restrict__ r2, T *__restrict__ r,
+                                                    double *__restrict__ part)
+{
+    __shared__ double sh[kBlock / kWave];
+    const size_t chunk = (n + kCombBlocks - 1) / kCombBlocks;
+    const size_t lo = (size_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double ss = 0.0;
+    for (size_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const double v = a * (double)r1[i] + b * (double)r2[i];
+        r[i] = (T)v;
+        ss += v * v;
+    }
+    ss = block_sum_256(ss, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = ss;
+}
+
+template <typename T>
+int do_combine(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2, void *r, double *rsq,
+               hipStream_t st)
+{
+    const size_t n = (size_t)plan->d.nVOX * plan->nS_ref;
+    double *part = plan->rpart + plan->d.nVOX + kSumBlocks;
+    hipLaunchKernelGGL(k_combine<T>, dim3(kCombBlocks), dim3(kBlock), 0, st, n, a, static_cast<const T *>(r1), b,
+                       static_cast<const T *>(r2), static_cast<T *>(r), part);
+    hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(kBlock), 0, st, part, kCombBlocks, rsq);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int commit_hip_residual_combine(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2, void *r,
+                                double *rsq, void *stream)
+{
+    if (!plan || !r1 || !r2 || !r || !rsq) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return plan->precision == COMMIT_HIP_FP32 ? do_combine<float>(plan, a, r1, b, r2, r, rsq, st)
+                                              : do_combine<double>(plan, a, r1, b, r2, r, rsq, st);
+}
 
 int commit_hip_A(commit_hip_plan *plan, const void *x, void *y, void *stream)
 {

@@ -26,6 +26,9 @@ constexpr int kBlock = kWave * kWavesPerBlock;
 // Deterministic sums: kSumBlocks blocks each reduce a fixed contiguous chunk in
 // a fixed order, then one block reduces the partials.
 constexpr int kSumBlocks = 256;
+// Blocks of the residual combination (commit_hip_residual_combine), each
+// summing a fixed range of rows; their partials are reduced by one block.
+constexpr int kCombBlocks = 2048;
 
 // Sum over a 256-thread block; the result is valid in thread 0.  Fixed order:
 // xor-butterfly inside each wavefront, then the 4 wavefront sums in order.

@@ -307,8 +307,13 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
     sc = torch.empty(10, dtype=torch.float64, device=device)
     st = _stream(yd)
 
-    def residual(v, with_y=True, store=True, out=None):
-        _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yd) if with_y else None, P(sw), P(res) if store else None,
+    # residuals of the trial point and of the previous iterate: the momentum
+    # point's residual is their affine combination (the residual is affine in x)
+    rbuf = [torch.empty(m, **vec), torch.empty(m, **vec)]
+
+    def residual(v, with_y=True, store=True, out=None, into=None):
+        dst = (res if into is None else into) if store else None
+        _lib.check(lib.commit_hip_A_residual(plan, P(v), P(yd) if with_y else None, P(sw), P(dst),
                                              P(rsq if out is None else out), st), "commit_hip_A_residual")
         return float(rsq.item()) if out is None else None
 
@@ -360,7 +365,7 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
         nonlocal qfval, qf_pending
         reg.prox(x, step, xin=xhat, grad=grad, step=step)
         reg.sums(x, xhat, grad, prev_x, out=sc[:8])
-        residual(x, store=False, out=sc[8:9])                 # the trial point needs only the norm
+        residual(x, out=sc[8:9], into=rbuf[0])                # r(x), kept for the momentum point
         v = sc.cpu().numpy()                                  # one host sync per trial step
         if qf_pending:
             qfval = 0.5 * sqrt(v[9]) ** 2
@@ -403,13 +408,21 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
             break
         # FISTA update (solvers.py:359-375): xhat = x + (told-1)/t * (x - prev_x)
         t = 0.5 * (1 + sqrt(1 + 4 * told ** 2))
+        mom = (told - 1) / t
         _lib.check(axpby(n, 1.0, P(x), -1.0, P(prev_x), P(tmp), st), "commit_hip_axpby")
-        _lib.check(axpby(n, 1.0, P(x), (told - 1) / t, P(tmp), P(xhat), st), "commit_hip_axpby")
-        residual(xhat, out=sc[9:10])                          # read with the next trial's scalars
+        _lib.check(axpby(n, 1.0, P(x), mom, P(tmp), P(xhat), st), "commit_hip_axpby")
+        # r(xhat) = (1 + mom) r(x) - mom r(prev_x), with its norm (read with the
+        # next trial's scalars); the first update has no r(prev_x) and runs A xhat
+        if it > 1:
+            _lib.check(lib.commit_hip_residual_combine(plan, 1.0 + mom, P(rbuf[0]), -mom, P(rbuf[1]), P(res),
+                                                       P(sc[9:10]), st), "commit_hip_residual_combine")
+        else:
+            residual(xhat, out=sc[9:10])
         gradient()
         it += 1
         prev_obj = curr_obj
         prev_x.copy_(x)
+        rbuf.reverse()                                        # r(prev_x) := r(x)
         told = t
         qf_pending = True
     if verbose == 4:

@@ -145,6 +145,17 @@ int commit_hip_At(commit_hip_plan *plan, const void *y, void *x, void *stream);
 int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const void *sqrtw,
                           void *res, double *rsq, void *stream);
 
+/*
+ * r = a r1 + b r2 over the operator's nrows rows, in the plan's precision, and
+ * *rsq = ||r||^2 (FP64, from the unrounded values, fixed order).  The FISTA
+ * momentum point's residual: the residual is affine in x, so
+ * r(xhat) = (1 + beta) r(x) - beta r(prev_x) for xhat = x + beta (x - prev_x),
+ * one pass over three row vectors instead of the product A xhat that
+ * solvers.py:363-368 recomputes (no counterpart in the reference).
+ */
+int commit_hip_residual_combine(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2, void *r,
+                                double *rsq, void *stream);
+
 /* x = A^T (sqrtw .* y); sqrtw may be NULL (solvers.py:266-267, 365-366). */
 int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, void *stream);
 

@@ -170,3 +170,44 @@ def test_fit_is_deterministic(problem):
     x1, d1 = solvers.solve(y, A, A.T, max_iter=30, verbose=0, regularisation=reg)
     x2, d2 = solvers.solve(y, A, A.T, max_iter=30, verbose=0, regularisation=reg)
     assert np.array_equal(x1, x2) and d1 == d2
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_residual_combine(problem, fp32):
+    """commit_hip_residual_combine (the momentum point's residual from r(x) and
+    r(prev_x)): r = a r1 + b r2 elementwise, ||r||^2 to 1e-13; and, the
+    residual being affine in x, it equals sqrtw (A xhat - y) to the FP64 bar
+    (1e-6 with float32 storage)."""
+    import ctypes
+    from commit_amd import _lib
+    D, K, T, _, _, y = problem
+    A = LinearOperator(D, K, T)
+    if fp32:
+        A.set_precision(_lib.FP32)
+    dt = torch.float32 if fp32 else torch.float64
+    lib = _lib.load()
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(3)
+    m, n = A.shape
+    x, xp = rng.random(n), rng.random(n)
+    w = rng.uniform(0.5, 1.5, m)
+    mom = 0.61803
+    xh = x + mom * (x - xp)
+    dev = lambda a: torch.from_numpy(a).to(device="cuda", dtype=dt)
+    yd, wd = dev(y), dev(w)
+    r1, r2, rh, r = (torch.empty(m, dtype=dt, device="cuda") for _ in range(4))
+    rsq = torch.empty(3, dtype=torch.float64, device="cuda")
+    for v, out, k in ((x, r1, 0), (xp, r2, 1), (xh, rh, 2)):
+        _lib.check(lib.commit_hip_A_residual(A._plan.handle, P(dev(v)), P(yd), P(wd), P(out), P(rsq[k:k + 1]), st),
+                   "A_residual")
+    _lib.check(lib.commit_hip_residual_combine(A._plan.handle, 1.0 + mom, P(r1), -mom, P(r2), P(r), P(rsq[2:3]), st),
+               "residual_combine")
+    r_np = r.double().cpu().numpy()
+    expect = (1.0 + mom) * r1.double().cpu().numpy() - mom * r2.double().cpu().numpy()
+    tol_e = 1e-6 if fp32 else 1e-15
+    assert np.max(np.abs(r_np - expect)) <= tol_e * np.max(np.abs(expect))
+    assert abs(rsq[2].item() - expect @ expect) <= (1e-6 if fp32 else 1e-13) * (expect @ expect)
+    e = rel(r_np, rh.double().cpu().numpy())
+    print(f"fp32={fp32}: combined vs direct residual {e:.2e}", flush=True)
+    assert e <= (1e-6 if fp32 else 1e-12)
