@@ -716,21 +716,73 @@ int do_A_residual(commit_hip_plan *plan, const void *x, const void *yobs, const 
 }
 
 // r = a r1 + b r2 over n rows, with per-block sums of r^2 (FP64, from the
-// unrounded values): block k owns a fixed contiguous range, reduced in a
-// fixed order, so the norm is repeatable.
+// unrounded values): block k owns a fixed contiguous range (a multiple of 16
+// bytes long), read with 16-byte loads and reduced in a fixed order, so the
+// norm is repeatable.
+template <typename T>
+__device__ __forceinline__ void combine_vec(const T (&u)[16 / sizeof(T)], const T (&w)[16 / sizeof(T)], double a,
+                                            double b, T (&o)[16 / sizeof(T)], double &ss)
+{
+#pragma unroll
+    for (int k = 0; k < (int)(16 / sizeof(T)); k++) {
+        const double v = a * (double)u[k] + b * (double)w[k];
+        o[k] = (T)v;
+        ss += v * v;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_combine(size_t n, double a, const T *__restrict__ r1, double b,
                                                     const T *__restrict__ r2, T *__restrict__ r,
                                                     double *__restrict__ part)
 {
+    constexpr int VEC = 16 / sizeof(T);
+    typedef T Vec[VEC];
     __shared__ double sh[kBlock / kWave];
-    const size_t chunk = (n + kCombBlocks - 1) / kCombBlocks;
-    const size_t lo = (size_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    const size_t chunk = ((n + kCombBlocks - 1) / kCombBlocks + VEC - 1) / VEC * VEC;
+    const size_t lo = (size_t)blockIdx.x * chunk;
+    const size_t hi = lo + chunk < n ? lo + chunk : n;
     double ss = 0.0;
-    for (size_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double v = a * (double)r1[i] + b * (double)r2[i];
-        r[i] = (T)v;
-        ss += v * v;
+    if (lo < hi) {
+        const size_t nv = (hi - lo) / VEC;                     // whole 16-byte vectors of the range
+        const Vec *u = reinterpret_cast<const Vec *>(r1 + lo);
+        const Vec *w = reinterpret_cast<const Vec *>(r2 + lo);
+        Vec *o = reinterpret_cast<Vec *>(r + lo);
+        size_t i = threadIdx.x;
+        for (; i + kBlock < nv; i += 2 * kBlock) {             // 4 16-byte loads in flight per thread
+            Vec u0, w0, u1, w1, o0, o1;
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+                u0[k] = u[i][k];
+                w0[k] = w[i][k];
+                u1[k] = u[i + kBlock][k];
+                w1[k] = w[i + kBlock][k];
+            }
+            combine_vec<T>(u0, w0, a, b, o0, ss);
+            combine_vec<T>(u1, w1, a, b, o1, ss);
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+                o[i][k] = o0[k];
+                o[i + kBlock][k] = o1[k];
+            }
+        }
+        for (; i < nv; i += kBlock) {
+            Vec u0, w0, o0;
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+                u0[k] = u[i][k];
+                w0[k] = w[i][k];
+            }
+            combine_vec<T>(u0, w0, a, b, o0, ss);
+#pragma unroll
+            for (int k = 0; k < VEC; k++) o[i][k] = o0[k];
+        }
+        const size_t t = lo + nv * VEC + threadIdx.x;          // the range's last elements (the end of r)
+        if (t < hi) {
+            const double v = a * (double)r1[t] + b * (double)r2[t];
+            r[t] = (T)v;
+            ss += v * v;
+        }
     }
     ss = block_sum_256(ss, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = ss;
@@ -741,6 +793,8 @@ int do_combine(commit_hip_plan *plan, double a, const void *r1, double b, const 
                hipStream_t st)
 {
     const size_t n = (size_t)plan->d.nVOX * plan->nS_ref;
+    for (const void *q : {r1, r2, static_cast<const void *>(r)})
+        if (reinterpret_cast<uintptr_t>(q) % 16 != 0) return fail(COMMIT_HIP_EINVAL, "residual vectors must be 16-byte aligned");
     double *part = plan->rpart + plan->d.nVOX + kSumBlocks;
     hipLaunchKernelGGL(k_combine<T>, dim3(kCombBlocks), dim3(kBlock), 0, st, n, a, static_cast<const T *>(r1), b,
                        static_cast<const T *>(r2), static_cast<T *>(r), part);

@@ -151,7 +151,8 @@ int commit_hip_A_residual(commit_hip_plan *plan, const void *x, const void *yobs
  * momentum point's residual: the residual is affine in x, so
  * r(xhat) = (1 + beta) r(x) - beta r(prev_x) for xhat = x + beta (x - prev_x),
  * one pass over three row vectors instead of the product A xhat that
- * solvers.py:363-368 recomputes (no counterpart in the reference).
+ * solvers.py:363-368 recomputes (no counterpart in the reference).  r1, r2
+ * and r must be 16-byte aligned (as hipMalloc and torch allocations are).
  */
 int commit_hip_residual_combine(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2, void *r,
                                 double *rsq, void *stream);
