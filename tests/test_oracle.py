@@ -285,6 +285,52 @@ def test_dictionary_preprocessing_matches_oracle():
     assert D["IC"]["nVOX"] == R["IC"]["nVOX"]
 
 
+def test_dictionary_preprocessing_hand_example():
+    """sort_and_normalize on a hand-made raw dictionary (core.pyx:509-525,
+    571-580): stable sort by linear voxel id, length / TRK_norm[fibre], linear
+    ids x + X(y + Y z) mapped to their rank among the mask voxels in Fortran
+    order.  Mask (3, 2, 2) with voxels (1,0,0), (2,0,0), (0,1,0), (2,1,1):
+    linear ids 1, 2, 3, 11 -> compact 0, 1, 2, 3."""
+    mask = np.zeros((3, 2, 2), dtype=bool)
+    for v in ((1, 0, 0), (2, 0, 0), (0, 1, 0), (2, 1, 1)):
+        mask[v] = True
+    raw = {"TRK_norm": np.array([2.0, 0.5, 4.0], np.float32),
+           "IC_f": np.array([0, 0, 1, 1, 2, 2], np.uint32),
+           "IC_v": np.array([11, 2, 3, 11, 2, 1], np.uint32),
+           "IC_o": np.array([4, 1, 0, 2, 3, 0], np.uint16),
+           "IC_len": np.array([0.5, 1.0, 0.25, 0.75, 0.5, 1.0], np.float32),
+           "EC_v": np.array([3, 1, 11, 1], np.uint32), "EC_o": np.array([7, 8, 9, 10], np.uint16),
+           "ISO_v": np.array([11, 2, 1, 3], np.uint32)}
+    D = oracle.sort_and_normalize(raw, mask)
+    # stable order of the segments by linear voxel: 5 | 1, 4 | 2 | 0, 3
+    assert D["IC"]["vox"].tolist() == [0, 1, 1, 2, 3, 3]
+    assert D["IC"]["dir"].tolist() == [0, 1, 3, 0, 4, 2]
+    assert D["IC"]["str"].tolist() == [2, 0, 2, 1, 0, 1]
+    assert np.array_equal(D["IC"]["len"], np.array([1.0 / 4, 1.0 / 2, 0.5 / 4, 0.25 / 0.5, 0.5 / 2, 0.75 / 0.5],
+                                                    np.float32))
+    assert D["IC"]["nVOX"] == 4 and D["IC"]["n"] == 6 and D["IC"]["nSTR"] == 3
+    assert D["EC"]["vox"].tolist() == [0, 0, 2, 3] and D["EC"]["dir"].tolist() == [8, 10, 7, 9]
+    assert D["ISO"]["vox"].tolist() == [0, 1, 2, 3]
+    U = oracle.sort_and_normalize(raw, mask, normalize=False)
+    assert np.array_equal(U["IC"]["len"], np.array([1.0, 1.0, 0.5, 0.25, 0.5, 0.75], np.float32))
+
+
+def test_kernel_normalisation_hand_example():
+    """normalize_kernels (core.pyx:412-431): every row of an IC/EC response is
+    divided by the norm of its orientation-0 row (so only row 0 gets norm 1),
+    each ISO response by its own norm."""
+    K = {"wmr": np.array([[[3.0, 4.0], [6.0, 8.0]]], np.float32),
+         "wmh": np.array([[[0.0, 2.0], [1.0, 1.0]]], np.float32),
+         "iso": np.array([[3.0, 4.0], [0.0, 0.5]], np.float32)}
+    N = oracle.normalize_kernels(K)
+    assert np.allclose(N["wmr"], [[[0.6, 0.8], [1.2, 1.6]]], rtol=1e-7, atol=0)
+    assert np.allclose(N["wmh"], [[[0.0, 1.0], [0.5, 0.5]]], rtol=1e-7, atol=0)
+    assert np.allclose(N["iso"], [[0.6, 0.8], [0.0, 1.0]], rtol=1e-7, atol=0)
+    assert N["wmr_norm"].tolist() == [5.0] and N["wmh_norm"].tolist() == [2.0]
+    assert N["iso_norm"].tolist() == [5.0, 0.5]
+    assert N["wmr"].dtype == np.float32
+
+
 def test_kernel_normalisation_matches_oracle():
     bvals, bvecs = synth.scheme((1000, 2000), 30)
     K = synth.szb_kernels(bvals, bvecs, synth.hemisphere_directions(50))
