@@ -47,6 +47,8 @@ SIGNATURES = {
     "commit_hip_prox": (_I, [_P, ctypes.c_int64, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P]),
     "commit_hip_fista_sums": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
                                    _P, _P, _P]),
+    "commit_hip_fista_sums_noalloc": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P,
+                                           _P, _P, _P]),
     "commit_hip_axpby": (_I, [ctypes.c_int64, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P]),
     "commit_hip_prox_f32": (_I, [_P, ctypes.c_int64, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P]),
     "commit_hip_fista_sums_f32": (_I, [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
@@ -71,6 +73,7 @@ _lib = None
 
 REG_NONE, REG_LASSO, REG_GROUP_LASSO, REG_SPARSE_GROUP_LASSO = 0, 1, 2, 3
 SUMS_BLOCKS = 256
+NOALLOC_MAX_GROUPS = 65536       # COMMIT_HIP_NOALLOC_MAX_GROUPS
 
 
 class ProxParams(ctypes.Structure):

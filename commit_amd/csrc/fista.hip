@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "../../include/commit_hip.h"
 #include "common.h"
@@ -297,6 +298,38 @@ int sums_impl(int64_t n, const T *x, const T *xhat, const T *grad, const T *prev
     return 0;
 }
 
+// Workspace of commit_hip_fista_sums_noalloc: a file-scope device array that
+// stands in for the `work` buffer commit_hip_fista_sums takes from its caller
+// (one copy per GPU, held until the process ends).  Its name is a host-side
+// stand-in: kernels get the device address that hipGetSymbolAddress returns,
+// looked up once per device and kept in g_scratch_ptr.  Every partial it holds
+// is written before it is read in each call, so nothing needs clearing.
+constexpr size_t kScratchDoubles = COMMIT_HIP_SUMS_WORK((size_t)COMMIT_HIP_NOALLOC_MAX_GROUPS);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch (the current device), looked up on the
+// first call on that device.
+int scratch_ptr(double **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(COMMIT_HIP_EUNSUPPORTED, "device %d outside 0..%d", dev, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(COMMIT_HIP_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 template <typename T>
 int axpby_impl(int64_t n, double a, const T *x, double b, const T *y, T *out, void *stream)
 {
@@ -342,6 +375,21 @@ int commit_hip_fista_sums_f32(int64_t n, const float *x, const float *xhat, cons
 {
     return sums_impl<float>(n, x, xhat, grad, prev, l1_range, l1_weighted, w, ngroups, group_ptr, group_idx,
                             group_w, out, work, stream);
+}
+
+int commit_hip_fista_sums_noalloc(int64_t n, const double *x, const double *xhat, const double *grad,
+                                  const double *prev, const int64_t *l1_range, const int *l1_weighted,
+                                  const double *w, int64_t ngroups, const int32_t *group_ptr,
+                                  const int32_t *group_idx, const double *group_w, double *out, void *stream)
+{
+    if (ngroups > COMMIT_HIP_NOALLOC_MAX_GROUPS)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "%lld groups exceed the %d of the fixed workspace", (long long)ngroups,
+                    COMMIT_HIP_NOALLOC_MAX_GROUPS);
+    double *work = nullptr;
+    const int rc = scratch_ptr(&work);
+    if (rc) return rc;
+    return sums_impl<double>(n, x, xhat, grad, prev, l1_range, l1_weighted, w, ngroups, group_ptr, group_idx,
+                             group_w, out, work, stream);
 }
 
 int commit_hip_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out, void *stream)

@@ -237,6 +237,28 @@ int commit_hip_fista_sums(int64_t n, const double *x, const double *xhat, const 
                           const int32_t *group_idx, const double *group_w, double *out, double *work,
                           void *stream);
 
+/* Most groups commit_hip_fista_sums_noalloc accepts. */
+#define COMMIT_HIP_NOALLOC_MAX_GROUPS 65536
+
+/*
+ * commit_hip_fista_sums without the `work` argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels, split and order as commit_hip_fista_sums, so the sums are
+ * bitwise equal to it.  The workspace is a fixed device array of
+ * COMMIT_HIP_SUMS_WORK(COMMIT_HIP_NOALLOC_MAX_GROUPS) doubles (about 540 KB)
+ * that every GPU the library runs on holds until the process ends; the call
+ * allocates nothing (no hipMalloc, no hipFree).  Runs on the current device.
+ * ngroups > COMMIT_HIP_NOALLOC_MAX_GROUPS returns COMMIT_HIP_EUNSUPPORTED.
+ * All calls on one GPU share that array: calls that overlap in time on one GPU
+ * (on different streams, or from different host threads) race; calls queued
+ * on one stream are ordered and safe.
+ */
+int commit_hip_fista_sums_noalloc(int64_t n, const double *x, const double *xhat, const double *grad,
+                                  const double *prev, const int64_t *l1_range, const int *l1_weighted,
+                                  const double *w, int64_t ngroups, const int32_t *group_ptr,
+                                  const int32_t *group_idx, const double *group_w, double *out, void *stream);
+
 /* out = a * x + b * y (FISTA extrapolation, solvers.py:360). */
 int commit_hip_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
                      void *stream);
