@@ -57,13 +57,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="brain")
     ap.add_argument("--fibres", type=int, default=None)
-    ap.add_argument("--cpu-pairs", type=int, default=2, help="timed pairs of the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-pairs", type=int, default=5, help="timed pairs of the CPU baseline (0: skip)")
     ap.add_argument("--rhs", type=int, default=16, help="right-hand sides of the batched leg (0: skip)")
     ap.add_argument("--fit-iters", type=int, default=500,
                     help="FISTA iterations of the mixed-precision fit leg (0: skip); batched-fit mode: max_iter")
-    ap.add_argument("--mode", default="headline", choices=("headline", "batched-fit", "autograd"),
+    ap.add_argument("--mode", default="headline", choices=("headline", "batched-fit", "autograd", "products"),
                     help="headline: the operator pairs/s line; batched-fit: 8 tractograms x 16 lambdas; "
-                         "autograd: differentiable operator on 200k fibres")
+                         "autograd: differentiable operator on 200k fibres; products: the single-vector "
+                         "products alone with a fixed launch option (--slab), for counter and trace runs")
+    ap.add_argument("--slab", type=int, default=2, help="products mode: slab option of the plan (0, 1, 2)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
@@ -76,6 +78,9 @@ def main():
         return
     if args.mode == "autograd":
         autograd_mode(args)
+        return
+    if args.mode == "products":
+        products_mode(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -465,6 +470,53 @@ def autograd_mode(args):
                                "loss_first": hist[0][0], "loss_last": hist[-1][0]},
                        "autotune": tuned}}
     print(json.dumps(line), flush=True)
+
+
+def products_mode(args):
+    """The single-vector products on the full-brain workload with the plan's
+    slab option fixed (no autotuning, so a profiler sees only these kernels):
+    --steps pairs in FP64, then in FP32 storage, each entry point and A^T pass
+    timed with HIP events on its launch stream."""
+    import ctypes
+
+    import torch
+    from commit_amd import _lib, autotune, synth
+    from commit_amd.operator.operator import LinearOperator
+    dev = torch.device("cuda", 0)
+    D, K, _ = synth.make_problem(args.workload, seed=1002, nF=args.fibres)
+    A = LinearOperator(D, K, None)
+    At = A.T
+    autotune.set_option(A, _lib.OPT_SLAB, args.slab)
+    lib = _lib.load()
+    h = A._plan.handle
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    g = torch.Generator(device=dev).manual_seed(1002)
+    out = {"workload": f"{args.workload}: {D['IC']['nSTR']} fibres, {D['IC']['n']} segments, "
+                       f"{D['IC']['nVOX']} voxels", "slab": args.slab, "steps": args.steps}
+    for name, prec, dt in (("fp64", _lib.FP64, torch.float64), ("fp32", _lib.FP32, torch.float32)):
+        A.set_precision(prec)
+        x = torch.rand(A.shape[1], dtype=torch.float64, device=dev, generator=g).to(dt)
+        y = torch.empty(A.shape[0], dtype=dt, device=dev)
+        xo = torch.empty_like(x)
+        ms = {}
+        for key, fn in (("A", lambda: A.matvec(x, out=y)), ("At", lambda: At.matvec(y, out=xo)),
+                        ("At_pass1", lambda: _lib.check(lib.commit_hip_At_pass(
+                            h, ctypes.c_void_p(y.data_ptr()), None, ctypes.c_void_p(xo.data_ptr()), 1, sp), "pass 1")),
+                        ("At_pass2", lambda: _lib.check(lib.commit_hip_At_pass(
+                            h, ctypes.c_void_p(y.data_ptr()), None, ctypes.c_void_p(xo.data_ptr()), 2, sp), "pass 2"))):
+            fn()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(args.steps):
+                fn()
+            ev1.record(stream)
+            ev1.synchronize()
+            ms[key] = round(ev0.elapsed_time(ev1) / args.steps, 4)
+        out[name] = ms
+        print(f"[bench] products {name}: {ms}", file=sys.stderr, flush=True)
+    A.set_precision(_lib.FP64)
+    print(json.dumps(out), flush=True)
 
 
 def copy_calibration():

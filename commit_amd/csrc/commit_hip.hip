@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_A(DevDict d, const T *__restrict__ x
 #pragma unroll
     for (int j = 0; j < NJ; j++) acc[j] = 0.0;
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    const bool tail = Lanes<NJ, kQuads<T>>::tail(lane, d.nS);
     a_comp<NJ>(acc, d.ic, d.ic.ptr[v], d.ic.ptr[v + 1], xic, (size_t)d.nF, d.nIC, d.wmr, lut_rf, d.nSp, lane, tail);
     if (d.nEC > 0)
         a_comp<NJ>(acc, d.ec, d.ec.ptr[v], d.ec.ptr[v + 1], x + (size_t)d.nIC * d.nF, (size_t)d.nE, d.nEC,
@@ -167,7 +167,7 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
 {
     const bool ic = cp.len != nullptr;
     for (int i = 0; i < nrf; i++) {
-        const float *L = lut + (size_t)i * lut_rf + lane;
+        const float *L = lut + (size_t)i * lut_rf;
         for (uint32_t base = b; base < e; base += kWave) {
             const uint32_t k = base + lane;
             const int cnt = (int)min((uint32_t)kWave, e - base);
@@ -180,11 +180,11 @@ __device__ __forceinline__ void at_comp(const double (&yv)[NJ], const Comp &cp, 
             }
             float ra[NJ], rb[NJ], rc[NJ];
             int t = 0;
-            fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, 0) * nSp, tail);
-            if (cnt > 1) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, 1) * nSp, tail);
+            fetch_row<NJ, kQuads<T>>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, 0) * nSp, lane, tail);
+            if (cnt > 1) fetch_row<NJ, kQuads<T>>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, 1) * nSp, lane, tail);
 #define AT_STAGE(CUR, NEXT2)                                                                              \
     {                                                                                                     \
-        if (t + 2 < cnt) fetch_row<NJ>(NEXT2, L + (size_t)__builtin_amdgcn_readlane((int)o, t + 2) * nSp, tail); \
+        if (t + 2 < cnt) fetch_row<NJ, kQuads<T>>(NEXT2, L + (size_t)__builtin_amdgcn_readlane((int)o, t + 2) * nSp, lane, tail); \
         ring[(size_t)(t % kRing) * kRingStride + lane] = dot_row<NJ>(CUR, yv);                            \
         if ((t & 7) == 7) {                                                                               \
             __builtin_amdgcn_wave_barrier();                                                              \
@@ -222,12 +222,12 @@ __global__ __launch_bounds__(kBlock) void k_At(DevDict d, const T *__restrict__ 
     const T *yr = y + (size_t)v * d.nS;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int s = lane + kWave * j;
+        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
         yv[j] = (s < d.nS) ? (double)stream_load(yr + s) : 0.0;
         if (sqrtw && s < d.nS) yv[j] *= (double)stream_load(sqrtw + (size_t)v * d.nS + s);   // (solvers.py:267)
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
-    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    const bool tail = Lanes<NJ, kQuads<T>>::tail(lane, d.nS);
     at_comp<NJ>(yv, d.ic, d.zpos, d.ic.ptr[v], d.ic.ptr[v + 1], z, x, 0, d.nIC, d.wmr, lut_rf, d.nSp, lane, tail,
                 ring);
     if (d.nEC > 0)

@@ -29,15 +29,49 @@ __device__ __forceinline__ T stream_load(const T *p)
     return __builtin_nontemporal_load(p);    // read once per product: keep L2 for LUT rows and x
 }
 
-// NJ coalesced loads of one padded LUT row (row points at this lane's sample
-// 0).  tail: this lane's sample of the last chunk is < nS; lanes past nS skip
-// that load (their padded entries are zero), so the padding costs no traffic.
-template <int NJ>
-__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row, bool tail)
+// Lane <-> sample layout of the NJ = nSp / 64 samples a lane holds for one
+// voxel (a LUT row, an accumulator row, a y row).  Q false: sample 64 j + l
+// for j < NJ.  Q true (the float32-vector kernels): the first 4 (NJ / 4) are
+// quads, quad q of lane l being samples 256 q + 4 l .. 256 q + 4 l + 3, read
+// as one 16-byte access (the LDS serves 16-byte reads at twice the bandwidth
+// of 4- and 8-byte ones); the remaining NJ % 4 are single samples
+// 256 (NJ / 4) + 64 r + l.  The FP64 kernels keep Q false: with their FP64
+// y rows and accumulators in registers they measured faster with the strided
+// reads.  Padded LUT samples (>= nS) are zero.
+template <int NJ, bool Q>
+struct Lanes {
+    static constexpr int NQ = Q ? NJ / 4 : 0, NR = NJ - 4 * NQ;
+    static __device__ __forceinline__ int sample(int lane, int j)
+    {
+        return j < 4 * NQ ? 256 * (j / 4) + 4 * lane + (j % 4) : 256 * NQ + 64 * (j - 4 * NQ) + lane;
+    }
+    // the lane's last single sample is < nS (a lane past nS skips that load;
+    // quads are always read whole)
+    static __device__ __forceinline__ bool tail(int lane, int nS)
+    {
+        return NR == 0 || 256 * NQ + 64 * (NR - 1) + lane < nS;
+    }
+};
+
+// the layout of the kernels whose vectors are of type T
+template <typename T>
+constexpr bool kQuads = sizeof(T) == sizeof(float);
+
+// One padded LUT row (row: its sample 0, 16-byte aligned) in the lane layout:
+// a 16-byte load per quad, a 4-byte load per single sample; without tail the
+// lane's last single sample is past nS and is not loaded (it is zero).
+template <int NJ, bool Q>
+__device__ __forceinline__ void fetch_row(float (&raw)[NJ], const float *__restrict__ row, int lane, bool tail)
 {
+    constexpr int NQ = Lanes<NJ, Q>::NQ, NR = Lanes<NJ, Q>::NR;
 #pragma unroll
-    for (int j = 0; j < NJ - 1; j++) raw[j] = row[kWave * j];
-    raw[NJ - 1] = tail ? row[kWave * (NJ - 1)] : 0.0f;
+    for (int q = 0; q < NQ; q++) {
+        const float4 v = *reinterpret_cast<const float4 *>(row + 256 * q + 4 * lane);
+        raw[4 * q] = v.x, raw[4 * q + 1] = v.y, raw[4 * q + 2] = v.z, raw[4 * q + 3] = v.w;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        raw[4 * NQ + r] = (r < NR - 1 || tail) ? row[256 * NQ + 64 * r + lane] : 0.0f;
 }
 
 // next set bit of m (consumed), or -1
@@ -66,7 +100,7 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
                                        const float *__restrict__ lut, size_t lut_rf, int nSp, int lane, bool tail)
 {
     for (int i = 0; i < nrf; i++) {
-        const float *L = lut + (size_t)i * lut_rf + lane;
+        const float *L = lut + (size_t)i * lut_rf;
         for (uint32_t base = b; base < e; base += kWave) {
             const uint32_t k = base + lane;
             double c = 0.0;
@@ -81,18 +115,18 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
             unsigned long long m = __ballot(c != 0.0);
             float ra[NJ], rb[NJ], rc[NJ];
             int ta = take(m), tb = -1, tc = -1;
-            if (ta >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
-            if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
+            if (ta >= 0) fetch_row<NJ, kQuads<T>>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, lane, tail);
+            if ((tb = take(m)) >= 0) fetch_row<NJ, kQuads<T>>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, lane, tail);
             // three rows rotate through the pipeline; entries keep their order
             while (true) {
                 if (ta < 0) break;
-                if ((tc = take(m)) >= 0) fetch_row<NJ>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp, tail);
+                if ((tc = take(m)) >= 0) fetch_row<NJ, kQuads<T>>(rc, L + (size_t)__builtin_amdgcn_readlane((int)o, tc) * nSp, lane, tail);
                 fma_row<NJ>(acc, readlane_f64(c, ta), ra);
                 if (tb < 0) break;
-                if ((ta = take(m)) >= 0) fetch_row<NJ>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, tail);
+                if ((ta = take(m)) >= 0) fetch_row<NJ, kQuads<T>>(ra, L + (size_t)__builtin_amdgcn_readlane((int)o, ta) * nSp, lane, tail);
                 fma_row<NJ>(acc, readlane_f64(c, tb), rb);
                 if (tc < 0) break;
-                if ((tb = take(m)) >= 0) fetch_row<NJ>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, tail);
+                if ((tb = take(m)) >= 0) fetch_row<NJ, kQuads<T>>(rb, L + (size_t)__builtin_amdgcn_readlane((int)o, tb) * nSp, lane, tail);
                 fma_row<NJ>(acc, readlane_f64(c, tc), rc);
             }
         }
@@ -109,7 +143,7 @@ __device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &
     if constexpr (!RES) {
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            const int s = lane + kWave * j;
+            const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
             if (s < d.nS) __builtin_nontemporal_store((T)acc[j], out.y + row + s);
         }
         return;
@@ -120,14 +154,14 @@ __device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &
     T yo[NJ], wv[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int s = lane + kWave * j;
+        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
         const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
         yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
         wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
     }
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int s = lane + kWave * j;
+        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
         if (s < d.nS) {
             double r = acc[j];
             if (out.yobs) r -= (double)yo[j];
@@ -153,7 +187,7 @@ __device__ __forceinline__ void res_load(T (&yo)[NJ], T (&wv)[NJ], const DevDict
     const size_t row = (size_t)v * d.nS;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int s = lane + kWave * j;
+        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
         const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
         yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
         wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
@@ -168,7 +202,7 @@ __device__ __forceinline__ void res_store(const double (&acc)[NJ], const T (&yo)
     double sq = 0.0;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int s = lane + kWave * j;
+        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
         if (s < d.nS) {
             double r = acc[j];
             if (out.yobs) r -= (double)yo[j];

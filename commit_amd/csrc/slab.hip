@@ -57,20 +57,24 @@ struct SlabLds {
     static constexpr int floats = 40960 * W / 16;
 };
 
-// Per-wavefront table of the current chunk: coefficients (A x) and row
-// offsets inside the slab, read back by the whole wavefront at one address
-// (an LDS broadcast) instead of lane reads on the VALU.
+// Per-wavefront table of the current chunk: coefficients (A x) or dot
+// products (A^T y) and row offsets inside the slab, read back by the whole
+// wavefront at one address (an LDS broadcast) instead of lane reads on the
+// VALU.  C: the value type (float for the float32 A x, double otherwise).
+template <typename C>
 struct Bcast {
-    double c[kWave];
+    C c[kWave];
     int32_t off[kWave];
 };
-constexpr int kBcastFloats = (int)(sizeof(Bcast) / sizeof(float));
-static_assert(sizeof(Bcast) % 16 == 0, "broadcast tables must stay 16-byte aligned");
+// LDS floats reserved for each wavefront's table (room for the double one)
+constexpr int kBcastFloats = (int)(sizeof(Bcast<double>) / sizeof(float));
+static_assert(sizeof(Bcast<double>) % 16 == 0 && sizeof(Bcast<float>) % 16 == 0,
+              "broadcast tables must stay 16-byte aligned");
 
-template <int W>
-__device__ __forceinline__ Bcast *bcast_table(float *L, int w)
+template <int W, typename C>
+__device__ __forceinline__ Bcast<C> *bcast_table(float *L, int w)
 {
-    return reinterpret_cast<Bcast *>(L + SlabLds<W>::floats - (w + 1) * kBcastFloats);
+    return reinterpret_cast<Bcast<C> *>(L + SlabLds<W>::floats - (w + 1) * kBcastFloats);
 }
 
 // Copy nfloats contiguous floats (a multiple of 64) into L: 1 KiB pieces,
@@ -85,11 +89,20 @@ __device__ __forceinline__ void slab_load(float *L, const float *__restrict__ sr
     }
 }
 
-template <int NJ>
-__device__ __forceinline__ void lds_row(float (&raw)[NJ], const float *R)
+// One LUT row of the slab in the lane layout (Lanes<NJ, Q>): Lq is the row's
+// sample 0 plus 4 lanes, Ls the same plus one lane; a 16-byte LDS read per
+// quad, a 4-byte read per single sample (padded samples are zero).
+template <int NJ, bool Q>
+__device__ __forceinline__ void lds_row(float (&raw)[NJ], const float *Lq, const float *Ls)
 {
+    constexpr int NQ = Lanes<NJ, Q>::NQ, NR = Lanes<NJ, Q>::NR;
 #pragma unroll
-    for (int j = 0; j < NJ; j++) raw[j] = R[kWave * j];
+    for (int q = 0; q < NQ; q++) {
+        const float4 v = *reinterpret_cast<const float4 *>(Lq + 256 * q);
+        raw[4 * q] = v.x, raw[4 * q + 1] = v.y, raw[4 * q + 2] = v.z, raw[4 * q + 3] = v.w;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) raw[4 * NQ + r] = Ls[256 * NQ + 64 * r];
 }
 
 // Entry records of the first chunk (up to 64 entries) of a range [b, e): one
@@ -111,17 +124,20 @@ __device__ __forceinline__ Rec rec_load(const Comp &cp, const uint32_t *__restri
 // orientation to merge.
 constexpr int kMergeMin = 12;
 
-// A x: apply up to 64 entries of one voxel and one slab, rows in LDS (Lb: this
-// lane's sample 0 of the slab's first orientation o0); lane l holds entry b+l
-// (record r, x value xv), valid when ok.  The nonzero coefficients and their
-// row offsets go, in order, to slots 0..nz-1 of the wavefront's broadcast
-// table (zero coefficients are skipped, setup_operator.py:88); the other lanes
-// fill the slots past nz with zero coefficients and valid rows.  Entries are
-// then applied in groups of 4 with all four rows in flight; a group's tail
-// adds exact zeros.
+// A x: apply up to 64 entries of one voxel and one slab, rows in LDS (Lq, Ls:
+// this lane's bases in the slab's first orientation o0, see lds_row); lane l
+// holds entry b+l (record r, x value xv), valid when ok.  The nonzero
+// coefficients and their row offsets go, in order, to slots 0..nz-1 of the
+// wavefront's broadcast table (zero coefficients are skipped,
+// setup_operator.py:88); the other lanes fill the slots past nz with zero
+// coefficients and valid rows.  Entries are then applied in groups of 4 with
+// all four rows in flight; a group's tail adds exact zeros.  T = double: FP64
+// FMAs of the FP32 rows.  T = float (blocked FP32): each group's four
+// products are summed with FP32 FMAs and the group sum is added to the FP64
+// accumulators, so the FP32 rounding is that of a 4-term sum.
 template <int NJ, typename T>
-__device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T xv, bool ok, const float *Lb,
-                                             Bcast *bt, int o0, int nSp, int lane)
+__device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T xv, bool ok, const float *Lq,
+                                             const float *Ls, Bcast<T> *bt, int o0, int nSp, int lane)
 {
     // IC coefficient l * x * eval                                  (setup_operator.py:63-96)
     double c = (double)xv;
@@ -158,21 +174,35 @@ __device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T 
     if (nz == 0) return;
     const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
     const int dst = nzl ? below : nz + (lane - below);
-    bt->c[dst] = c;
+    bt->c[dst] = (T)c;
     bt->off[dst] = ((int)r.o - o0) * nSp;
     for (int t = 0; t < nz; t += 4) {
         const int4 of = *reinterpret_cast<const int4 *>(&bt->off[t]);
-        const double2 c01 = *reinterpret_cast<const double2 *>(&bt->c[t]);
-        const double2 c23 = *reinterpret_cast<const double2 *>(&bt->c[t + 2]);
         float ra[NJ], rb[NJ], rc[NJ], rd[NJ];
-        lds_row<NJ>(ra, Lb + of.x);
-        lds_row<NJ>(rb, Lb + of.y);
-        lds_row<NJ>(rc, Lb + of.z);
-        lds_row<NJ>(rd, Lb + of.w);
-        fma_row<NJ>(acc, c01.x, ra);
-        fma_row<NJ>(acc, c01.y, rb);
-        fma_row<NJ>(acc, c23.x, rc);
-        fma_row<NJ>(acc, c23.y, rd);
+        lds_row<NJ, kQuads<T>>(ra, Lq + of.x, Ls + of.x);
+        lds_row<NJ, kQuads<T>>(rb, Lq + of.y, Ls + of.y);
+        lds_row<NJ, kQuads<T>>(rc, Lq + of.z, Ls + of.z);
+        lds_row<NJ, kQuads<T>>(rd, Lq + of.w, Ls + of.w);
+        if constexpr (sizeof(T) == sizeof(float)) {
+            const float4 cf = *reinterpret_cast<const float4 *>(&bt->c[t]);
+            float g[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                g[j] = cf.x * ra[j];
+                g[j] = fmaf(cf.y, rb[j], g[j]);
+                g[j] = fmaf(cf.z, rc[j], g[j]);
+                g[j] = fmaf(cf.w, rd[j], g[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < NJ; j++) acc[j] += (double)g[j];
+        } else {
+            const double2 c01 = *reinterpret_cast<const double2 *>(&bt->c[t]);
+            const double2 c23 = *reinterpret_cast<const double2 *>(&bt->c[t + 2]);
+            fma_row<NJ>(acc, c01.x, ra);
+            fma_row<NJ>(acc, c01.y, rb);
+            fma_row<NJ>(acc, c23.x, rc);
+            fma_row<NJ>(acc, c23.y, rd);
+        }
     }
 }
 
@@ -182,8 +212,8 @@ __device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T 
 // applied.  Later chunks of a long range are loaded in place.
 template <int NJ, int V, typename T>
 __device__ __forceinline__ void slab_a_pass(double (&acc)[V][NJ], const Comp &cp, const uint32_t (&rb)[V],
-                                            const uint32_t (&re)[V], const T *__restrict__ xb, const float *Lb,
-                                            Bcast *bt, int o0, int nSp, int lane, Rec r0, Rec r1)
+                                            const uint32_t (&re)[V], const T *__restrict__ xb, const float *Lq,
+                                            const float *Ls, Bcast<T> *bt, int o0, int nSp, int lane, Rec r0, Rec r1)
 {
     T x0 = xb[r0.col];
 #pragma unroll
@@ -193,10 +223,10 @@ __device__ __forceinline__ void slab_a_pass(double (&acc)[V][NJ], const Comp &cp
         T x1 = x0;
         if (u + 1 < V) x1 = xb[r1.col];
         if (rb[u] < re[u]) {
-            slab_a_chunk<NJ, T>(acc[u], r0, x0, rb[u] + lane < re[u], Lb, bt, o0, nSp, lane);
+            slab_a_chunk<NJ, T>(acc[u], r0, x0, rb[u] + lane < re[u], Lq, Ls, bt, o0, nSp, lane);
             for (uint32_t base = rb[u] + kWave; base < re[u]; base += kWave) {
                 const Rec r = rec_load(cp, cp.col, base, re[u], lane);
-                slab_a_chunk<NJ, T>(acc[u], r, xb[r.col], base + lane < re[u], Lb, bt, o0, nSp, lane);
+                slab_a_chunk<NJ, T>(acc[u], r, xb[r.col], base + lane < re[u], Lq, Ls, bt, o0, nSp, lane);
             }
         }
         r0 = r1;
@@ -263,7 +293,7 @@ __device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x
     __shared__ __align__(16) float L[SlabLds<W>::floats];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    Bcast *bt = bcast_table<W>(L, w);
+    Bcast<T> *bt = bcast_table<W, T>(L, w);
     const int32_t *vl = sa.vlist + ((size_t)blockIdx.x * W + w) * V;
     int vox[V];
 #pragma unroll
@@ -292,10 +322,10 @@ __device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x
             slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows * d.nSp, w, lane);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();                                         // slab in LDS
-            slab_a_pass<NJ, V, T>(acc, d.ic, rb, re, xb, L + lane, bt, o0, d.nSp, lane, r0, r1);
+            slab_a_pass<NJ, V, T>(acc, d.ic, rb, re, xb, L + 4 * lane, L + lane, bt, o0, d.nSp, lane, r0, r1);
         }
     }
-    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    const bool tail = Lanes<NJ, kQuads<T>>::tail(lane, d.nS);
     // EC and ISO: coefficient x of every item of the wavefront's voxels at once
     const Item it = item_decode<V>(d, vox, lut_rf, lane);
     double cit = 0.0;
@@ -311,7 +341,7 @@ __device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++)
                     if (j0 + jj < m)
-                        fetch_row<NJ>(r[jj], readlane_ptr(it.row, kItemLanes * u + j0 + jj) + lane, tail);
+                        fetch_row<NJ, kQuads<T>>(r[jj], readlane_ptr(it.row, kItemLanes * u + j0 + jj), lane, tail);
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++)
                     if (j0 + jj < m) fma_row<NJ>(acc[u], readlane_f64(cit, kItemLanes * u + j0 + jj), r[jj]);
@@ -359,6 +389,9 @@ __global__ __launch_bounds__(W * kWave, 4) void k_A_slab_res(DevDict d, const T 
     a_slab<NJ, V, W, T, true>(d, x, xic, out, sa);
 }
 
+// This lane's part of LUT[o] . y[v]: FP64 FMAs for a double y row; for a
+// float y row, FP32 FMAs over the lane's NJ samples, widened once (the 64
+// partials are then added in FP64).
 template <int NJ>
 __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (&yv)[NJ])
 {
@@ -366,6 +399,15 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (
 #pragma unroll
     for (int j = 0; j < NJ; j++) a = fma((double)raw[j], yv[j], a);
     return a;
+}
+
+template <int NJ>
+__device__ __forceinline__ double dot_row(const float (&raw)[NJ], const float (&yv)[NJ])
+{
+    float a = raw[0] * yv[0];
+#pragma unroll
+    for (int j = 1; j < NJ; j++) a = fmaf(raw[j], yv[j], a);
+    return (double)a;
 }
 
 // A^T y pass 1: up to 64 IC entries of one voxel and one slab, lane l holding
@@ -379,10 +421,12 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const double (
 // c[t0+g]).  Slots past nu in the last group read valid rows and are not
 // kept.  Each entry then reads its run's dot from the table and stores
 // l * dot.                                            (setup_operator.py:384-428)
-template <int NJ>
-__device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec &r, int cnt, double *__restrict__ zi,
-                                              const float *Lb, Bcast *bt, int o0, int nSp, int lane)
+template <int NJ, typename Y>
+__device__ __forceinline__ void slab_at_chunk(const Y (&yv)[NJ], const Rec &r, int cnt, double *__restrict__ zi,
+                                              const float *Lq, const float *Ls, Bcast<double> *bt, int o0, int nSp,
+                                              int lane)
 {
+    constexpr bool Q = kQuads<Y>;
     const uint32_t op = (uint32_t)__shfl((int)r.o, lane > 0 ? lane - 1 : 0);
     const bool head = lane < cnt && (lane == 0 || r.o != op);
     const unsigned long long m = __ballot(head);
@@ -399,7 +443,7 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             float raw[NJ];
-            lds_row<NJ>(raw, Lb + of[q]);
+            lds_row<NJ, Q>(raw, Lq + of[q], Ls + of[q]);
             p[q] = dot_row<NJ>(raw, yv);
         }
         const int tn = t0 + 8 < kWave ? t0 + 8 : t0;      // next group's offsets load under the reduction
@@ -415,7 +459,7 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             float raw[NJ];
-            lds_row<NJ>(raw, Lb + of[q]);
+            lds_row<NJ, Q>(raw, Lq + of[q], Ls + of[q]);
             p[q] = dot_row<NJ>(raw, yv);
         }
         const double sum = reduce4(p);
@@ -428,16 +472,16 @@ __device__ __forceinline__ void slab_at_chunk(const double (&yv)[NJ], const Rec 
 // EC / ISO entries [b, e) of one voxel, rows from global memory:
 // out[col + i*xs] = dot                                 (setup_operator.py:444-563)
 template <int NJ, typename T>
-__device__ __forceinline__ void glob_at(const double (&yv)[NJ], const Comp &cp, uint32_t b, uint32_t e,
+__device__ __forceinline__ void glob_at(const T (&yv)[NJ], const Comp &cp, uint32_t b, uint32_t e,
                                         T *__restrict__ out, size_t xs, int nrf, const float *__restrict__ lut,
                                         size_t lut_rf, int nSp, int lane, bool tail)
 {
     for (int i = 0; i < nrf; i++) {
-        const float *Li = lut + (size_t)i * lut_rf + lane;
+        const float *Li = lut + (size_t)i * lut_rf;
         for (uint32_t k = b; k < e; k++) {
             const uint32_t o = cp.o ? (uint32_t)cp.o[k] : 0u;
             float raw[NJ];
-            fetch_row<NJ>(raw, Li + (size_t)o * nSp, tail);
+            fetch_row<NJ, kQuads<T>>(raw, Li + (size_t)o * nSp, lane, tail);
             const double sum = wave_sum(dot_row<NJ>(raw, yv));
             if (lane == 0) out[(size_t)cp.col[k] + (size_t)i * xs] = (T)sum;
         }
@@ -451,28 +495,29 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
     __shared__ __align__(16) float L[SlabLds<W>::floats];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    Bcast *bt = bcast_table<W>(L, w);
+    Bcast<double> *bt = bcast_table<W, double>(L, w);
     const int32_t *vl = sa.vlist + ((size_t)blockIdx.x * W + w) * V;
     int vox[V];
 #pragma unroll
     for (int u = 0; u < V; u++) vox[u] = __builtin_amdgcn_readfirstlane(vl[u]);
-    // y rows of the wavefront's voxels (times sqrtw)          (solvers.py:267)
-    double yv[V][NJ];
+    // y rows of the wavefront's voxels (times sqrtw), in the vectors' type:
+    // FP64 dot products for double, FP32 lane partials for float (solvers.py:267)
+    T yv[V][NJ];
 #pragma unroll
     for (int u = 0; u < V; u++) {
         const size_t row = (size_t)(vox[u] < 0 ? 0 : vox[u]) * d.nS;
         T raw[NJ], wv[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            const int s = lane + kWave * j;
+            const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
             raw[j] = stream_load(y + row + (s < d.nS ? s : 0));
             wv[j] = sqrtw ? stream_load(sqrtw + row + (s < d.nS ? s : 0)) : (T)1;
         }
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            double t = (double)raw[j];
-            if (sqrtw) t *= (double)wv[j];
-            yv[u][j] = (vox[u] >= 0 && lane + kWave * j < d.nS) ? t : 0.0;
+            T t = raw[j];
+            if (sqrtw) t *= wv[j];
+            yv[u][j] = (vox[u] >= 0 && Lanes<NJ, kQuads<T>>::sample(lane, j) < d.nS) ? t : (T)0;
         }
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
@@ -492,23 +537,23 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
             slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows * d.nSp, w, lane);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
-            const float *Lb = L + lane;
+            const float *Lq = L + 4 * lane, *Ls = L + lane;
 #pragma unroll
             for (int u = 0; u < V; u++) {
                 Rec r1 = r0;                                            // next voxel's records in flight
                 if (u + 1 < V) r1 = rec_load(d.ic, d.zpos, rb[u + 1 < V ? u + 1 : 0], re[u + 1 < V ? u + 1 : 0], lane);
                 if (rb[u] < re[u]) {
-                    slab_at_chunk<NJ>(yv[u], r0, (int)min((uint32_t)kWave, re[u] - rb[u]), zi, Lb, bt, o0, d.nSp,
-                                      lane);
+                    slab_at_chunk<NJ, T>(yv[u], r0, (int)min((uint32_t)kWave, re[u] - rb[u]), zi, Lq, Ls, bt, o0,
+                                         d.nSp, lane);
                     for (uint32_t base = rb[u] + kWave; base < re[u]; base += kWave)
-                        slab_at_chunk<NJ>(yv[u], rec_load(d.ic, d.zpos, base, re[u], lane),
-                                          (int)min((uint32_t)kWave, re[u] - base), zi, Lb, bt, o0, d.nSp, lane);
+                        slab_at_chunk<NJ, T>(yv[u], rec_load(d.ic, d.zpos, base, re[u], lane),
+                                             (int)min((uint32_t)kWave, re[u] - base), zi, Lq, Ls, bt, o0, d.nSp, lane);
                 }
                 r0 = r1;
             }
         }
     }
-    const bool tail = lane + kWave * (NJ - 1) < d.nS;
+    const bool tail = Lanes<NJ, kQuads<T>>::tail(lane, d.nS);
     // EC and ISO: a voxel's items (at most 8) reduced together by reduce8;
     // item j ends in lanes 8j..8j+7 and lane 8j stores it (setup_operator.py:444-563)
     const Item it = item_decode<V>(d, vox, lut_rf, lane);
@@ -525,7 +570,7 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
                 p[jj] = 0.0;
                 if (jj < m) {
                     float raw[NJ];
-                    fetch_row<NJ>(raw, readlane_ptr(it.row, kItemLanes * u + jj) + lane, tail);
+                    fetch_row<NJ, kQuads<T>>(raw, readlane_ptr(it.row, kItemLanes * u + jj), lane, tail);
                     p[jj] = dot_row<NJ>(raw, yv[u]);
                 }
             }
