@@ -36,7 +36,10 @@ class OperatorFunction(torch.autograd.Function):
 
     x: device vector of the operator's precision; wmr [nIC,ndirs,nS],
     wmh [nEC,ndirs,nS], iso [nISO,nS]: float32 device tensors (or None to keep
-    the plan's table).  Gradients: x -> A^T gy; tables -> commit_hip_lut_grad."""
+    the plan's table).  Gradients: x -> A^T gy; tables -> commit_hip_lut_grad.
+    The tables are uploaded into the operator's device plan, which every
+    operator sharing it uses: they stay there after the call (a later
+    A.dot runs with them) until other tables are uploaded."""
 
     @staticmethod
     def forward(ctx, x, wmr, wmh, iso, op):
@@ -98,7 +101,10 @@ def fit_diffusivities(op, x, y, K, theta0, iters=30, d_isos=(1.7e-3, 3.0e-3), hi
 
     op: LinearOperator (its precision sets the vector dtype); x, y: device
     vectors; K: kernels dict holding 'bvals', 'bvecs' and the directions under
-    'dirs'; theta0: starting (d_par, d_perp).  Returns the fitted pair."""
+    'dirs'; theta0: starting (d_par, d_perp).  Returns the fitted pair.
+    On return the device plan holds the tables of the fitted pair (the last
+    L-BFGS evaluation need not be at it), and op.KERNELS['wmr' / 'wmh' /
+    'iso'] are replaced by those float32 tables."""
     dev = op.device
     scale = 1e-3
     p = torch.tensor([theta0[0] / scale, theta0[1] / scale], dtype=torch.float64, device=dev, requires_grad=True)
@@ -118,4 +124,15 @@ def fit_diffusivities(op, x, y, K, theta0, iters=30, d_isos=(1.7e-3, 3.0e-3), hi
         return loss
 
     opt.step(closure)
-    return float(p[0].item()) * scale, float(p[1].item()) * scale
+    dp, dq = float(p[0].item()) * scale, float(p[1].item()) * scale
+    with torch.no_grad():
+        tt = lambda v: torch.tensor(v, dtype=torch.float64, device=dev)
+        wmr, wmh, iso = (t.float().contiguous() for t in
+                         szb_tables(tt(dp), tt(dq), K["bvals"], K["bvecs"], K["dirs"], d_isos))
+        _upload(op, wmr, wmh if K["wmh"].shape[0] else None, iso)
+    if isinstance(op.KERNELS, dict):
+        op.KERNELS["wmr"] = wmr.cpu().numpy()
+        if K["wmh"].shape[0]:
+            op.KERNELS["wmh"] = wmh.cpu().numpy()
+        op.KERNELS["iso"] = iso.cpu().numpy()
+    return dp, dq

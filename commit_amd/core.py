@@ -591,13 +591,23 @@ class Evaluation:
             self.set_config("confidence_map_filename", confidence_map_filename)
             D = self.DICTIONARY
             confidence_array = cm[D["MASK_ix"], D["MASK_iy"], D["MASK_iz"], :].flatten().astype(np.float32)
-            confidence_array[np.isnan(confidence_array)] = 0.0
+            changed = False
+            if np.isnan(confidence_array).any():               # (core.pyx:1434-1437)
+                confidence_array[np.isnan(confidence_array)] = 0.0
+                changed = True
             cmax, cmin = confidence_array.max(), confidence_array.min()
-            if cmin == cmax:
+            if cmin == cmax:                                    # a constant map is not used
                 confidence_array = None
-            elif confidence_map_rescale:
-                confidence_array = (confidence_array - cmin) / (cmax - cmin)
-            if confidence_array is not None:
+                self.set_config("confidence_map_filename", None)
+            else:
+                if confidence_map_rescale:
+                    confidence_array = (confidence_array - cmin) / (cmax - cmin)
+                    changed = True
+                if changed:
+                    # the processed values replace the map at the mask voxels, so that
+                    # save_results weights its adjusted errors with them (core.pyx:1454-1457)
+                    cm[D["MASK_ix"], D["MASK_iy"], D["MASK_iz"], :] = np.reshape(
+                        confidence_array, (D["IC"]["nVOX"], -1)).astype(np.float32)
                 self.confidence_map_img = cm
         if x0 is not None and x0.shape[0] != self.A.shape[1]:
             raise ValueError("x0 dimension does not match the number of columns of the dictionary")
@@ -685,19 +695,42 @@ class Evaluation:
         ix, iy, iz = D["MASK_ix"], D["MASK_iy"], D["MASK_iz"]
         nVOX = D["IC"]["nVOX"]
         y_est_all = np.asarray(self.A.dot(self.x))
-        y_mea = np.reshape(self.niiDWI_img[ix, iy, iz, :].flatten().astype(np.float32), (nVOX, -1))
-        y_est = np.reshape(y_est_all, (nVOX, -1)).astype(np.float32)
         stats = {}
-        tmp = np.sqrt(np.mean((y_mea - y_est) ** 2, axis=1))
-        stats["RMSE"] = (float(tmp.mean()), float(tmp.std()))
-        self._save_map(tmp, pjoin(out, "fit_RMSE.nii.gz"), dim, affine, pix)
-        den = np.sum(y_mea ** 2, axis=1)
-        idx = den < 1e-12
-        den[idx] = 1
-        tmp = np.sqrt(np.sum((y_mea - y_est) ** 2, axis=1) / den)
-        tmp[idx] = 0
-        stats["NRMSE"] = (float(tmp.mean()), float(tmp.std()))
-        self._save_map(tmp, pjoin(out, "fit_NRMSE.nii.gz"), dim, affine, pix)
+
+        def rmse_nrmse(ym, ye):
+            r = np.sqrt(np.mean((ym - ye) ** 2, axis=1))
+            den = np.sum(ym ** 2, axis=1)
+            idx = den < 1e-12
+            den[idx] = 1
+            n = np.sqrt(np.sum((ym - ye) ** 2, axis=1) / den)
+            n[idx] = 0
+            return r, n
+
+        if self.debias_mask is not None:
+            # after a debiased fit the errors compare A x with the masked signal
+            # the fit saw (get_y); the statistics cover the rows in the mask only
+            # (core.pyx:1624-1658)
+            y_fit = self.get_y()
+            nS = self.niiDWI_img.shape[3]
+            nV = int(np.sum(self.debias_mask) / nS)
+            if nV == 0:
+                raise ValueError("the debias mask holds no voxel: the regularised fit left every IC "
+                                 "coefficient at zero")
+            ind = np.nonzero(self.debias_mask > 0)[0]
+            r, n = rmse_nrmse(np.reshape(y_fit[ind], (nV, -1)), np.reshape(y_est_all[ind], (nV, -1)))
+            stats["RMSE"] = (float(r.mean()), float(r.std()))
+            stats["NRMSE"] = (float(n.mean()), float(n.std()))
+            y_mea = np.reshape(y_fit, (nVOX, -1))
+            y_est = np.reshape(y_est_all, (nVOX, -1)).astype(np.float32)
+            r, n = rmse_nrmse(y_mea, y_est)
+        else:
+            y_mea = np.reshape(self.niiDWI_img[ix, iy, iz, :].flatten().astype(np.float32), (nVOX, -1))
+            y_est = np.reshape(y_est_all, (nVOX, -1)).astype(np.float32)
+            r, n = rmse_nrmse(y_mea, y_est)
+            stats["RMSE"] = (float(r.mean()), float(r.std()))
+            stats["NRMSE"] = (float(n.mean()), float(n.std()))
+        self._save_map(r, pjoin(out, "fit_RMSE.nii.gz"), dim, affine, pix)
+        self._save_map(n, pjoin(out, "fit_NRMSE.nii.gz"), dim, affine, pix)
         if self.confidence_map_img is not None:
             ca = np.reshape(self.confidence_map_img[ix, iy, iz, :].flatten().astype(np.float32), (nVOX, -1))
             den = np.sum(ca, axis=1)
@@ -727,8 +760,11 @@ class Evaluation:
         else:
             maps["EC"] = np.zeros(nVOX, np.float32)
         if len(K["iso"]) > 0:
+            # the reference's map takes the block of the first ISO response
+            # function only (offset_iso = offset + ISO['n'], core.pyx:1737-1740),
+            # also with several ISO RFs; kept, so the maps are the reference's
             o2 = off + nE * len(K["wmh"])
-            t = x[o2:o2 + D["ISO"]["n"] * len(K["iso"])].reshape((-1, D["ISO"]["n"])).sum(axis=0)
+            t = x[o2:o2 + D["ISO"]["n"]].reshape((-1, D["ISO"]["n"])).sum(axis=0)
             maps["ISO"] = np.bincount(D["ISO"]["vox"], weights=t, minlength=nVOX).astype(np.float32)
         else:
             maps["ISO"] = np.zeros(nVOX, np.float32)

@@ -20,13 +20,22 @@ CPU restatement of the reference algorithm for the operator + FISTA hot path:
                                terms are those of setup_operator.py:62-218)
 * ``normalize_kernels``     <- Evaluation.load_kernels        core.pyx:412-431
 * ``sort_and_normalize``    <- Evaluation.load_dictionary     core.pyx:509-525, 571-580
+* ``lambda_max_lasso`` / ``lambda_max_group``  <- set_regularisation  core.pyx:961-976
+* ``group_weights``         <- set_regularisation             core.pyx:1123-1192
+* ``debias_eval`` / ``debias_rows``  <- fit (debias)          core.pyx:1476-1501
+* ``confidence``            <- fit (confidence map)           core.pyx:1434-1457
+* ``fit_errors`` / ``compartment_maps``  <- save_results      core.pyx:1614-1756
+* ``iso_weights``           <- set_wLasso_ISO                 core.pyx:828-843
 
 Pinning (tests/test_oracle.py): the C operator against the dense brute-force
 assembly, adjointness, the single-segment closed form and the all-ones LUT
 incidence case; the proximals against brute-force minimisation of
 0.5||u-x||^2 + lam*Omega(u); FISTA against scipy.optimize.nnls on small dense
-problems, KKT conditions, and the orthogonal-A closed form max(A^T y, 0).
-No reference golden vectors exist offline (their inputs are network-only).
+problems, KKT conditions, and the orthogonal-A closed form max(A^T y, 0);
+the FISTA trajectory against hand-derived iterates, backtracking counts and
+stopping cases (tests/test_oracle_fista.py); the Evaluation pieces against
+hand-computed examples (tests/test_oracle_core.py).  No reference golden
+vectors exist offline (their inputs are network-only).
 """
 import ctypes
 import os
@@ -653,3 +662,193 @@ def fista(y, A, At, omega, prox, sqrt_W=None, tol_fun=1e-4, tol_x=1e-6, max_iter
         "iterations": it, "stopping_criterion": criterion,
     }
     return x, details
+
+
+# ------------------------------------------------ Evaluation (core.pyx) --
+# Restatements of the pieces of commit.Evaluation that sit around the fit.
+def lambda_max_lasso(aty, start, size, w_coeff):
+    """compute_lambda_max_lasso (core.pyx:961-966): max |A^T y| / w over
+    the compartment's slice [start, start + size)."""
+    return np.max(np.abs(aty[start:start + size]) / w_coeff)
+
+
+def lambda_max_group(aty, w_group, idx_group):
+    """compute_lambda_max_group (core.pyx:968-976): max over groups of
+    ||(A^T y)[g]|| / w_g."""
+    norm_group = np.zeros(w_group.shape, dtype=np.float64)
+    for g in range(w_group.size):
+        norm_group[g] = np.sqrt(np.sum(aty[idx_group[g]] ** 2)) / w_group[g]
+    return np.max(norm_group)
+
+
+def group_weights(group_idx, kept, nSTR, extra=None, cardinality=True, adaptive=True, x_nnls=None):
+    """Group structure and weights of set_regularisation (core.pyx:1123-1192):
+    empty groups are dropped; with streamlines left out of the dictionary
+    (kept == 0) each group's indices are mapped to the kept ones and groups
+    left empty are dropped with their weight; then weights * sqrt(|g|) and
+    / (||x_nnls[g]|| + 1e-12).  Returns (groups, weights)."""
+    groups = [np.asarray(g) for g in group_idx if np.asarray(g).size > 0]
+    w = np.array(extra, dtype=np.float64) if extra is not None else np.ones(len(groups), dtype=np.float64)
+    if 0 in kept:
+        idx_in_kept = np.zeros(kept.size, dtype=np.int32) - 1
+        idx_in_kept[kept == 1] = list(range(nSTR))
+        new_g, new_w = [], []
+        for count, group in enumerate(groups):
+            group = idx_in_kept[group]
+            group = np.delete(group, np.where(group == -1)[0])
+            if group.size > 0:
+                new_g.append(group)
+                new_w.append(w[count])
+        groups, w = new_g, np.array(new_w, dtype=np.float64)
+    if cardinality:
+        w = w * np.sqrt(np.array([g.size for g in groups], dtype=np.int32))
+    if adaptive:
+        w = w / np.array([np.linalg.norm(x_nnls[g]) + 1e-12 for g in groups], dtype=np.float64)
+    return groups, w
+
+
+def debias_eval(x, offset):
+    """The debias step's IC column mask (core.pyx:1481-1483): 0 where
+    x < 1e-15."""
+    mask = np.ones(offset, dtype=np.uint32)
+    mask[x[:offset] < 0.000000000000001] = 0
+    return mask
+
+
+def debias_rows(A_masked, x, mask, offset):
+    """The debias step's row mask (core.pyx:1494-1501): A x_debias binarised,
+    x_debias being x with the masked IC columns and every EC/ISO column set
+    to zero, A the operator built with the new eval mask."""
+    x_debias = x.copy()
+    x_debias[:offset] *= mask
+    x_debias[offset:] = 0
+    y_mask = np.asarray(A_masked.dot(x_debias))
+    y_mask[y_mask < 0] = 0
+    y_mask[y_mask > 0] = 1
+    return y_mask
+
+
+def confidence(values, rescale):
+    """Confidence values of fit (core.pyx:1434-1457): NaN -> 0; a constant
+    map is dropped (None); rescale maps [min, max] to [0, 1].  Returns
+    (array or None, whether the values were changed)."""
+    a = values.astype(np.float32).copy()
+    changed = False
+    if np.isnan(a).any():
+        a[np.isnan(a)] = 0.0
+        changed = True
+    cmax, cmin = np.max(a), np.min(a)
+    if cmin == cmax:
+        return None, False
+    if rescale:
+        a = (a - cmin) / (cmax - cmin)
+        changed = True
+    return a, changed
+
+
+def _rmse_nrmse(y_mea, y_est):
+    rmse = np.sqrt(np.mean((y_mea - y_est) ** 2, axis=1))
+    den = np.sum(y_mea ** 2, axis=1)
+    idx = np.where(den < 1e-12)
+    den[idx] = 1
+    nrmse = np.sqrt(np.sum((y_mea - y_est) ** 2, axis=1) / den)
+    nrmse[idx] = 0
+    return rmse, nrmse
+
+
+def fit_errors(y_dwi_mask, y_fit, y_est_all, nVOX, nS, debias_mask=None, conf=None):
+    """Fit error maps and statistics of save_results (core.pyx:1614-1707).
+    y_dwi_mask: the DWI at the mask voxels, voxel-major; y_fit: get_y() (the
+    signal the fit saw, times the debias mask when set); y_est_all: A x.
+    Without debias the maps and statistics use the DWI (as float32) against
+    A x (float32).  With debias the statistics cover only the rows in the
+    mask and the maps use get_y() against A x.  conf: the confidence values
+    at the mask voxels (the RMSE/NRMSE adjusted maps).  Returns a dict of
+    per-voxel maps and (mean, std) statistics."""
+    out = {}
+    if debias_mask is not None:
+        nV = int(np.sum(debias_mask) / nS)
+        ind = np.where(debias_mask > 0)[0]
+        r, n = _rmse_nrmse(np.reshape(y_fit[ind], (nV, -1)), np.reshape(np.asarray(y_est_all)[ind], (nV, -1)))
+        out["RMSE_stats"], out["NRMSE_stats"] = (r.mean(), r.std()), (n.mean(), n.std())
+        y_mea = np.reshape(y_fit, (nVOX, -1))
+        y_est = np.reshape(y_est_all, (nVOX, -1)).astype(np.float32)
+        out["RMSE"], out["NRMSE"] = _rmse_nrmse(y_mea, y_est)
+    else:
+        y_mea = np.reshape(y_dwi_mask.astype(np.float32), (nVOX, -1))
+        y_est = np.reshape(y_est_all, (nVOX, -1)).astype(np.float32)
+        r, n = _rmse_nrmse(y_mea, y_est)
+        out["RMSE"], out["NRMSE"] = r, n
+        out["RMSE_stats"], out["NRMSE_stats"] = (r.mean(), r.std()), (n.mean(), n.std())
+    if conf is not None:
+        ca = np.reshape(conf.astype(np.float32), (nVOX, -1))
+        den = np.sum(ca, axis=1)
+        idx = np.where(den < 1e-12)
+        den[idx] = 1
+        t = np.sqrt(np.sum(ca * (y_mea - y_est) ** 2, axis=1) / den)
+        t[idx] = 0
+        out["RMSE_adjusted"] = t
+        den = np.sum(ca * y_mea ** 2, axis=1)
+        idx = np.where(den < 1e-12)
+        den[idx] = 1
+        t = np.sqrt(np.sum(ca * (y_mea - y_est) ** 2, axis=1) / den)
+        t[idx] = 0
+        out["NRMSE_adjusted"] = t
+    return out
+
+
+def compartment_maps(x, DICTIONARY, KERNELS, nVOX, normalize_kernels=True, normalize_maps=False):
+    """Voxelwise compartment contributions of save_results (core.pyx:1709-1756)
+    from the fitted x: renormalised by the kernel and fibre norms when the
+    kernels were normalised (core.pyx:1595-1601), IC summed over RFs and
+    spread over each fibre's segments by length, EC summed over RFs per peak,
+    ISO from the first ISO response function's block only (the reference's
+    slice x[offset : offset + nISOvoxels]).  normalize_maps divides each by
+    the voxel's total + 1e-16.  Returns float32 (IC, EC, ISO)."""
+    D, K = DICTIONARY, KERNELS
+    nSTR, nE, nI = D["IC"]["nSTR"], D["EC"]["n"], D["ISO"]["n"]
+    nIC, nEC = K["wmr"].shape[0], K["wmh"].shape[0]
+    norm_fib = np.ones(nSTR)
+    if normalize_kernels:
+        norm1 = np.repeat(K["wmr_norm"], nSTR)
+        norm2 = np.repeat(K["wmh_norm"], nE)
+        norm3 = np.repeat(K["iso_norm"], nI)
+        norm_fib = np.kron(np.ones(nIC), D["TRK"]["norm"])
+        x = x / np.hstack((norm1 * norm_fib, norm2, norm3))
+    ic = np.zeros(nVOX, np.float32)
+    ec = np.zeros(nVOX, np.float32)
+    iso = np.zeros(nVOX, np.float32)
+    if nIC > 0:
+        off = nSTR * nIC
+        t = (x[:off].reshape((-1, nSTR)) * norm_fib.reshape((-1, nSTR))).sum(axis=0)
+        ic = np.bincount(D["IC"]["vox"], minlength=nVOX, weights=t[D["IC"]["str"]] * D["IC"]["len"]).astype(np.float32)
+    if nEC > 0:
+        off = nSTR * nIC
+        t = x[off:off + nE * nEC].reshape((-1, nE)).sum(axis=0)
+        ec = np.bincount(D["EC"]["vox"], weights=t, minlength=nVOX).astype(np.float32)
+    if K["iso"].shape[0] > 0:
+        off = nSTR * nIC + nE * nEC
+        t = x[off:off + nI].reshape((-1, nI)).sum(axis=0)
+        iso = np.bincount(D["ISO"]["vox"], weights=t, minlength=nVOX).astype(np.float32)
+    if normalize_maps:
+        tot = ic + ec + iso + 1e-16
+        return ic / tot, ec / tot, iso / tot
+    return ic, ec, iso
+
+
+def iso_weights(img, DICTIONARY, nISO):
+    """Weights of set_wLasso_ISO (core.pyx:828-843): NaN -> 1, the image at
+    the mask voxels (compact order) indexed by each ISO entry's voxel, tiled
+    over the ISO response functions; every weight must be >= 1."""
+    D = DICTIONARY
+    w_img = img.astype(np.float32).copy()
+    w_img[np.isnan(w_img)] = 1.0
+    w = w_img[D["MASK_ix"], D["MASK_iy"], D["MASK_iz"]].flatten().astype(np.float32)
+    w = w[D["ISO"]["vox"]]
+    if w.size != D["ISO"]["n"]:
+        raise ValueError("Number of voxels in the weights image does not match the number of voxels in the dictionary")
+    if np.any(w < 1):
+        raise ValueError("All weights must be greater or equal to 1")
+    if nISO > 1:
+        w = np.tile(w, nISO)
+    return w

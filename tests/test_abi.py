@@ -66,3 +66,14 @@ def test_oracle_is_not_imported_by_the_package():
             if f.endswith((".py", ".hip", ".c", ".h")):
                 src = open(os.path.join(dirpath, f), encoding="utf-8", errors="replace").read()
                 assert "oracle" not in re.sub(r"#.*|//.*", "", src).lower().replace("oracle_", "x"), f
+
+
+def test_plan_create_rejects_more_than_twenty_response_functions():
+    """The reference generates code for at most 20 RFs per compartment
+    (setup_operator.py:70, 128, 183); 21 is refused before any device call."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    for rf in ((21, 0, 0), (1, 21, 0), (1, 0, 21)):
+        rc = lib.commit_hip_plan_create(ctypes.byref(h), 8, 1, 1, 0, 0, 0, 1, None, None, None, None, None,
+                                        None, None, None, None, None, None, rf[0], rf[1], rf[2], 0, 0, 0)
+        assert rc == -4 and b"response functions" in lib.commit_hip_last_error(), rf

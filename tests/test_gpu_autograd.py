@@ -116,3 +116,124 @@ def test_fit_diffusivities_recovers_truth():
           f"(loss {hist[0][0]:.3e} -> {hist[-1][0]:.3e})", flush=True)
     assert abs(dp - truth[0]) <= 1e-3 * truth[0]
     assert abs(dq - truth[1]) <= 1e-3 * truth[1]
+
+
+# ------------------------------------------ 200k fibres, nS=288, float32 --
+@pytest.fixture(scope="module")
+def brain200k():
+    """The differentiable-operator workload: 200k fibres on the full-brain grid,
+    nS = 288, float32 vectors, LDS-slab products."""
+    from commit_amd import autotune
+    D, K, raw = synth.make_problem("brain200k", seed=1003)
+    K = dict(K)
+    K["dirs"] = raw["dirs"]
+    A = LinearOperator(D, K, None, precision=_lib.FP32)
+    autotune.set_option(A, _lib.OPT_SLAB, 2)
+    print(f"brain200k: {D['IC']['n']} segments, {D['IC']['nVOX']} voxels", flush=True)
+    return D, K, A
+
+
+def _set_tables(A, wmr, wmh, iso):
+    lib = _lib.load()
+    P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.commit_hip_plan_set_lut(A._plan.handle, P(wmr), P(wmh), P(iso), st), "commit_hip_plan_set_lut")
+
+
+def test_lut_grad_bilinear_identity_200k(brain200k):
+    """A is linear in the tables, so for any perturbation delta of one table
+    sum(G * delta) = <gy, A(delta) x> exactly (G the table gradient of
+    <gy, A x>); checked per table with the other tables set to zero."""
+    D, K, A = brain200k
+    g = torch.Generator(device="cuda").manual_seed(21)
+    x = torch.rand(A.shape[1], dtype=torch.float32, device="cuda", generator=g)
+    gy = torch.randn(A.shape[0], dtype=torch.float32, device="cuda", generator=g)
+    tabs = [torch.from_numpy(np.ascontiguousarray(K[k], dtype=np.float32)).cuda() for k in ("wmr", "wmh", "iso")]
+    G = _lut_grad(A, x, gy, [t.shape for t in tabs])
+    Gabs = _lut_grad(A, x, gy.abs(), [t.shape for t in tabs])     # x >= 0: sum |terms| = Gabs . |delta|
+    for k in range(3):
+        delta = torch.randn(tabs[k].shape, dtype=torch.float32, device="cuda", generator=g)
+        parts = [torch.zeros_like(t) for t in tabs]
+        parts[k] = delta
+        _set_tables(A, *parts)
+        rhs = float(torch.dot(gy.double(), A.dot(x).double()).item())
+        lhs = float((G[k] * delta.double().cpu().numpy()).sum())
+        # FP32 products: the error is relative to sum |terms|
+        scale = float((Gabs[k] * np.abs(delta.double().cpu().numpy())).sum())
+        print(f"table {k}: sum G*delta {lhs:.9e}  <gy, A(delta) x> {rhs:.9e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-6 * scale, (k, lhs, rhs, scale)
+    _set_tables(A, *tabs)
+
+
+def test_x_gradient_is_the_adjoint_200k(brain200k):
+    """x.grad of <gy, A x> through autograd equals A^T gy: <g_x, v> = <A v, gy>."""
+    D, K, A = brain200k
+    g = torch.Generator(device="cuda").manual_seed(22)
+    x = torch.rand(A.shape[1], dtype=torch.float32, device="cuda", generator=g).requires_grad_(True)
+    gy = torch.randn(A.shape[0], dtype=torch.float32, device="cuda", generator=g)
+    loss = torch.dot(autograd.apply(A, x), gy)
+    loss.backward()
+    for s in range(3):
+        v = torch.rand(A.shape[1], dtype=torch.float32, device="cuda", generator=g)
+        a = float(torch.dot(x.grad.double(), v.double()).item())
+        b = float(torch.dot(A.dot(v).double(), gy.double()).item())
+        scale = float(torch.dot(A.T.dot(gy.abs()).double(), v.double()).item())
+        assert abs(a - b) <= 1e-6 * scale, (a, b, scale)
+
+
+def test_diffusivity_gradient_vs_finite_differences_200k(brain200k):
+    """d/d(d_par, d_perp) of 0.5 ||A(theta) x - y||^2 through the tables
+    (szb_tables -> commit_hip_lut_grad) against central differences with
+    h = 1e-4 relative, in FP64 vectors on the same plan.  The tables are
+    float32, which bounds the differences' accuracy near 6e-8 / 1e-4."""
+    D, K, A = brain200k
+    A.set_precision(_lib.FP64)
+    try:
+        xt = torch.from_numpy(synth.x_true(A.shape[1], 1004)).cuda()
+        tt = lambda v: torch.tensor(v, dtype=torch.float64, device="cuda")
+        with torch.no_grad():
+            y = autograd.apply(A, xt, *[t.float() for t in autograd.szb_tables(
+                tt(1.9e-3), tt(0.5e-3), K["bvals"], K["bvecs"], K["dirs"])]).clone()
+
+        def loss(dp, dq, grad=False):
+            p = tt(dp).requires_grad_(grad), tt(dq).requires_grad_(grad)
+            tabs = autograd.szb_tables(p[0], p[1], K["bvals"], K["bvecs"], K["dirs"])
+            r = autograd.apply(A, xt, *[t.float() for t in tabs]) - y
+            L = 0.5 * torch.dot(r, r)
+            if grad:
+                L.backward()
+                return float(L.item()), float(p[0].grad.item()), float(p[1].grad.item())
+            return float(L.item())
+
+        theta = (1.7e-3, 0.6e-3)
+        _, g0, g1 = loss(*theta, grad=True)
+        for k, gk in ((0, g0), (1, g1)):
+            h = 1e-4 * theta[k]
+            up, dn = list(theta), list(theta)
+            up[k] += h
+            dn[k] -= h
+            fd = (loss(*up) - loss(*dn)) / (2 * h)
+            print(f"theta {k}: autograd {gk:.9e}  central difference {fd:.9e}", flush=True)
+            assert abs(fd - gk) <= 5e-3 * abs(fd), (k, fd, gk)
+    finally:
+        A.set_precision(_lib.FP32)
+        _set_tables(A, *[torch.from_numpy(np.ascontiguousarray(K[k], dtype=np.float32)).cuda()
+                         for k in ("wmr", "wmh", "iso")])
+
+
+def test_fit_diffusivities_200k(brain200k):
+    """(d_par, d_perp) = (1.9e-3, 0.5e-3) recovered from (1.7e-3, 0.6e-3) to
+    1e-3 relative; afterwards the plan holds the fitted tables (A.dot equals a
+    fresh operator built on the updated KERNELS)."""
+    D, K, A = brain200k
+    xt = torch.from_numpy(synth.x_true(A.shape[1], 1004)).cuda().float()
+    tt = lambda v: torch.tensor(v, dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        y = autograd.apply(A, xt, *[t.float() for t in autograd.szb_tables(
+            tt(1.9e-3), tt(0.5e-3), K["bvals"], K["bvecs"], K["dirs"])]).clone()
+    hist = []
+    dp, dq = autograd.fit_diffusivities(A, xt, y, A.KERNELS, (1.7e-3, 0.6e-3), iters=40, history=hist)
+    print(f"200k fit: d_par {dp:.6e} d_perp {dq:.6e} in {len(hist)} evaluations", flush=True)
+    assert abs(dp - 1.9e-3) <= 1e-3 * 1.9e-3 and abs(dq - 0.5e-3) <= 1e-3 * 0.5e-3
+    B = LinearOperator(D, A.KERNELS, None, precision=_lib.FP32)
+    assert rel(A.dot(xt).double().cpu().numpy(), B.dot(xt).double().cpu().numpy()) <= 1e-6

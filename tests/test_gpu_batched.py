@@ -3,9 +3,11 @@ block-diagonal plan, each with a lambda sweep of group-lasso fits, advanced
 together by the device-side FISTA control (commit_amd.batched).
 
 Bars: every (tractogram, lambda) fit runs the reference's iteration count and
-its x agrees with the oracle's FP64 fit of that tractogram alone to 1e-4
+its x agrees with the oracle's FP64 fit of that tractogram alone to 1e-5
 relative (the batched products are float32 matrix-core products with FP32
 accumulation per voxel); the graph-replayed run equals the eager run bitwise.
+Both are checked on small tractograms and on the full batch of eight
+tractograms of 50k-400k fibres, nS = 288, 16 lambdas each.
 """
 import numpy as np
 import pytest
@@ -16,7 +18,7 @@ from commit_amd import batched, synth
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
-TOL_BATCHED = 1e-4
+TOL_BATCHED = 1e-5
 SIZES = (600, 1200, 2000)
 LAMS = (1e-4, 1e-3, 1e-2, 5e-2)
 
@@ -152,3 +154,94 @@ def test_fused_residual_equals_two_calls(stack, waves, R):
                 rows = C.rows(s)
                 ref = (r64[rows] ** 2).sum(axis=0)
                 assert np.allclose(a[s * R:(s + 1) * R], ref, rtol=1e-6)
+
+
+# ------------------------------------ 8 tractograms of 50k-400k, 16 lambdas --
+SIZES8 = tuple(50_000 * (k + 1) for k in range(8))
+
+
+@pytest.fixture(scope="module")
+def stack8():
+    """Eight tractograms of 50k..400k fibres on the 2.5 mm head (nS = 288),
+    stacked block-diagonally; y = A x_true + Rician noise (SNR 30) rounded to
+    float32, the input of both the batched fits and the oracle; 64 random
+    bundles per tractogram; 16 lambdas log-spaced 1e-4..1e-1 x lambda_max."""
+    from commit_amd import _lib
+    probs = [synth.make_problem("brain25", seed=2000 + k, nF=nf) for k, nf in enumerate(SIZES8)]
+    K = probs[0][1]
+    C = batched.Combined([p[0] for p in probs], K)
+    A = C.operator(precision=_lib.FP64)
+    xt = torch.from_numpy(synth.x_true(C.n, 3001)).cuda()
+    sgn = A.matvec(xt).cpu().numpy()
+    y = synth.rician(sgn, 30.0, 3002).astype(np.float32).astype(np.float64)
+    aty = A.T.dot(y)
+    rng = np.random.default_rng(3000)
+    groups, gw, lam_max, per = [], [], np.zeros(C.S), []
+    for s_ in range(C.S):
+        lab = rng.integers(0, 64, C.nF[s_])
+        mine = []
+        for b in range(64):
+            gi = np.nonzero(lab == b)[0].astype(np.int32)
+            if gi.size:
+                mine.append(gi)
+                groups.append(gi + C.fo[s_])
+                gw.append(np.sqrt(gi.size))
+                lam_max[s_] = max(lam_max[s_], np.linalg.norm(aty[gi + C.fo[s_]]) / gw[-1])
+        per.append(mine)
+    gidx = np.empty(len(groups), dtype=object)
+    for k, gi in enumerate(groups):
+        gidx[k] = gi
+    lam = lam_max[:, None] * np.logspace(-4, -1, 16)[None, :]
+    print(f"stack8: {C.F} fibres, {C.D['IC']['n']} segments, {C.m} rows", flush=True)
+    return probs, C, y, gidx, np.asarray(gw), lam, per
+
+
+def _batched8(stack8, iters, use_graph):
+    probs, C, y, gidx, gw, lam, per = stack8
+    A = C.operator()
+    rp = _rp(C.nIC, C.F, C.nEC, C.E, C.nISO, C.I, gidx, gw)
+    bf = batched.BatchedFista(A, y, rp, lam, 16, layout=C, tol_fun=0.0, tol_x=0.0, max_iter=iters)
+    bf.initialise()
+    steps = bf.run(use_graph=use_graph)
+    return bf, steps
+
+
+def test_batched_8x16_graph_replay_equals_eager(stack8):
+    a, sa = _batched8(stack8, 6, use_graph=True)
+    b, sb = _batched8(stack8, 6, use_graph=False)
+    assert sa == sb and torch.equal(a.x, b.x)
+    da, db = a.details(), b.details()
+    assert all(da[s][r] == db[s][r] for s in range(len(da)) for r in range(16))
+
+
+ITERS8 = 10
+
+
+@pytest.fixture(scope="module")
+def fit8(stack8):
+    bf, steps = _batched8(stack8, ITERS8, use_graph=True)
+    return bf.solution(), bf.details()
+
+
+@pytest.mark.parametrize("s, r", [(0, 8), (7, 8), (7, 15)])
+def test_batched_8x16_vs_oracle(stack8, fit8, s, r):
+    """One lambda of the 50k and the 400k tractogram against the oracle's FP64
+    fit of that tractogram alone, 10 iterations."""
+    probs, C, y, gidx, gw, lam, per = stack8
+    X, details = fit8
+    iters = ITERS8
+    D, K = probs[s][0], probs[s][1]
+    R = oracle.OracleOperator(D, K, oracle.set_threads(D, 16))
+    ga = np.empty(len(per[s]), dtype=object)
+    for k, g in enumerate(per[s]):
+        ga[k] = g
+    rp = _rp(K["wmr"].shape[0], D["IC"]["nSTR"], K["wmh"].shape[0], D["EC"]["n"], K["iso"].shape[0],
+             D["ISO"]["n"], ga, np.sqrt([g.size for g in per[s]]), float(lam[s, r]))
+    xo, do = oracle.solve(y[C.rows(s)], R, R.T, tol_fun=0.0, tol_x=0.0, max_iter=iters,
+                          regularisation=oracle.init_regularisation(rp))
+    det = details[s][r]
+    e = rel(X[C.columns(s), r], xo)
+    print(f"stack8 tractogram {s} ({D['IC']['nSTR']} fibres) lambda {r}: iterations {det['iterations']}/"
+          f"{do['iterations']}, rel x {e:.2e}", flush=True)
+    assert det["iterations"] == do["iterations"]
+    assert e <= TOL_BATCHED

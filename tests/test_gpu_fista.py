@@ -211,3 +211,23 @@ def test_residual_combine(problem, fp32):
     e = rel(r_np, rh.double().cpu().numpy())
     print(f"fp32={fp32}: combined vs direct residual {e:.2e}", flush=True)
     assert e <= (1e-6 if fp32 else 1e-12)
+
+
+def test_c1_nonnegative_fit_100_iterations():
+    """The CPU-sized workload (32x32x20 grid, 10,000 fibres, nS = 60, SZB):
+    the 100-iteration FP64 non-negative fit against the oracle's, with the
+    reference test's bar (tests/test_demo.py:120-141)."""
+    D, K, _ = synth.make_problem("small", seed=1001)
+    T = oracle.set_threads(D, 8)
+    A = LinearOperator(D, K, T)
+    R = oracle.OracleOperator(D, K, T)
+    y = synth.rician(R.dot(synth.x_true(A.shape[1], 2001)), 30.0, 3001)
+    rp = _rp(D, K)
+    xg, dg = solvers.solve(y, A, A.T, tol_fun=0.0, tol_x=0.0, max_iter=100, verbose=0,
+                           regularisation=solvers.init_regularisation(dict(rp)))
+    xo, do = oracle.solve(y, R, R.T, tol_fun=0.0, tol_x=0.0, max_iter=100,
+                          regularisation=oracle.init_regularisation(dict(rp)))
+    print(f"C1 fit: {D['IC']['nSTR']} fibres, {D['IC']['n']} segments, iterations {dg['iterations']}, "
+          f"rel x {rel(xg, xo):.2e}", flush=True)
+    assert dg["iterations"] == 100
+    _check(xg, dg, xo, do)

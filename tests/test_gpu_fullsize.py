@@ -115,3 +115,77 @@ def test_fullsize_multi_rhs_and_fused_residual(brain):
     d = ((rsq1 - rsq2).abs() / rsq2).max().item()
     print(f"1M fibres, fused residual: max rel. difference of the 16 norms {d:.1e}", flush=True)
     assert d <= 1e-13
+
+
+# --------------------------------------- fits at the headline size vs the oracle --
+def _nn_rp(D, K):
+    nF, nE, nI = D["IC"]["nSTR"], D["EC"]["n"], D["ISO"]["n"]
+    nIC, nEC, nISO = K["wmr"].shape[0], K["wmh"].shape[0], K["iso"].shape[0]
+    return {"startIC": 0, "sizeIC": nF * nIC, "startEC": nF * nIC, "sizeEC": nE * nEC,
+            "startISO": nF * nIC + nE * nEC, "sizeISO": nI * nISO, "regIC": None, "regEC": None, "regISO": None,
+            "nnIC": True, "nnEC": True, "nnISO": True}
+
+
+def _compare_fits(dg, do, xg, xo, tol_x):
+    e = rel(xg, xo)
+    print(f"1M-fibre fit: iterations {dg['iterations']}/{do['iterations']}, rel x {e:.2e}, "
+          f"cost {dg['cost_function']:.12e} / {do['cost_function']:.12e}", flush=True)
+    assert dg["iterations"] == do["iterations"]
+    assert dg["stopping_criterion"] == do["stopping_criterion"]
+    assert e <= tol_x
+    return e
+
+
+@pytest.fixture(scope="module")
+def brain_signal(brain):
+    A, O, x, y, Ax_o, Aty_o = brain
+    return synth.rician(Ax_o, 30.0, 7)
+
+
+def test_fullsize_fp64_nn_fit_vs_oracle(brain, brain_signal):
+    """The 1M-fibre FP64 non-negative fit, 10 iterations, against the oracle's
+    FISTA on the oracle operator (host cores): same iterations, x to 1e-9,
+    the costs to 1e-9 relative."""
+    from commit_amd import solvers
+    A, O, x, y, Ax_o, Aty_o = brain
+    autotune.set_option(A, _lib.OPT_SLAB, 2)
+    rp = _nn_rp(A.DICTIONARY, A.KERNELS)
+    xg, dg = solvers.solve(brain_signal, A, A.T, tol_fun=0.0, tol_x=0.0, max_iter=10, verbose=0,
+                           regularisation=solvers.init_regularisation(dict(rp)))
+    xo, do = oracle.solve(brain_signal, O, O.T, tol_fun=0.0, tol_x=0.0, max_iter=10,
+                          regularisation=oracle.init_regularisation(dict(rp)))
+    _compare_fits(dg, do, xg, xo, 1e-9)
+    for k in ("residual", "cost_function", "abs_cost"):
+        assert dg[k] == pytest.approx(do[k], rel=1e-9), k
+
+
+def test_fullsize_fp32_group_lasso_fit_vs_oracle(brain, brain_signal):
+    """The mixed-precision solve (float32 vectors, FP64 step constant, stops and
+    final residual) with group lasso over 1,000 random bundles and
+    non-negativity, 10 iterations, against the oracle's FP64 fit: x to 1e-5."""
+    from commit_amd import solvers
+    A, O, x, y, Ax_o, Aty_o = brain
+    autotune.set_option(A, _lib.OPT_SLAB, 2)
+    D, K = A.DICTIONARY, A.KERNELS
+    nF = D["IC"]["nSTR"]
+    rng = np.random.default_rng(13)
+    lab = rng.integers(0, 1000, nF)
+    order = np.argsort(lab, kind="stable")
+    groups = np.empty(1000, dtype=object)
+    for k, g in enumerate(np.split(order, np.cumsum(np.bincount(lab, minlength=1000))[:-1])):
+        groups[k] = g.astype(np.int32)
+    gw = np.sqrt([g.size for g in groups]).astype(np.float64)
+    aty = O.T.dot(brain_signal)
+    lam = 0.05 * oracle.lambda_max_group(aty, gw, groups)
+    rp = dict(_nn_rp(D, K), regIC="group_lasso", lambdaIC=lam,
+              dictIC_params={"group_idx_kept": groups, "group_weights": gw})
+    xo, do = oracle.solve(brain_signal, O, O.T, tol_fun=0.0, tol_x=0.0, max_iter=10,
+                          regularisation=oracle.init_regularisation(dict(rp)))
+    A.set_precision(_lib.FP32)
+    try:
+        xg, dg = solvers.solve(brain_signal, A, A.T, tol_fun=0.0, tol_x=0.0, max_iter=10, verbose=0,
+                               regularisation=solvers.init_regularisation(dict(rp), n=A.shape[1], device=A.device))
+    finally:
+        A.set_precision(_lib.FP64)
+    _compare_fits(dg, do, xg, xo, 1e-5)
+    assert dg["cost_function"] == pytest.approx(do["cost_function"], rel=1e-6)
