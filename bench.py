@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP64_LANE_OPS = 256 * 64 * 2.4e9        # FP64 VALU lane-operations per second (39.3e12)
 
 
 def algorithmic_bytes(D, K, bx=8, by=8):
@@ -42,12 +43,21 @@ def algorithmic_bytes(D, K, bx=8, by=8):
             + 4 * nS * (ndirs * (nIC + nEC) + nISO))
 
 
-def algorithmic_flops(D, K):
-    """Multiply-adds of one product (A x or A^T y), 2 flops each: nS per IC
-    segment and RF, per EC peak and RF, per ISO voxel and RF."""
+def distinct_pairs(D, K):
+    """Distinct (voxel, LUT row) pairs of one product: IC segments of one voxel
+    and orientation share their row product (per IC RF), plus every EC peak
+    and RF and every ISO voxel and RF."""
     nIC, nEC, nISO = K["wmr"].shape[0], K["wmh"].shape[0], K["iso"].shape[0]
-    nS = K["wmr"].shape[2]
-    return 2 * nS * (D["IC"]["n"] * nIC + D["EC"]["n"] * nEC + D["ISO"]["n"] * nISO)
+    ndirs = int(K["wmr"].shape[1])
+    seen = np.zeros(int(D["IC"]["nVOX"]) * ndirs, dtype=bool)
+    seen[D["IC"]["vox"].astype(np.int64) * ndirs + D["IC"]["dir"].astype(np.int64)] = True
+    return int(seen.sum()) * nIC + D["EC"]["n"] * nEC + D["ISO"]["n"] * nISO
+
+
+def algorithmic_flops(D, K, pairs):
+    """Multiply-adds of one product (A x or A^T y), 2 flops each: nS per
+    distinct (voxel, LUT row) pair."""
+    return 2 * int(K["wmr"].shape[2]) * pairs
 
 
 def main():
@@ -171,8 +181,13 @@ def main():
     dom_name = max(kernels, key=kernels.get)
     dom_ms = kernels[dom_name]
     dom_gbs = bytes_pass / (dom_ms * 1e-3) / 1e9
-    flops_pass = algorithmic_flops(D, K)
+    pairs = distinct_pairs(D, K)
+    flops_pass = algorithmic_flops(D, K, pairs)
     dom_tflops = flops_pass / (dom_ms * 1e-3) / 1e12
+    # the compute roof of the FP64 kernels: a v_cvt_f64_f32 and a v_fma_f64 per
+    # sample and distinct pair, at 256 CUs x 64 lanes x 2.4 GHz FP64 lane-ops/s
+    lane_ops = 2 * int(K["wmr"].shape[2]) * pairs
+    dom_lops = lane_ops / (dom_ms * 1e-3)
     print(f"[bench] A x {ms_A:.3f} ms; A^T y {ms_At:.3f} ms (pass 1 {ms_At1:.3f}, pass 2 {ms_At2:.3f}); "
           f"dominant {dom_name}: {dom_gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
     traffic = measured_traffic(dom_name)
@@ -237,7 +252,12 @@ def main():
             # the same launch against the FP64 arithmetic peak (vector and matrix FP64 peaks
             # coincide on MI355X): multiply-adds of one product, 2 flops each
             "fp64": {"achieved": round(dom_tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(dom_tflops / FP64_PEAK_TFLOPS, 4), "flops": int(flops_pass)},
+                     "frac": round(dom_tflops / FP64_PEAK_TFLOPS, 4), "flops": int(flops_pass),
+                     "distinct_pairs": pairs},
+            "compute": {"achieved": round(dom_lops / 1e12, 3), "peak": FP64_LANE_OPS / 1e12,
+                        "unit": "T FP64 lane-op/s", "frac": round(dom_lops / FP64_LANE_OPS, 4),
+                        "lane_ops": int(lane_ops),
+                        "model": "one v_cvt_f64_f32 and one v_fma_f64 per sample and distinct (voxel, LUT row)"},
         },
         "cpu_baseline": cpu,
     }
