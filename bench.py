@@ -76,6 +76,8 @@ def main():
                          "autograd: differentiable operator on 200k fibres; products: the single-vector "
                          "products alone with a fixed launch option (--slab), for counter and trace runs")
     ap.add_argument("--slab", type=int, default=2, help="products mode: slab option of the plan (0, 1, 2)")
+    ap.add_argument("--residual", default="combine", choices=("combine", "product"),
+                    help="batched-fit mode: momentum residual from r(x) and r(x_prev) inside A^T, or a second A X")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
@@ -387,7 +389,8 @@ def batched_fit_mode(args):
           "dictIC_params": {"group_idx_kept": gidx, "group_weights": np.asarray(gw)}}
     tuned = autotune.tune(A, nrhs=16, reps=2)
     y32 = y.float()
-    bf = batched.BatchedFista(A, y32, rp, lam, 16, layout=C, tol_fun=1e-5, tol_x=1e-6, max_iter=args.fit_iters)
+    bf = batched.BatchedFista(A, y32, rp, lam, 16, layout=C, tol_fun=1e-5, tol_x=1e-6, max_iter=args.fit_iters,
+                              combine=args.residual == "combine")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bf.initialise()
@@ -426,7 +429,8 @@ def batched_fit_mode(args):
             "config": {"workload": f"8 tractograms of {sizes[0]}..{sizes[-1]} fibres (2.5 mm head), "
                                    f"{C.F} fibres, {C.D['IC']['n']} segments, {C.V} voxels, 16 lambdas "
                                    "log-spaced 1e-4..1e-1 x lambda_max over 64 random bundles each",
-                       "fits": C.S * 16, "trial_steps": steps, "s_run": round(t_run, 3), "s_init": round(t_init, 3),
+                       "fits": C.S * 16, "residual": args.residual,
+                       "trial_steps": steps, "s_run": round(t_run, 3), "s_init": round(t_init, 3),
                        "ms_per_trial_step": round(1e3 * t_run / max(steps, 1), 3),
                        "iterations_min": min(iters), "iterations_max": max(iters),
                        "check_fit": {"tractogram": s0, "lambda_index": r0, "rel_err_x_vs_fp64_single": err,

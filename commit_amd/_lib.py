@@ -63,6 +63,9 @@ SIGNATURES = {
     "commit_hip_fista_multi_control": (_I, [_P, _P, ctypes.c_double, ctypes.c_double, _I, _P, _P, _P, _P]),
     "commit_hip_fista_multi_update": (_I, [_P, _P, _P, _P, _P, _P]),
     "commit_hip_fista_multi_qf": (_I, [_P, _P, _P, _P]),
+    "commit_hip_A_multi_trial": (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "commit_hip_At_multi_momentum": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "commit_hip_fista_multi_roles": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_plan_set_lut": (_I, [_P, _P, _P, _P, _P]),
     "commit_hip_lut_grad": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_last_error": (ctypes.c_char_p, []),

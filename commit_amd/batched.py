@@ -41,7 +41,7 @@ class MultiLayout(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("m", ctypes.c_int64), ("nrhs", ctypes.c_int), ("nprob", ctypes.c_int),
                 ("segx", ctypes.c_void_p), ("xpieces", ctypes.c_void_p), ("xpseg", ctypes.c_void_p),
                 ("nxpieces", ctypes.c_int64), ("ypieces", ctypes.c_void_p), ("ypseg", ctypes.c_void_p),
-                ("nypieces", ctypes.c_int64), ("gseg", ctypes.c_void_p)]
+                ("nypieces", ctypes.c_int64), ("gseg", ctypes.c_void_p), ("vseg", ctypes.c_void_p)]
 
 
 class Combined:
@@ -144,10 +144,14 @@ class BatchedFista:
             stacked x (group indices in stacked x; groups must not overlap)
     lambdas: per fit penalty, array [S, R] (lasso or group lasso of IC) or
             [S, R, 2] (sparse group lasso: l1, group); EC/ISO lambdas from rp
+    combine: form the residual at the momentum point from the trial point's
+            and the previous iterate's, r(xhat) = (1 + coef) r(x) - coef
+            r(prev_x), inside the A^T product (one product A X per trial step);
+            False recomputes A xhat as the reference does (solvers.py:362-368)
     """
 
     def __init__(self, A, y, rp, lambdas, R, layout=None, sqrt_W=None, tol_fun=1e-4, tol_x=1e-6, max_iter=100,
-                 chunk=1 << 15):
+                 chunk=1 << 15, combine=True):
         self.lib = _lib.load()
         self.A, self.At = A, A.T
         self.dev = A.device
@@ -177,14 +181,19 @@ class BatchedFista:
                     yr.append((int(rr[0]), int(rr[-1]) + 1, s))
         xp, xs = _pieces(xr, chunk)
         yp, ys = _pieces(yr, chunk * 8)
+        nS = int(A.nSAMPLES)
+        vseg = np.zeros(m // nS if nS else 0, np.int32)          # problem of each voxel
+        for b, e, s in yr:
+            vseg[b // nS:e // nS] = s
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-        self._keep = [t(segx), t(xp), t(xs), t(yp), t(ys)]
+        self._keep = [t(segx), t(xp), t(xs), t(yp), t(ys), t(vseg)]
         L = MultiLayout()
         L.n, L.m, L.nrhs, L.nprob = n, m, self.R, S
         L.segx, L.xpieces, L.xpseg = (self._keep[0].data_ptr(), self._keep[1].data_ptr(), self._keep[2].data_ptr())
         L.nxpieces = int(xp.shape[0])
         L.ypieces, L.ypseg = self._keep[3].data_ptr(), self._keep[4].data_ptr()
         L.nypieces = int(yp.shape[0])
+        L.vseg = self._keep[5].data_ptr()
         # penalty
         from .solvers import DeviceRegularisation
         reg = DeviceRegularisation(dict(rp), n, dev)
@@ -226,7 +235,13 @@ class BatchedFista:
         self.x = torch.zeros((n, self.R), **f32)
         self.prev = torch.zeros((n, self.R), **f32)
         self.grad = torch.zeros((n, self.R), **f32)
-        self.res = torch.zeros((m, self.R), **f32)
+        self.combine = bool(combine)
+        # residual rows: with combine, r(prev_x) in r[sel] and the trial point's in r[1 - sel]
+        self.r = [torch.zeros((m, self.R), **f32)]
+        if self.combine:
+            self.r.append(torch.zeros((m, self.R), **f32))
+        self.sel = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.res = self.r[0]
         nq = S * self.R
         self.sums = torch.zeros(nq * 8, dtype=torch.float64, device=dev)
         self.rsq_x = torch.zeros(nq, dtype=torch.float64, device=dev)
@@ -320,9 +335,10 @@ class BatchedFista:
         """FISTA initialisation (solvers.py:260-286) for every fit."""
         if x0 is not None:
             self.xhat.copy_(torch.as_tensor(x0, dtype=torch.float32).reshape(self.n, self.R))
-        # residual and gradient at x0
-        self._resid(self.xhat, self.y, self.ypc, self.res, self.rsq_h)
-        self.At.matmat(self.res, out=self.grad, sqrtw=self.sw)
+        # residual and gradient at x0 (r[0]: r(prev_x) of the first momentum step, whose coef is 0)
+        self.sel.zero_()
+        self._resid(self.xhat, self.y, self.ypc, self.r[0], self.rsq_h)
+        self.At.matmat(self.r[0], out=self.grad, sqrtw=self.sw)
         # in-place prox(xhat, 1) with step 1 (col.step = 1 from the constructor)
         zero = torch.zeros_like(self.grad)
         self._prox(self.xhat, zero, self.x)
@@ -356,7 +372,14 @@ class BatchedFista:
         """One trial step of every unfinished fit (the captured sequence)."""
         self._prox(self.xhat, self.grad, self.x)
         self._sums(self.x, self.xhat, self.grad, self.prev)
-        self._resid(self.x, self.y, self.ypc, None, self.rsq_x)
+        if self.combine:
+            self._check(self.lib.commit_hip_A_multi_trial(self.A._plan.handle, ctypes.byref(self.L), self._P(self.x),
+                                                          self._P(self.y), self.ypc, self._P(self.sw),
+                                                          self._P(self.r[0]), self._P(self.r[1]), self._P(self.sel),
+                                                          self._P(self.rsq_x), self._P(self.rwork), self._st()),
+                        "commit_hip_A_multi_trial")
+        else:
+            self._resid(self.x, self.y, self.ypc, None, self.rsq_x)
         self._check(self.lib.commit_hip_fista_multi_control(ctypes.byref(self.params), ctypes.byref(self.L),
                                                             self.tol_fun, self.tol_x, self.max_iter,
                                                             self._P(self.sums), self._P(self.rsq_x),
@@ -365,6 +388,21 @@ class BatchedFista:
         self._check(self.lib.commit_hip_fista_multi_update(ctypes.byref(self.L), self._P(self.x), self._P(self.xhat),
                                                            self._P(self.prev), self._P(self.col), self._st()),
                     "commit_hip_fista_multi_update")
+        if self.combine:
+            # gradient and ||r(xhat)||^2 of the fits that advanced, then r(prev_x) back into r[sel]
+            self._check(self.lib.commit_hip_At_multi_momentum(self.A._plan.handle, ctypes.byref(self.L),
+                                                              self._P(self.r[0]), self._P(self.r[1]),
+                                                              self._P(self.sel), self._P(self.sw), self._P(self.col),
+                                                              self._P(self.grad), self._P(self.rsq_h),
+                                                              self._P(self.rwork), self._st()),
+                        "commit_hip_At_multi_momentum")
+            self._check(self.lib.commit_hip_fista_multi_qf(ctypes.byref(self.L), self._P(self.rsq_h),
+                                                           self._P(self.col), self._st()), "commit_hip_fista_multi_qf")
+            self._check(self.lib.commit_hip_fista_multi_roles(self.A._plan.handle, ctypes.byref(self.L),
+                                                              self._P(self.r[0]), self._P(self.r[1]),
+                                                              self._P(self.sel), self._P(self.col), self._st()),
+                        "commit_hip_fista_multi_roles")
+            return
         self._resid(self.xhat, self.y, self.ypc, self.res, self.rsq_h)
         self._check(self.lib.commit_hip_fista_multi_qf(ctypes.byref(self.L), self._P(self.rsq_h), self._P(self.col),
                                                        self._st()), "commit_hip_fista_multi_qf")

@@ -233,6 +233,8 @@ struct ResArgs {
     const float *w;       // sqrtw [rows] or NULL
     float *res;           // [rows][R] or NULL
     double *part;
+    float *res1;          // with sel: the output is (*sel ? res : res1), the buffer not holding r(prev_x)
+    const int32_t *sel;
 };
 
 // Output stage of the fused residual for one lane (column rhs of voxel v,
@@ -241,11 +243,11 @@ struct ResArgs {
 // memory; 2: y = 0.  The loop issues no memory load for YM 0 and 2, so the
 // stores never wait for one another.
 template <int NH, int NT, int YM>
-__device__ __forceinline__ double res_out(const f4 (&acc)[NH], const ResArgs &ra, const float *sy, const float *sw,
-                                          size_t rb, int nS, int g, int soff, int R, int rhs)
+__device__ __forceinline__ double res_out(const f4 (&acc)[NH], const ResArgs &ra, float *res, const float *sy,
+                                          const float *sw, size_t rb, int nS, int g, int soff, int R, int rhs)
 {
     double ss = 0.0;
-    float *rp = ra.res ? ra.res + rb * R + rhs : nullptr;
+    float *rp = res ? res + rb * R + rhs : nullptr;
 #pragma unroll
     for (int m = 0; m < NH; m++) {
         float yv[4], wv[4];
@@ -326,12 +328,13 @@ __global__ __launch_bounds__(kBlock) void k_A_mm(DevDict d, const float *__restr
         double ss = 0.0;
         if (rhs < R) {
             const size_t rb = (size_t)v * d.nS;
+            float *res = ra.sel ? (*ra.sel ? ra.res : ra.res1) : ra.res;
             if (!ra.y)
-                ss = res_out<NH, NT, 2>(acc, ra, sy, sw, rb, d.nS, g, soff, R, rhs);
+                ss = res_out<NH, NT, 2>(acc, ra, res, sy, sw, rb, d.nS, g, soff, R, rhs);
             else if (ra.ystride)
-                ss = res_out<NH, NT, 1>(acc, ra, sy, sw, rb, d.nS, g, soff, R, rhs);
+                ss = res_out<NH, NT, 1>(acc, ra, res, sy, sw, rb, d.nS, g, soff, R, rhs);
             else
-                ss = res_out<NH, NT, 0>(acc, ra, sy, sw, rb, d.nS, g, soff, R, rhs);
+                ss = res_out<NH, NT, 0>(acc, ra, res, sy, sw, rb, d.nS, g, soff, R, rhs);
         }
         ss = swap32_add(ss, ss);           // lanes q, q+16, q+32, q+48 hold column q
         ss = swap16_add(ss, ss);
@@ -549,14 +552,30 @@ __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDi
     if (pe0 < vr.total) at_store(pend, auxp, pe0, vr.total, g, q, z, Xr, R, rhs_ok);
 }
 
-template <int NT>
+// The momentum point's residual of the batched fits, formed while A^T loads
+// it (MOM): r(xhat) = (1 + coef) r(x) - coef r(prev_x) for xhat = x + coef (x -
+// prev_x), since the residual is affine in x.  r(prev_x) is in r[*sel], the
+// trial point's r(x) in r[1 - *sel].  Only the columns of fits that advanced
+// are formed and written (col[problem * R + column].advance); their sums of
+// rhat^2 go to part[v * 16 + q] (FP64, from the unrounded combination).
+struct MomArgs {
+    const float *r0, *r1;
+    const int32_t *sel;
+    const int32_t *vseg;                  // [nVOX] problem of each voxel
+    const commit_hip_fista_col *col;
+    double *part;
+};
+
+template <int NT, bool MOM>
 __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__restrict__ Y,
                                                   const float *__restrict__ sqrtw, float *__restrict__ z,
                                                   float *__restrict__ X, int R, int rg,
                                                   const uint32_t *__restrict__ uptr,
-                                                  const uint32_t *__restrict__ uhead)
+                                                  const uint32_t *__restrict__ uhead, MomArgs ma)
 {
     __shared__ __align__(16) float tiles[kWavesPerBlock][256 * NT];   // 16 rows x 16NT floats per wave
+    constexpr int kW = (16 * NT + kWave - 1) / kWave * kWave;             // whole 64-lane copies of nS <= 16 NT
+    __shared__ float wst[MOM ? kWavesPerBlock : 1][MOM ? kW : 1];          // MOM: the voxel's sqrtw row
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int v = wave_index(d.xcd_remap);
@@ -564,19 +583,63 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
     float *T = tiles[w];
     const int g = lane >> 4, q = lane & 15;
     const int rhs = rg * kTileR + q;
-    const bool rhs_ok = rhs < R;
+    bool rhs_ok = rhs < R;
     // B operand: samples past nS and RHS columns past R read sample 0 / column
     // 0 of this voxel and are zeroed, so the loads are unconditional and in
     // flight together.
     float yb[4 * NT];
     const size_t row0 = (size_t)v * d.nS;
     const int rc = rhs_ok ? rhs : 0;
+    if constexpr (!MOM) {
 #pragma unroll
-    for (int t = 0; t < 4 * NT; t++) {
-        const int s = g * 4 * NT + t;
-        yb[t] = Y[(row0 + (s < d.nS ? s : 0)) * R + rc];
+        for (int t = 0; t < 4 * NT; t++) {
+            const int s = g * 4 * NT + t;
+            yb[t] = Y[(row0 + (s < d.nS ? s : 0)) * R + rc];
+        }
+    } else {
+        const int sl = *ma.sel;
+        const float *rx = sl ? ma.r0 : ma.r1, *rp = sl ? ma.r1 : ma.r0;
+        const commit_hip_fista_col &c = ma.col[(size_t)ma.vseg[v] * R + rc];
+        rhs_ok = rhs_ok && c.advance != 0;
+        const double a = 1.0 + c.coef, b = -c.coef;
+        // r(prev_x) of the voxel (nS * R <= 256 NT floats) goes to this
+        // wavefront's LDS tile, which is free until the first LUT tile, and
+        // its sqrtw row to a small LDS row; r(x) goes to registers.
+        const int cnt = d.nS * R, gs = g * 4 * NT;
+        const float *src = rp + row0 * R;
+        float *Wl = wst[w];
+        for (int j = 0; j * kWave < cnt; j++)
+            __builtin_amdgcn_global_load_lds(src + min(j * kWave + lane, cnt - 1), (lds_void *)(T + kWave * j), 4, 0, 0);
+        if (sqrtw)
+            for (int j = 0; j * kWave < d.nS; j++)
+                __builtin_amdgcn_global_load_lds(sqrtw + row0 + min(j * kWave + lane, d.nS - 1),
+                                                 (lds_void *)(Wl + kWave * j), 4, 0, 0);
+        // Samples gs + t >= nS (t >= lim, last lane group only) read clamped
+        // rows and stale LDS and are zeroed below.  The later stages index
+        // with t and lim only, so no per-sample index outlives the loads.
+        const int lim = d.nS - gs;
+        const float *px = rx + row0 * R + rc + (size_t)gs * R;
+#pragma unroll
+        for (int t = 0; t < 4 * NT; t++) yb[t] = px[(size_t)min(t, lim - 1) * R];
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");          // r(prev_x) and sqrtw are in LDS
+        __builtin_amdgcn_wave_barrier();
+        const float *Tq = T + gs * R + rc;                         // (gs + t) * R + rc < 256 NT
+        const float *Wg = Wl + gs;                                 // gs + t < 16 NT <= kW
+        double ss = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4 * NT; t++) {
+            const double u = a * (double)yb[t] + b * (double)Tq[t * R];   // as commit_hip_residual_combine
+            const bool in = t < lim;
+            ss = fma(in ? u : 0.0, u, ss);
+            yb[t] = in && rhs_ok ? (sqrtw ? (float)u * Wg[t] : (float)u) : 0.0f;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");        // T is read before the first tile lands
+        __builtin_amdgcn_wave_barrier();
+        ss = swap32_add(ss, ss);           // lanes q, q+16, q+32, q+48 hold column q
+        ss = swap16_add(ss, ss);
+        if (g == 0) ma.part[(size_t)v * kTileR + q] = rhs_ok ? ss : 0.0;
     }
-    if (sqrtw) {
+    if (!MOM && sqrtw) {
         float wv[4 * NT];
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++) {
@@ -586,9 +649,11 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++) yb[t] *= wv[t];
     }
+    if constexpr (!MOM) {
 #pragma unroll
-    for (int t = 0; t < 4 * NT; t++)
-        if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
+        for (int t = 0; t < 4 * NT; t++)
+            if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
+    }
     const VRows vr = vrows_at(d, v, uptr, uhead);
     mm_vox_At<NT>(yb, d, vr, lane, z, X + rhs, R, rhs_ok, T);
 }
@@ -596,10 +661,13 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
 // voxel order) of len * z[head], in FP64.  16 lanes per (fibre, RF): one
 // 64-byte row of z per entry; heads and lengths are read in fibre order.
+// With col (the momentum form), only the columns of fits that advanced are written.
 __global__ __launch_bounds__(kBlock) void k_At_fibre_mm(DevDict d, const float *__restrict__ z,
                                                         float *__restrict__ X, int R, int rg,
                                                         const uint32_t *__restrict__ hpos,
-                                                        const float *__restrict__ lenF)
+                                                        const float *__restrict__ lenF,
+                                                        const commit_hip_fista_col *__restrict__ col,
+                                                        const int32_t *__restrict__ segx)
 {
     const size_t total = (size_t)d.nF * d.nIC * kTileR;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -623,7 +691,8 @@ __global__ __launch_bounds__(kBlock) void k_At_fibre_mm(DevDict d, const float *
         }
         for (; k < ke; k++) acc += (double)lenF[k] * (double)zi[(size_t)hpos[k] * kTileR];
         const int rhs = rg * kTileR + q;
-        if (rhs < R) X[gcol * R + rhs] = d.eval[gcol] ? (float)acc : 0.0f;
+        if (rhs < R && (!col || col[(size_t)segx[gcol] * R + rhs].advance))
+            X[gcol * R + rhs] = d.eval[gcol] ? (float)acc : 0.0f;
     }
 }
 
@@ -783,6 +852,113 @@ int launch_A_mm(commit_hip_plan *plan, const float *x, float *y, int nrhs, const
     return 0;
 }
 
+int check_layout(const commit_hip_plan *plan, const commit_hip_multi_layout *L)
+{
+    const DevDict &d = plan->d;
+    if (L->m != (int64_t)d.nVOX * d.nS) return fail(COMMIT_HIP_EINVAL, "layout rows %lld != plan rows", (long long)L->m);
+    if (L->nprob < 1 || L->nypieces < 1 || L->nypieces > INT32_MAX || !L->ypieces || !L->ypseg)
+        return fail(COMMIT_HIP_EINVAL, "layout pieces missing");
+    return 0;
+}
+
+// The fused A X residual (res chosen by ra) and its per-fit norms.
+int residual_norms(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *X, ResArgs ra, double *rsq,
+                   double *work, hipStream_t st)
+{
+    const DevDict &d = plan->d;
+    const int R = L->nrhs, nrg = (R + kTileR - 1) / kTileR;
+    double *part = work, *pp = work + (size_t)d.nVOX * 2 * nrg * kTileR;
+    ra.part = part;
+    int rc;
+    if ((rc = launch_A_mm<true>(plan, X, nullptr, R, ra, st))) return rc;
+    hipLaunchKernelGGL(k_res_pieces, dim3((unsigned)L->nypieces), dim3(kBlock), 0, st, part, L->ypieces, d.nS, d.nVOX,
+                       plan->multi_a_waves, nrg, R, pp);
+    hipLaunchKernelGGL(k_res_problems, dim3((unsigned)((L->nprob * R + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(kBlock), 0, st, pp, L->ypseg, (int)L->nypieces, L->nprob, R, rsq);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// A^T over 16-RHS groups: pass 1 (k_At_mm) then the per-fibre sums.  MOM: the
+// momentum form (Y formed from ma, columns of fits that advanced only; nrhs <= 16).
+template <bool MOM>
+int launch_At_mm(commit_hip_plan *plan, const float *y, const float *w, float *x, int nrhs, const MomArgs &ma,
+                 const int32_t *segx, hipStream_t st)
+{
+    const DevDict &d = plan->d;
+    int rc;
+    if (!plan->zmulti && d.n > 0 && d.nIC > 0) {
+        void *zp = nullptr;
+        if ((rc = commit_internal::dev_alloc(plan, (size_t)d.n * d.nIC * kTileR * sizeof(float), &zp))) return rc;
+        plan->zmulti = static_cast<float *>(zp);
+    }
+    if (!plan->mm_uhead && d.n > 0 && d.nIC > 0 && (rc = build_mm_heads(plan))) return rc;
+    const int nrg = (nrhs + kTileR - 1) / kTileR;
+    const unsigned grid = (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock);
+    const size_t total = (size_t)d.nF * d.nIC * kTileR;
+    const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
+    for (int rg = 0; rg < nrg; rg++) {
+        if (grid > 0) {
+            switch (nt_of(d)) {
+#define CASE(NT)                                                                                            \
+    case NT:                                                                                                \
+        hipLaunchKernelGGL((k_At_mm<NT, MOM>), dim3(grid), dim3(kBlock), 0, st, d, y, w, plan->zmulti, x, nrhs, \
+                           rg, plan->mm_uptr, plan->mm_uhead, ma);                                          \
+        break;
+                CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+                CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
+#undef CASE
+                default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        if (total > 0) {
+            hipLaunchKernelGGL(k_At_fibre_mm, dim3(g2), dim3(kBlock), 0, st, d, plan->zmulti, x, nrhs, rg, plan->mm_hpos,
+                               plan->mm_lenF, MOM ? ma.col : nullptr, MOM ? segx : nullptr);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+// Roles of the two residual buffers after a momentum step: r(prev_x) of every
+// unfinished fit must be in r[*sel].  Fits that advanced have it in r[1 - *sel]
+// (the trial point became prev_x), fits that backtracked in r[*sel].  When
+// both kinds exist the advanced columns are copied into r[*sel]; when no
+// unfinished fit backtracked the roles swap (k_roles_flip).  Finished fits are
+// ignored.
+__global__ __launch_bounds__(kBlock) void k_roles_copy(int64_t m, int R, int nS, const int32_t *__restrict__ vseg,
+                                                       const int32_t *__restrict__ sel,
+                                                       const commit_hip_fista_col *__restrict__ col, int nq,
+                                                       float *__restrict__ r0, float *__restrict__ r1)
+{
+    int adv = 0, back = 0;
+    for (int q = threadIdx.x; q < nq; q += kBlock) {
+        adv |= col[q].advance != 0;
+        back |= !col[q].done && !col[q].advance;
+    }
+    adv = __syncthreads_or(adv);
+    back = __syncthreads_or(back);
+    if (!(adv && back)) return;
+    const int sl = *sel;
+    const float *src = sl ? r0 : r1;
+    float *dst = sl ? r1 : r0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += (int64_t)gridDim.x * kBlock) {
+        const commit_hip_fista_col *c = col + (size_t)vseg[i / nS] * R;
+        for (int r = 0; r < R; r++)
+            if (c[r].advance) dst[i * R + r] = src[i * R + r];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_roles_flip(const commit_hip_fista_col *__restrict__ col, int nq,
+                                                       int32_t *__restrict__ sel)
+{
+    int back = 0;
+    for (int q = threadIdx.x; q < nq; q += kBlock) back |= !col[q].done && !col[q].advance;
+    back = __syncthreads_or(back);
+    if (!back && threadIdx.x == 0) *sel = 1 - *sel;
+}
+
 }  // namespace
 
 extern "C" {
@@ -812,22 +988,10 @@ int commit_hip_A_multi_residual(commit_hip_plan *plan, const commit_hip_multi_la
     int rc = check_multi(plan, L->nrhs, COMMIT_HIP_FP32);
     if (rc) return rc;
     if (L->nrhs > kTileR) return fail(COMMIT_HIP_EUNSUPPORTED, "batched fits take 1..%d columns", kTileR);
-    const DevDict &d = plan->d;
-    if (L->m != (int64_t)d.nVOX * d.nS) return fail(COMMIT_HIP_EINVAL, "layout rows %lld != plan rows", (long long)L->m);
-    if (L->nprob < 1 || L->nypieces < 1 || L->nypieces > INT32_MAX || !L->ypieces || !L->ypseg)
-        return fail(COMMIT_HIP_EINVAL, "layout pieces missing");
+    if ((rc = check_layout(plan, L))) return rc;
     HIP_TRY(hipSetDevice(plan->device));
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const int R = L->nrhs, nrg = (R + kTileR - 1) / kTileR;
-    double *part = work, *pp = work + (size_t)d.nVOX * 2 * nrg * kTileR;
-    const ResArgs ra{y, y_per_column ? 1 : 0, sqrtw, res, part};
-    if ((rc = launch_A_mm<true>(plan, X, nullptr, R, ra, st))) return rc;
-    hipLaunchKernelGGL(k_res_pieces, dim3((unsigned)L->nypieces), dim3(kBlock), 0, st, part, L->ypieces, d.nS, d.nVOX,
-                       plan->multi_a_waves, nrg, R, pp);
-    hipLaunchKernelGGL(k_res_problems, dim3((unsigned)((L->nprob * R + kWavesPerBlock - 1) / kWavesPerBlock)),
-                       dim3(kBlock), 0, st, pp, L->ypseg, (int)L->nypieces, L->nprob, R, rsq);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    ResArgs ra{y, y_per_column ? 1 : 0, sqrtw, res, nullptr, nullptr, nullptr};
+    return residual_norms(plan, L, X, ra, rsq, work, static_cast<hipStream_t>(stream));
 }
 
 int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw, void *X, int nrhs, int dtype,
@@ -837,42 +1001,66 @@ int commit_hip_At_multi(commit_hip_plan *plan, const void *Y, const void *sqrtw,
     int rc = check_multi(plan, nrhs, dtype);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(plan->device));
-    const DevDict &d = plan->d;
-    if (!plan->zmulti && d.n > 0 && d.nIC > 0) {
-        void *zp = nullptr;
-        if ((rc = commit_internal::dev_alloc(plan, (size_t)d.n * d.nIC * kTileR * sizeof(float), &zp))) return rc;
-        plan->zmulti = static_cast<float *>(zp);
-    }
-    if (!plan->mm_uhead && d.n > 0 && d.nIC > 0 && (rc = build_mm_heads(plan))) return rc;
+    return launch_At_mm<false>(plan, static_cast<const float *>(Y), static_cast<const float *>(sqrtw),
+                               static_cast<float *>(X), nrhs, MomArgs{}, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int commit_hip_A_multi_trial(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *X, const float *y,
+                             int y_per_column, const float *sqrtw, float *r0, float *r1, const int32_t *sel,
+                             double *rsq, double *work, void *stream)
+{
+    if (!plan || !L || !X || !r0 || !r1 || !sel || !rsq || !work) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    int rc = check_multi(plan, L->nrhs, COMMIT_HIP_FP32);
+    if (rc) return rc;
+    if (L->nrhs > kTileR) return fail(COMMIT_HIP_EUNSUPPORTED, "batched fits take 1..%d columns", kTileR);
+    if ((rc = check_layout(plan, L))) return rc;
+    HIP_TRY(hipSetDevice(plan->device));
+    ResArgs ra{y, y_per_column ? 1 : 0, sqrtw, r0, nullptr, r1, sel};
+    return residual_norms(plan, L, X, ra, rsq, work, static_cast<hipStream_t>(stream));
+}
+
+int commit_hip_At_multi_momentum(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *r0,
+                                 const float *r1, const int32_t *sel, const float *sqrtw,
+                                 const commit_hip_fista_col *col, float *grad, double *rsq, double *work,
+                                 void *stream)
+{
+    if (!plan || !L || !r0 || !r1 || !sel || !col || !grad || !rsq || !work)
+        return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    int rc = check_multi(plan, L->nrhs, COMMIT_HIP_FP32);
+    if (rc) return rc;
+    if (L->nrhs > kTileR) return fail(COMMIT_HIP_EUNSUPPORTED, "batched fits take 1..%d columns", kTileR);
+    if ((rc = check_layout(plan, L))) return rc;
+    if (!L->vseg || !L->segx) return fail(COMMIT_HIP_EINVAL, "layout: vseg and segx are needed");
+    HIP_TRY(hipSetDevice(plan->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const float *y = static_cast<const float *>(Y);
-    const float *w = static_cast<const float *>(sqrtw);
-    float *x = static_cast<float *>(X);
-    const int nrg = (nrhs + kTileR - 1) / kTileR;
-    const unsigned grid = (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock);
-    const size_t total = (size_t)d.nF * d.nIC * kTileR;
-    const unsigned g2 = (unsigned)std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
-    for (int rg = 0; rg < nrg; rg++) {
-        if (grid > 0) {
-            switch (nt_of(d)) {
-#define CASE(NT)                                                                                            \
-    case NT:                                                                                                \
-        hipLaunchKernelGGL(k_At_mm<NT>, dim3(grid), dim3(kBlock), 0, st, d, y, w, plan->zmulti, x, nrhs, rg,   \
-                           plan->mm_uptr, plan->mm_uhead);                                                \
-        break;
-                CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-                CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
-#undef CASE
-                default: return fail(COMMIT_HIP_EUNSUPPORTED, "nS=%d", d.nS);
-            }
-            HIP_TRY(hipGetLastError());
-        }
-        if (total > 0) {
-            hipLaunchKernelGGL(k_At_fibre_mm, dim3(g2), dim3(kBlock), 0, st, d, plan->zmulti, x, nrhs, rg, plan->mm_hpos,
-                               plan->mm_lenF);
-            HIP_TRY(hipGetLastError());
-        }
-    }
+    const DevDict &d = plan->d;
+    const int R = L->nrhs;
+    double *part = work, *pp = work + (size_t)d.nVOX * kTileR;
+    const MomArgs ma{r0, r1, sel, L->vseg, col, part};
+    if ((rc = launch_At_mm<true>(plan, nullptr, sqrtw, grad, R, ma, L->segx, st))) return rc;
+    hipLaunchKernelGGL(k_res_pieces, dim3((unsigned)L->nypieces), dim3(kBlock), 0, st, part, L->ypieces, d.nS, d.nVOX,
+                       1, 1, R, pp);
+    hipLaunchKernelGGL(k_res_problems, dim3((unsigned)((L->nprob * R + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(kBlock), 0, st, pp, L->ypseg, (int)L->nypieces, L->nprob, R, rsq);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int commit_hip_fista_multi_roles(const commit_hip_plan *plan, const commit_hip_multi_layout *L, float *r0, float *r1,
+                                 int32_t *sel, const commit_hip_fista_col *col, void *stream)
+{
+    if (!plan || !L || !r0 || !r1 || !sel || !col) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (L->nrhs < 1 || L->nrhs > kTileR || L->nprob < 1) return fail(COMMIT_HIP_EINVAL, "layout sizes");
+    if (!L->vseg) return fail(COMMIT_HIP_EINVAL, "layout: vseg is needed");
+    const DevDict &d = plan->d;
+    if (L->m != (int64_t)d.nVOX * d.nS) return fail(COMMIT_HIP_EINVAL, "layout rows %lld != plan rows", (long long)L->m);
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nq = L->nprob * L->nrhs;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((L->m + kBlock - 1) / kBlock, 8192));
+    hipLaunchKernelGGL(k_roles_copy, dim3(grid), dim3(kBlock), 0, st, L->m, L->nrhs, d.nS, L->vseg, sel, col, nq, r0, r1);
+    hipLaunchKernelGGL(k_roles_flip, dim3(1), dim3(kBlock), 0, st, col, nq, sel);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

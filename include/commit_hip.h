@@ -329,6 +329,7 @@ typedef struct {
     const int32_t *ypseg;
     int64_t nypieces;
     const int32_t *gseg;          /* [ngroups] problem of each group (group penalties), or NULL */
+    const int32_t *vseg;          /* [voxels] problem of each voxel (device): the momentum entry points, or NULL */
 } commit_hip_multi_layout;
 
 /* doubles of workspace commit_hip_fista_multi_sums needs; the residual needs nypieces * nrhs */
@@ -372,6 +373,36 @@ int commit_hip_fista_multi_update(const commit_hip_multi_layout *L, const float 
 /* accepted fits: qfval = ||res at the new xhat||^2 / 2 */
 int commit_hip_fista_multi_qf(const commit_hip_multi_layout *L, const double *rsq, commit_hip_fista_col *col,
                               void *stream);
+
+/* Momentum residual of the batched fits, without the product A xhat that the
+ * reference recomputes after every accepted step (solvers.py:362-368).  The
+ * residual is affine in x, so for xhat = x + coef (x - prev_x),
+ * r(xhat) = (1 + coef) r(x) - coef r(prev_x).  Two [rows][nrhs] float32
+ * buffers r0, r1 hold r(prev_x) of every unfinished fit (in r[*sel]) and the
+ * trial point's residual (in r[1 - *sel]); *sel is an int32 in device memory,
+ * so a captured trial step keeps working when the roles swap.  All three need
+ * L->vseg; work: commit_hip_A_multi_residual_work doubles.
+ *
+ * commit_hip_A_multi_trial: commit_hip_A_multi_residual with res = r[1 - *sel]. */
+int commit_hip_A_multi_trial(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *X, const float *y,
+                             int y_per_column, const float *sqrtw, float *r0, float *r1, const int32_t *sel,
+                             double *rsq, double *work, void *stream);
+
+/* After commit_hip_fista_multi_control and _update: for every fit with advance
+ * set, rhat = (1 + coef) r[1 - *sel] - coef r[*sel] (FP64, rounded to float as
+ * commit_hip_residual_combine), grad = A^T (sqrtw .* rhat) (the fit's columns
+ * of the [n][nrhs] matrix grad) and rsq = ||rhat||^2.  Columns of the other
+ * fits of grad keep their values; their rsq entries are unspecified. */
+int commit_hip_At_multi_momentum(commit_hip_plan *plan, const commit_hip_multi_layout *L, const float *r0,
+                                 const float *r1, const int32_t *sel, const float *sqrtw,
+                                 const commit_hip_fista_col *col, float *grad, double *rsq, double *work,
+                                 void *stream);
+
+/* After commit_hip_At_multi_momentum: moves r(prev_x) of every unfinished fit
+ * into r[*sel] (copies the advanced fits' columns when some unfinished fit
+ * backtracked, else swaps the roles by flipping *sel). */
+int commit_hip_fista_multi_roles(const commit_hip_plan *plan, const commit_hip_multi_layout *L, float *r0, float *r1,
+                                 int32_t *sel, const commit_hip_fista_col *col, void *stream);
 
 /* Thread-local message describing the last failure of this thread. */
 const char *commit_hip_last_error(void);

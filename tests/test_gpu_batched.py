@@ -156,6 +156,188 @@ def test_fused_residual_equals_two_calls(stack, waves, R):
                 assert np.allclose(a[s * R:(s + 1) * R], ref, rtol=1e-6)
 
 
+def _set_cols(bf, adv, done, coef):
+    cols = bf._download_cols()
+    for q in range(bf.S * bf.R):
+        cols[q].advance, cols[q].done, cols[q].coef = int(adv[q]), int(done[q]), float(coef[q])
+    bf._cols_host = cols
+    bf._upload_cols()
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("sel", [0, 1])
+def test_momentum_At_vs_At_of_combined_residual(stack, weighted, sel):
+    """commit_hip_At_multi_momentum against the plain 16-RHS A^T of the
+    residual combined on the host: rhat = (1 + coef) r[1 - sel] - coef r[sel]
+    (FP64, rounded to float32) in the columns of fits that advanced.  Those
+    gradient columns agree to FP32 rounding (1e-6 of the column norm) and their
+    ||rhat||^2 to 1e-12; the other fits' gradient columns keep their values."""
+    import ctypes
+    from commit_amd import _lib
+    probs, C, y, gidx, gw = stack
+    A = C.operator()
+    lib = _lib.load()
+    R = 16
+    rp = _rp(C.nIC, C.F, C.nEC, C.E, C.nISO, C.I, gidx, gw)
+    lam = np.tile(np.logspace(-4, -2, R), (C.S, 1))
+    rng = np.random.default_rng(40 + 2 * sel + weighted)
+    w = rng.uniform(0.5, 1.5, C.m) if weighted else None
+    bf = batched.BatchedFista(A, y, rp, lam, R, layout=C, sqrt_W=w, max_iter=2)
+    nq = C.S * R
+    adv = rng.random(nq) < 0.6
+    adv[0], adv[1] = True, False
+    coef = rng.uniform(0.0, 0.95, nq)
+    _set_cols(bf, adv, np.zeros(nq, bool), coef)
+    r = [rng.standard_normal((C.m, R)).astype(np.float32) for _ in range(2)]
+    for k in range(2):
+        bf.r[k].copy_(torch.from_numpy(r[k]))
+    bf.sel.fill_(sel)
+    g0 = torch.from_numpy(rng.standard_normal((C.n, R)).astype(np.float32)).cuda()
+    bf.grad.copy_(g0)
+    rsq = torch.zeros(nq, dtype=torch.float64, device="cuda")
+    P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.commit_hip_At_multi_momentum(A._plan.handle, ctypes.byref(bf.L), P(bf.r[0]), P(bf.r[1]),
+                                                P(bf.sel), P(bf.sw), P(bf.col), P(bf.grad), P(rsq), P(bf.rwork),
+                                                st), "At_multi_momentum")
+    got = bf.grad.cpu().numpy()
+    rx, rpv = r[1 - sel].astype(np.float64), r[sel].astype(np.float64)
+    yh = np.zeros((C.m, R), np.float32)
+    rsq_ref = np.zeros(nq)
+    for s_ in range(C.S):
+        rows = C.rows(s_)
+        for c in range(R):
+            q = s_ * R + c
+            if adv[q]:
+                u = (1.0 + coef[q]) * rx[rows, c] - coef[q] * rpv[rows, c]
+                yh[rows, c] = u.astype(np.float32)
+                rsq_ref[q] = np.sum(u * u)
+    ref = A.T.matmat(torch.from_numpy(yh).cuda(), sqrtw=bf.sw).cpu().numpy()
+    g0n = g0.cpu().numpy()
+    rs = rsq.cpu().numpy()
+    for s_ in range(C.S):
+        cols = C.columns(s_)
+        for c in range(R):
+            q = s_ * R + c
+            if adv[q]:
+                e = np.linalg.norm(got[cols, c] - ref[cols, c]) / np.linalg.norm(ref[cols, c])
+                assert e <= 1e-6, (s_, c, e)
+                assert abs(rs[q] - rsq_ref[q]) <= 1e-12 * rsq_ref[q], (s_, c, rs[q], rsq_ref[q])
+            else:
+                assert np.array_equal(got[cols, c], g0n[cols, c]), (s_, c)
+
+
+@pytest.mark.parametrize("case", ["all", "mixed", "backtrack", "finished"])
+def test_residual_roles(stack, case):
+    """commit_hip_fista_multi_roles: afterwards r(prev_x) of every unfinished
+    fit is in r[sel].  all: every unfinished fit advanced, the roles swap;
+    mixed: the advanced fits' columns are copied into r[sel]; backtrack: no fit
+    advanced, nothing moves; finished: only finished fits, no buffer changes."""
+    import ctypes
+    from commit_amd import _lib
+    probs, C, y, gidx, gw = stack
+    A = C.operator()
+    lib = _lib.load()
+    R = 4
+    rp = _rp(C.nIC, C.F, C.nEC, C.E, C.nISO, C.I, gidx, gw)
+    lam = np.tile(np.asarray(LAMS), (C.S, 1))
+    bf = batched.BatchedFista(A, y, rp, lam, R, layout=C, max_iter=2)
+    nq = C.S * R
+    rng = np.random.default_rng(77)
+    done = np.zeros(nq, bool)
+    if case == "all":
+        adv = np.ones(nq, bool)
+        adv[3], done[3] = False, True
+    elif case == "mixed":
+        adv = rng.random(nq) < 0.5
+        adv[0], adv[1] = True, False
+        done[2], adv[2] = True, False
+    elif case == "backtrack":
+        adv = np.zeros(nq, bool)
+    else:
+        adv = np.zeros(nq, bool)
+        done[:] = True
+    _set_cols(bf, adv, done, np.zeros(nq))
+    r = [rng.standard_normal((C.m, R)).astype(np.float32) for _ in range(2)]
+    for k in range(2):
+        bf.r[k].copy_(torch.from_numpy(r[k]))
+    sel0 = 1
+    bf.sel.fill_(sel0)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.commit_hip_fista_multi_roles(A._plan.handle, ctypes.byref(bf.L), P(bf.r[0]), P(bf.r[1]),
+                                                P(bf.sel), P(bf.col), st), "fista_multi_roles")
+    sel1 = int(bf.sel.item())
+    out = [bf.r[k].cpu().numpy() for k in range(2)]
+    if case in ("all", "finished"):
+        assert sel1 == 1 - sel0
+        assert np.array_equal(out[0], r[0]) and np.array_equal(out[1], r[1])
+        return
+    assert sel1 == sel0
+    assert np.array_equal(out[1 - sel0], r[1 - sel0])
+    exp = r[sel0].copy()
+    if case == "mixed":
+        for s_ in range(C.S):
+            rows = C.rows(s_)
+            for c in range(R):
+                if adv[s_ * R + c]:
+                    exp[rows, c] = r[1 - sel0][rows, c]
+    assert np.array_equal(out[sel0], exp)
+
+
+def test_combined_residual_equals_product_path(stack):
+    """The fits with the momentum residual formed from r(x) and r(prev_x)
+    against the fits that recompute A xhat: equal iteration counts and stopping
+    tests, x to 1e-5, with early stops so that fits finish at different steps."""
+    probs, C, y, gidx, gw = stack
+    A = C.operator()
+    rp = _rp(C.nIC, C.F, C.nEC, C.E, C.nISO, C.I, gidx, gw)
+    lam = np.tile(np.asarray(LAMS), (C.S, 1))
+    out = []
+    for combine in (True, False):
+        bf = batched.BatchedFista(A, y, rp, lam, len(LAMS), layout=C, tol_fun=1e-3, tol_x=1e-4, max_iter=60,
+                                  combine=combine)
+        bf.initialise()
+        bf.run(use_graph=True)
+        out.append((bf.solution(), bf.details()))
+    (xa, da), (xb, db) = out
+    its = set()
+    for s_ in range(C.S):
+        cols = C.columns(s_)
+        for r in range(len(LAMS)):
+            assert da[s_][r]["iterations"] == db[s_][r]["iterations"], (s_, r)
+            assert da[s_][r]["stopping_criterion"] == db[s_][r]["stopping_criterion"], (s_, r)
+            assert rel(xa[cols, r], xb[cols, r]) <= 1e-5, (s_, r)
+            its.add(da[s_][r]["iterations"])
+    assert len(its) > 1, its                                   # fits stopped at different iterations
+
+
+def test_batched_early_stops_vs_oracle(stack):
+    """Fits with the reference's tolerances stop at different iterations; each
+    stops at the oracle's iteration with the oracle's criterion, x to 1e-5."""
+    probs, C, y, gidx, gw = stack
+    A = C.operator()
+    rp = _rp(C.nIC, C.F, C.nEC, C.E, C.nISO, C.I, gidx, gw)
+    lam = np.tile(np.asarray(LAMS), (C.S, 1))
+    bf = batched.BatchedFista(A, y, rp, lam, len(LAMS), layout=C, tol_fun=1e-3, tol_x=1e-4, max_iter=60)
+    bf.initialise()
+    bf.run(use_graph=True)
+    X, det = bf.solution(), bf.details()
+    for s_, (D, K, R, ys, groups) in enumerate(probs):
+        ga = np.empty(len(groups), dtype=object)
+        for i, g in enumerate(groups):
+            ga[i] = g
+        nF, nE, nI = D["IC"]["nSTR"], D["EC"]["n"], D["ISO"]["n"]
+        for r, lamv in enumerate(LAMS):
+            rpo = _rp(K["wmr"].shape[0], nF, K["wmh"].shape[0], nE, K["iso"].shape[0], nI, ga,
+                      np.sqrt([g.size for g in groups]), lamv)
+            xo, do = oracle.solve(ys, R, R.T, tol_fun=1e-3, tol_x=1e-4, max_iter=60,
+                                  regularisation=oracle.init_regularisation(rpo))
+            assert det[s_][r]["iterations"] == do["iterations"], (s_, r, det[s_][r], do)
+            assert det[s_][r]["stopping_criterion"] == do["stopping_criterion"], (s_, r)
+            assert rel(X[C.columns(s_), r], xo) <= TOL_BATCHED, (s_, r)
+
+
 # ------------------------------------ 8 tractograms of 50k-400k, 16 lambdas --
 SIZES8 = tuple(50_000 * (k + 1) for k in range(8))
 
