@@ -68,6 +68,13 @@ SIGNATURES = {
     "commit_hip_fista_multi_roles": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_plan_set_lut": (_I, [_P, _P, _P, _P, _P]),
     "commit_hip_lut_grad": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "commit_hip_vox_create": (_I, [ctypes.POINTER(_P), _I]),
+    "commit_hip_vox_destroy": (_I, [_P]),
+    "commit_hip_trk2dictionary": (_I, [_P, _P, _P, _I, ctypes.c_int64, _I, _I, _I, _I] + [ctypes.c_float] * 9 +
+                                  [_P, _I, ctypes.c_float, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P,
+                                   _P]),
+    "commit_hip_vox_sizes": (_I, [_P, _I64P, _I64P, _I64P, _I64P, _I64P]),
+    "commit_hip_vox_fetch": (_I, [_P] * 13),
     "commit_hip_last_error": (ctypes.c_char_p, []),
     "commit_hip_version": (ctypes.c_char_p, []),
 }

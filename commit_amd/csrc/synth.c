@@ -378,3 +378,73 @@ int synth_stable_argsort_u32(const uint32_t *keys, size_t n, uint32_t nkeys, uin
     free(pos);
     return 0;
 }
+
+/* ------------------------------------------------------ streamlines ---- */
+/* The polylines themselves, for the tractogram voxeliser: streamline f is the
+ * B-spline of fibre f's first draw (no redraw: the voxeliser applies the mask
+ * and the length limits itself), sampled every step_vox voxels, with
+ * coordinates in mm in voxmm space (voxel i spans [i, i+1) * pixdim).
+ * Pass 1 (pts == NULL) fills counts[nF]; pass 2 fills pts at the offsets
+ * ptr[nF+1] (float32 x, y, z). */
+typedef struct {
+    const synth_params *p;
+    int f0, f1;
+    int64_t *counts;
+    const int64_t *ptr;
+    float *pts;
+} stream_job;
+
+static void *stream_worker(void *arg)
+{
+    stream_job *j = (stream_job *)arg;
+    const synth_params *p = j->p;
+    for (int f = j->f0; f < j->f1; f++) {
+        uint64_t st = p->seed * 0x9E3779B97F4A7C15ull ^ ((uint64_t)f << 20);
+        splitmix64(&st);
+        uint64_t rs = st;
+        double P[5][3];
+        control_points(p, &rs, P);
+        double poly = 0;
+        for (int k = 0; k < 4; k++) {
+            double s = 0;
+            for (int c = 0; c < 3; c++) s += (P[k + 1][c] - P[k][c]) * (P[k + 1][c] - P[k][c]);
+            poly += sqrt(s);
+        }
+        int M = (int)ceil(poly / p->step_vox) + 1;
+        if (M < 2) M = 2;
+        if (M > 9999) M = 9999;
+        if (!j->pts) {
+            j->counts[f] = M;
+            continue;
+        }
+        float *o = j->pts + 3 * j->ptr[f];
+        for (int m = 0; m < M; m++) {
+            double b[3];
+            bspline(P, (double)m / (M - 1), b);
+            for (int c = 0; c < 3; c++) o[3 * m + c] = (float)(b[c] * p->pixdim);
+        }
+    }
+    return NULL;
+}
+
+int synth_streamlines(const synth_params *p, int64_t *counts, const int64_t *ptr, float *pts)
+{
+    int nt = p->nthreads < 1 ? 1 : (p->nthreads > 256 ? 256 : p->nthreads);
+    if (nt > p->nF) nt = p->nF > 0 ? p->nF : 1;
+    stream_job *jobs = (stream_job *)calloc(nt, sizeof(stream_job));
+    pthread_t *th = (pthread_t *)calloc(nt, sizeof(pthread_t));
+    if (!jobs || !th) { free(jobs); free(th); return -1; }
+    for (int t = 0; t < nt; t++) {
+        jobs[t].p = p;
+        jobs[t].f0 = (int)((int64_t)p->nF * t / nt);
+        jobs[t].f1 = (int)((int64_t)p->nF * (t + 1) / nt);
+        jobs[t].counts = counts;
+        jobs[t].ptr = ptr;
+        jobs[t].pts = pts;
+        if (pthread_create(&th[t], NULL, stream_worker, &jobs[t]) != 0) stream_worker(&jobs[t]), th[t] = 0;
+    }
+    for (int t = 0; t < nt; t++) if (th[t]) pthread_join(th[t], NULL);
+    free(jobs);
+    free(th);
+    return 0;
+}

@@ -1,0 +1,953 @@
+// Tractogram voxeliser on the GPU: streamlines -> the IC/EC/ISO dictionary
+// that trk2dictionary() exports (trk2dictionary_c.cpp:101-869), for
+// streamlines already read into memory.
+//
+// Pipeline (one stream, all device-resident):
+//   k_vox_walk<false>   one thread per streamline runs fiberForwardModel
+//                       (:584-735) with its replicas, rayBoxIntersection
+//                       (:795-848) and segmentForwardModel (:741-789), and
+//                       counts the pieces it would record, with FiberLen and
+//                       FiberLenTot (float sums in recording order, :782-788)
+//   exclusive scan      piece offsets per streamline
+//   k_vox_walk<true>    the same walk again, storing each piece as
+//                       key = o * nvox + v (v = x + X(y + Y z), so ascending
+//                       keys are the std::map<segKey> order (o, z, y, x), :38-41)
+//                       and its double value w * len, in recording order
+//   segmented sort      stable radix sort of each streamline's pieces by key
+//   k_vox_merge         equal keys summed into a float in recording order
+//                       (the map's `+=`, :782), kept flag (:503-505)
+//   segmented sort      the merged segments of each streamline by voxel
+//                       (stable, so each voxel's segments stay in map order)
+//   k_vox_norm          TRK_norm: per-voxel float sums, float sum of their
+//                       squares in (z, y, x) order, float square root (:524-537)
+//   scans + k_vox_write kept streamlines renumbered (trk2dictionary.pyx:551-563)
+//                       and their segments written in streamline order
+//   radix sort + k_vox_tdi   compute_tdi (pyx:46-54): float sums per voxel in
+//                       segment order, and which voxels hold an IC segment
+//   k_vox_ec_*          ECSegments (:266-383), voxels in (z, y, x) order
+//   k_vox_iso_*         ISOcompartments (:386-436)
+//
+// The arithmetic is the reference's, operation by operation, in IEEE double
+// with its float roundings, and this file is compiled without contraction
+// (no fused multiply-adds), so every output is bit-for-bit the host
+// restatement's.  The sines and cosines of the blur quaternions depend only on
+// the replica and are taken on the host.  The one transcendental left on the
+// device, atan2, feeds round(degrees): a last-place difference from the host
+// libm changes an orientation only for an angle within 1e-13 degrees of a
+// half degree.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "commit_hip.h"
+#include "common.h"
+
+using commit_internal::fail;
+
+namespace {
+
+constexpr int kVoxBlock = 256;
+constexpr int kMaxFibLen = 10000;      // MAX_FIB_LEN, trk2dictionary_c.cpp:16
+
+struct VoxArgs {
+    int X, Y, Z;
+    float px, py, pz;
+    float sx, sy, sz;                  // shift in mm: fiber_shift * pixdim, float (:124-126)
+    float min_seg, min_fib, max_fib;
+    int intersect;
+    int n_rep;
+    uint64_t nvox;
+    const float *mask;                 // C order [X][Y][Z] or null
+    const short *hash;                 // 181 * 181
+    const double *rho, *sinh, *cosh, *wt;   // per replica
+    const uint8_t *apply;              // per streamline or null
+};
+
+struct D3 {
+    double x, y, z;
+};
+
+// Vector<double>::norm (Vector.h:47-50)
+__device__ __forceinline__ float vnorm(D3 a) { return (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// Vector<double>::Normalize (Vector.h:54-61)
+__device__ __forceinline__ D3 vnormalize(D3 a)
+{
+    float len = (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    if (len == 0) return a;
+    a.x = a.x / len;
+    a.y = a.y / len;
+    a.z = a.z / len;
+    return a;
+}
+
+__device__ __forceinline__ int hash_index(const short *hash, double colat, double lon)
+{
+    int ox = (int)round(colat / M_PI * 180.0);
+    int oy = (int)round(lon / M_PI * 180.0);
+    long long idx = (long long)ox * 181 + oy;
+    if (idx < 0) idx = 0;              // the reference reads before its table here
+    if (idx >= 181 * 181) idx = 181 * 181 - 1;
+    return hash[idx];
+}
+
+// cap of one sub-step's voxel-crossing loop (the reference loops unbounded)
+__device__ __forceinline__ long long max_cross(double len, const VoxArgs &a)
+{
+    double m = a.px < a.py ? a.px : a.py;
+    if (a.pz < m) m = a.pz;
+    double c = 3.0 * len / m;
+    if (!(c < 1e7)) c = 1e7;
+    return 64 + 4 * (long long)c;
+}
+
+// rayBoxIntersection (trk2dictionary_c.cpp:795-848)
+__device__ __forceinline__ bool raybox(D3 o, D3 d, D3 vmin, D3 vmax, double &t)
+{
+    double tmin, tmax, tymin, tymax, tzmin, tzmax;
+    D3 inv = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    if (inv.x >= 0) { tmin = (vmin.x - o.x) * inv.x; tmax = (vmax.x - o.x) * inv.x; }
+    else            { tmin = (vmax.x - o.x) * inv.x; tmax = (vmin.x - o.x) * inv.x; }
+    if (inv.y >= 0) { tymin = (vmin.y - o.y) * inv.y; tymax = (vmax.y - o.y) * inv.y; }
+    else            { tymin = (vmax.y - o.y) * inv.y; tymax = (vmin.y - o.y) * inv.y; }
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    if (inv.z >= 0) { tzmin = (vmin.z - o.z) * inv.z; tzmax = (vmax.z - o.z) * inv.z; }
+    else            { tzmin = (vmax.z - o.z) * inv.z; tzmax = (vmin.z - o.z) * inv.z; }
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    t = tmin;
+    if (t <= 0) t = tmax;
+    return true;
+}
+
+// One streamline's recorder: counts pieces (EMIT = false) or stores them.
+template <bool EMIT>
+struct Recorder {
+    const VoxArgs &a;
+    uint64_t *keys;
+    double *vals;
+    uint32_t limit;                    // the count pass's total: stores never pass it
+    uint32_t n = 0;
+    float flen = 0.0f, flentot = 0.0f;
+
+    __device__ Recorder(const VoxArgs &a_, uint64_t *k, double *v, uint32_t lim)
+        : a(a_), keys(k), vals(v), limit(lim) {}
+
+    // segmentForwardModel (trk2dictionary_c.cpp:741-789)
+    __device__ void segment(D3 P1, D3 P2, int k, double w)
+    {
+        D3 dir;
+        dir.y = P2.y - P1.y;
+        if (dir.y >= 0) {
+            dir.x = P2.x - P1.x;
+            dir.z = P2.z - P1.z;
+        } else {
+            dir.x = P1.x - P2.x;
+            dir.y = P1.y - P2.y;
+            dir.z = P1.z - P2.z;
+        }
+        double len = vnorm(dir);
+        if (w * len <= a.min_seg) return;
+        dir = vnormalize(dir);
+        int vx = (int)floor(0.5 * (P1.x + P2.x) / a.px);
+        int vy = (int)floor(0.5 * (P1.y + P2.y) / a.py);
+        int vz = (int)floor(0.5 * (P1.z + P2.z) / a.pz);
+        if (vx >= a.X || vx < 0 || vy >= a.Y || vy < 0 || vz >= a.Z || vz < 0) return;
+        if (a.mask && a.mask[vz + (size_t)a.Z * (vy + (size_t)a.Y * vx)] == 0) return;
+        if (EMIT && n < limit) {
+            double lon = atan2(dir.y, dir.x);
+            double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
+            unsigned short o = (unsigned short)hash_index(a.hash, colat, lon);
+            uint64_t v = (uint64_t)vx + (uint64_t)a.X * ((uint64_t)vy + (uint64_t)a.Y * vz);
+            keys[n] = (uint64_t)o * a.nvox + v;
+            vals[n] = w * len;
+        }
+        n++;
+        flentot = (float)((double)flentot + w * len);
+        if (k == 0) flen = (float)((double)flen + len);
+    }
+};
+
+// fiberForwardModel (trk2dictionary_c.cpp:584-735) for streamline f.  MULTI:
+// more than one replica, their positions kept in Pbuf [nF][n_rep].
+template <bool EMIT, bool MULTI>
+__global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *__restrict__ pts,
+                                                        const int64_t *__restrict__ fptr, int nF,
+                                                        D3 *__restrict__ Pbuf, const uint64_t *__restrict__ off,
+                                                        uint64_t *__restrict__ keys, double *__restrict__ vals,
+                                                        uint32_t *__restrict__ cnt, float *__restrict__ flen,
+                                                        float *__restrict__ flentot, uint32_t *__restrict__ status)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f >= nF) return;
+    Recorder<EMIT> rec(a, EMIT ? keys + off[f] : nullptr, EMIT ? vals + off[f] : nullptr,
+                       EMIT ? (uint32_t)(off[f + 1] - off[f]) : 0u);
+    const int64_t p0 = fptr[f];
+    const int pts_n = (int)(fptr[f + 1] - p0);
+    const float *fib = pts + 3 * p0;
+    const bool blur = a.apply ? (a.apply[f] != 0) : true;
+    D3 P0 = {0, 0, 0};
+    D3 *Pm = MULTI ? Pbuf + (size_t)f * a.n_rep : nullptr;
+    bool capped = false;
+
+    if (pts_n > 2) {
+        D3 S1 = {(double)(fib[0] + a.sx), (double)(fib[1] + a.sy), (double)(fib[2] + a.sz)};
+        D3 dir2 = {(double)(fib[3] + a.sx) - S1.x, (double)(fib[4] + a.sy) - S1.y, (double)(fib[5] + a.sz) - S1.z};
+        dir2 = vnormalize(dir2);
+        D3 n = {dir2.y - dir2.z, dir2.z - dir2.x, dir2.x - dir2.y};
+        n = vnormalize(n);
+        for (int k = 0; k < a.n_rep; k++) {
+            if (!blur && k > 0) continue;
+            double R = a.rho[k];
+            double w = a.sinh[k];
+            D3 q = {dir2.x * w, dir2.y * w, dir2.z * w};
+            w = a.cosh[k];
+            D3 qxn = {2.0 * (q.y * n.z - q.z * n.y), 2.0 * (q.z * n.x - q.x * n.z), 2.0 * (q.x * n.y - q.y * n.x)};
+            D3 qq = {q.y * qxn.z - q.z * qxn.y, q.z * qxn.x - q.x * qxn.z, q.x * qxn.y - q.y * qxn.x};
+            D3 nr = {n.x + w * qxn.x + qq.x, n.y + w * qxn.y + qq.y, n.z + w * qxn.z + qq.z};
+            nr = vnormalize(nr);
+            D3 s = {S1.x + R * nr.x, S1.y + R * nr.y, S1.z + R * nr.z};
+            if (MULTI) Pm[k] = s;
+            else P0 = s;
+        }
+        for (int i = 1; i < pts_n; i++) {
+            D3 S2 = {(double)(fib[3 * i] + a.sx), (double)(fib[3 * i + 1] + a.sy), (double)(fib[3 * i + 2] + a.sz)};
+            D3 dir1 = {S2.x - (double)(fib[3 * i - 3] + a.sx), S2.y - (double)(fib[3 * i - 2] + a.sy),
+                       S2.z - (double)(fib[3 * i - 1] + a.sz)};
+            dir1 = vnormalize(dir1);
+            if (i == pts_n - 1) {
+                dir2 = dir1;
+            } else {
+                dir2.x = (double)(fib[3 * i + 3] + a.sx) - S2.x;
+                dir2.y = (double)(fib[3 * i + 4] + a.sy) - S2.y;
+                dir2.z = (double)(fib[3 * i + 5] + a.sz) - S2.z;
+                dir2 = vnormalize(dir2);
+            }
+            D3 nn = {0.5 * (dir1.x + dir2.x), 0.5 * (dir1.y + dir2.y), 0.5 * (dir1.z + dir2.z)};
+            double dot = dir1.x * nn.x + dir1.y * nn.y + dir1.z * nn.z;
+            nn.x /= dot;
+            nn.y /= dot;
+            nn.z /= dot;
+            for (int k = 0; k < a.n_rep; k++) {
+                if (!blur && k > 0) continue;
+                if (a.wt[k] < 1e-3) continue;
+                const double wk = a.wt[k];
+                D3 Po = MULTI ? Pm[k] : P0;
+                double len = (S2.x - Po.x) * nn.x + (S2.y - Po.y) * nn.y + (S2.z - Po.z) * nn.z;
+                if (!(len > 0)) continue;
+                D3 Pn = {Po.x + dir1.x * len, Po.y + dir1.y * len, Po.z + dir1.z * len};
+                if (MULTI) Pm[k] = Pn;
+                else P0 = Pn;
+                if (!a.intersect) {
+                    rec.segment(Po, Pn, k, wk);
+                    continue;
+                }
+                D3 S1m = Po;
+                const D3 S2m = Pn;
+                long long cap = -1;
+                for (long long it = 0;; it++) {
+                    double dx = S2m.x - S1m.x, dy = S2m.y - S1m.y, dz = S2m.z - S1m.z;
+                    double sl = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (sl <= a.min_seg) break;
+                    if (cap < 0) cap = max_cross(sl, a);
+                    if (it >= cap) {
+                        capped = true;
+                        break;
+                    }
+                    if (floor(S1m.x / a.px) == floor(S2m.x / a.px) && floor(S1m.y / a.py) == floor(S2m.y / a.py) &&
+                        floor(S1m.z / a.pz) == floor(S2m.z / a.pz)) {
+                        rec.segment(S1m, S2m, k, wk);
+                        break;
+                    }
+                    D3 vmin, vmax;
+                    vmin.x = floor((S1m.x + 1e-6 * dir1.x) / a.px) * a.px;
+                    vmin.y = floor((S1m.y + 1e-6 * dir1.y) / a.py) * a.py;
+                    vmin.z = floor((S1m.z + 1e-6 * dir1.z) / a.pz) * a.pz;
+                    vmax.x = vmin.x + a.px;
+                    vmax.y = vmin.y + a.py;
+                    vmax.z = vmin.z + a.pz;
+                    double t;
+                    if (raybox(S1m, dir1, vmin, vmax, t) && t > 0 && t < sl) {
+                        D3 Pi = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
+                        rec.segment(S1m, Pi, k, wk);
+                        S1m = Pi;
+                    } else {
+                        rec.segment(S1m, S2m, k, wk);
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    if (!EMIT) {
+        cnt[f] = rec.n;
+        flen[f] = rec.flen;
+        flentot[f] = rec.flentot;
+    }
+    if (capped || (EMIT && rec.n != rec.limit)) *status = 1;
+}
+
+// Equal keys of a streamline's sorted pieces summed into a float in recording
+// order; m[f] merged segments stored from the streamline's first slot.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restrict__ off, int nF,
+                                                         const uint64_t *__restrict__ skey,
+                                                         const double *__restrict__ sval, uint64_t nvox,
+                                                         const float *__restrict__ flen, float min_fib, float max_fib,
+                                                         uint64_t *__restrict__ mkey, float *__restrict__ mlen,
+                                                         uint64_t *__restrict__ vkey, uint32_t *__restrict__ mcnt,
+                                                         uint64_t *__restrict__ mend, uint32_t *__restrict__ kept)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f >= nF) return;
+    const uint64_t b = off[f], e = off[f + 1];
+    uint64_t m = 0;
+    float acc = 0.0f;
+    uint64_t cur = 0;
+    for (uint64_t i = b; i < e; i++) {
+        const uint64_t k = skey[i];
+        if (i == b || k != cur) {
+            if (i != b) {
+                mkey[b + m] = cur;
+                mlen[b + m] = acc;
+                vkey[b + m] = cur % nvox;
+                m++;
+            }
+            cur = k;
+            acc = 0.0f;
+        }
+        acc = (float)((double)acc + sval[i]);
+    }
+    if (e > b) {
+        mkey[b + m] = cur;
+        mlen[b + m] = acc;
+        vkey[b + m] = cur % nvox;
+        m++;
+    }
+    const float fl = flen[f];
+    const uint32_t kf = (m > 0 && fl > min_fib && fl < max_fib) ? 1u : 0u;
+    mcnt[f] = kf ? (uint32_t)m : 0u;
+    mend[f] = kf ? b + m : b;
+    kept[f] = kf;
+}
+
+// TRK_norm of each kept streamline from its segments sorted by voxel.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_norm(const uint64_t *__restrict__ off,
+                                                        const uint64_t *__restrict__ mend, int nF,
+                                                        const uint64_t *__restrict__ vsort,
+                                                        const float *__restrict__ lsort, float *__restrict__ norm)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f >= nF) return;
+    const uint64_t b = off[f], e = mend[f];
+    float fn = 0.0f, s = 0.0f;
+    for (uint64_t i = b; i < e; i++) {
+        if (i != b && vsort[i] != vsort[i - 1]) {
+            fn = (float)((double)fn + (double)s * (double)s);
+            s = 0.0f;
+        }
+        s = s + lsort[i];
+    }
+    if (e > b) fn = (float)((double)fn + (double)s * (double)s);
+    norm[f] = (float)sqrt((double)fn);
+}
+
+// Kept streamlines' segments and per-streamline values, in streamline order.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_write(
+    const uint64_t *__restrict__ off, int nF, const uint32_t *__restrict__ kept, const uint64_t *__restrict__ kidx,
+    const uint64_t *__restrict__ spos, const uint32_t *__restrict__ mcnt, const uint64_t *__restrict__ mkey,
+    const float *__restrict__ mlen, uint64_t nvox, const float *__restrict__ norm, const float *__restrict__ flen,
+    const float *__restrict__ flentot, uint32_t *__restrict__ IC_f, uint32_t *__restrict__ IC_v,
+    uint16_t *__restrict__ IC_o, float *__restrict__ IC_len, float *__restrict__ TRK_norm,
+    float *__restrict__ TRK_len, float *__restrict__ TRK_lenTot, uint8_t *__restrict__ TRK_kept)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f >= nF) return;
+    const uint32_t kf = kept[f];
+    TRK_kept[f] = (uint8_t)kf;
+    if (!kf) return;
+    const uint64_t kk = kidx[f], s = spos[f], b = off[f];
+    const uint32_t m = mcnt[f];
+    for (uint32_t i = 0; i < m; i++) {
+        const uint64_t key = mkey[b + i];
+        IC_f[s + i] = (uint32_t)kk;
+        IC_v[s + i] = (uint32_t)(key % nvox);
+        IC_o[s + i] = (uint16_t)(key / nvox);
+        IC_len[s + i] = mlen[b + i];
+    }
+    TRK_norm[kk] = norm[f];
+    TRK_len[kk] = flen[f];
+    TRK_lenTot[kk] = flentot[f];
+}
+
+// compute_tdi (trk2dictionary.pyx:46-54) from the segments sorted by voxel
+// (stable, so each voxel's lengths are in segment order); marks the voxels
+// holding an IC segment.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_tdi(const uint32_t *__restrict__ vs,
+                                                       const float *__restrict__ ls, uint64_t n,
+                                                       float *__restrict__ tdi, uint8_t *__restrict__ has)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = vs[i];
+    if (i > 0 && vs[i - 1] == v) return;
+    float s = 0.0f;
+    for (uint64_t j = i; j < n && vs[j] == v; j++) s = s + ls[j];
+    tdi[v] = s;
+    has[v] = 1;
+}
+
+__global__ __launch_bounds__(kVoxBlock) void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kVoxBlock) void k_fill_f32(float *p, uint64_t n, float v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kVoxBlock) void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct EcArgs {
+    int X, Y, Z, Np;
+    float vf_thr;
+    int fx, fy, fz;
+    double A[9];
+    const float *peaks;               // C order [X][Y][Z][3 Np]
+    const short *hash;
+};
+
+// ECSegments (trk2dictionary_c.cpp:266-383) for voxel v (x fastest, so the
+// reference's z, y, x loops visit ascending v).  WRITE = false counts.
+template <bool WRITE>
+__global__ __launch_bounds__(kVoxBlock) void k_vox_ec(EcArgs a, uint64_t nvox, const uint8_t *__restrict__ has,
+                                                      uint32_t *__restrict__ cnt, const uint64_t *__restrict__ eoff,
+                                                      uint32_t *__restrict__ EC_v, uint16_t *__restrict__ EC_o)
+{
+    const uint64_t v = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    if (v >= nvox) return;
+    uint32_t c = 0;
+    if (has[v]) {
+        const uint64_t ix = v % a.X, iy = (v / a.X) % a.Y, iz = v / ((uint64_t)a.X * a.Y);
+        const float *base = a.peaks + 3 * (size_t)a.Np * (iz + (size_t)a.Z * (iy + (size_t)a.Y * ix));
+        float peakMax = -1;
+        for (int id = 0; id < a.Np; id++) {
+            D3 d = {base[3 * id], base[3 * id + 1], base[3 * id + 2]};
+            float nm = vnorm(d);
+            if (nm > peakMax) peakMax = nm;
+        }
+        if (peakMax > 0) {
+            const uint64_t o0 = WRITE ? eoff[v] : 0;
+            for (int id = 0; id < a.Np; id++) {
+                const float *q = base + 3 * id;
+                D3 d0 = {q[0], q[1], q[2]};
+                float nm = vnorm(d0);
+                if (nm == 0 || nm < a.vf_thr * peakMax) continue;
+                if (WRITE) {
+                    D3 d;
+                    d.x = q[0] * a.A[0] + q[1] * a.A[1] + q[2] * a.A[2];
+                    d.y = q[0] * a.A[3] + q[1] * a.A[4] + q[2] * a.A[5];
+                    d.z = q[0] * a.A[6] + q[1] * a.A[7] + q[2] * a.A[8];
+                    d.x *= a.fx;
+                    d.y *= a.fy;
+                    d.z *= a.fz;
+                    if (d.y < 0) {
+                        d.x = -d.x;
+                        d.y = -d.y;
+                        d.z = -d.z;
+                    }
+                    double colat = atan2(sqrt(d.x * d.x + d.y * d.y), d.z);
+                    double lon = atan2(d.y, d.x);
+                    EC_v[o0 + c] = (uint32_t)v;
+                    EC_o[o0 + c] = (uint16_t)hash_index(a.hash, colat, lon);
+                }
+                c++;
+            }
+        }
+    }
+    if (!WRITE) cnt[v] = c;
+}
+
+// ISOcompartments (trk2dictionary_c.cpp:386-436).  WRITE = false flags.
+template <bool WRITE>
+__global__ __launch_bounds__(kVoxBlock) void k_vox_iso(int X, int Y, int Z, uint64_t nvox,
+                                                       const float *__restrict__ iso, const float *__restrict__ mask,
+                                                       const uint8_t *__restrict__ has, uint32_t *__restrict__ flag,
+                                                       const uint64_t *__restrict__ ioff, uint32_t *__restrict__ ISO_v)
+{
+    const uint64_t v = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    if (v >= nvox) return;
+    if (WRITE) {
+        if (flag[v]) ISO_v[ioff[v]] = (uint32_t)v;
+        return;
+    }
+    const uint64_t ix = v % X, iy = (v / X) % Y, iz = v / ((uint64_t)X * Y);
+    const size_t c = iz + (size_t)Z * (iy + (size_t)Y * ix);
+    uint32_t ok = has[v] ? 1u : 0u;
+    if (iso && iso[c] == 0) ok = 0;
+    if (mask && mask[c] == 0) ok = 0;
+    flag[v] = ok;
+}
+
+inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kVoxBlock - 1) / kVoxBlock); }
+
+inline int bit_length(uint64_t x)
+{
+    int b = 0;
+    while (x) {
+        b++;
+        x >>= 1;
+    }
+    return b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- context --
+struct commit_hip_vox {
+    int device = 0;
+    struct Buf {
+        void *p = nullptr;
+        size_t bytes = 0;
+    };
+    enum {
+        PTS, FPTR, MASK, ISO, PEAKS, HASH, REP, APPLY, PBUF, CNT, FLEN, FLENTOT, OFF, K1, V1, K2, V2, L1,
+        MCNT, MEND, KEPT, KIDX, SPOS, NORM, STATUS, TMP,
+        IC_F, IC_V, IC_O, IC_LEN, TRK_NORM, TRK_LEN, TRK_LENTOT, TRK_KEPT, TDI_V, TDI_L, TDI, HAS,
+        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, NBUF
+    };
+    Buf buf[NBUF];
+    int64_t n_ic = 0, n_kept = 0, n_ec = 0, n_iso = 0, n_count = 0, n_pieces = 0;
+    uint64_t nvox = 0;
+    bool valid = false;
+
+    ~commit_hip_vox()
+    {
+        for (auto &b : buf)
+            if (b.p) (void)hipFree(b.p);
+    }
+    // grow buffer i to at least `bytes` (contents not kept)
+    int need(int i, size_t bytes)
+    {
+        if (bytes == 0) bytes = 256;
+        if (buf[i].bytes >= bytes) return 0;
+        if (buf[i].p) (void)hipFree(buf[i].p);
+        buf[i].p = nullptr;
+        buf[i].bytes = 0;
+        if (hipMalloc(&buf[i].p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed (voxeliser buffer %d)", bytes, i);
+        }
+        buf[i].bytes = bytes;
+        return 0;
+    }
+    template <class T>
+    T *at(int i) { return (T *)buf[i].p; }
+};
+
+namespace {
+
+// temporary storage of the rocprim calls, grown on demand
+int ensure_tmp(commit_hip_vox *V, size_t bytes) { return V->need(commit_hip_vox::TMP, bytes); }
+
+int exscan_u32(commit_hip_vox *V, const uint32_t *in, uint64_t *out, size_t n, hipStream_t s)
+{
+    size_t bytes = 0;
+    HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, (uint64_t)0, n, rocprim::plus<uint64_t>(), s));
+    int rc = ensure_tmp(V, bytes);
+    if (rc) return rc;
+    bytes = V->buf[commit_hip_vox::TMP].bytes;
+    HIP_TRY(rocprim::exclusive_scan(V->buf[commit_hip_vox::TMP].p, bytes, in, out, (uint64_t)0, n,
+                                    rocprim::plus<uint64_t>(), s));
+    return 0;
+}
+
+template <class K, class Vt>
+int segsort(commit_hip_vox *V, const K *ki, K *ko, const Vt *vi, Vt *vo, size_t n, int nseg, const uint64_t *b,
+            const uint64_t *e, int bits, hipStream_t s)
+{
+    if (n == 0 || nseg == 0) return 0;
+    size_t bytes = 0;
+    HIP_TRY(rocprim::segmented_radix_sort_pairs(nullptr, bytes, ki, ko, vi, vo, (unsigned)n, (unsigned)nseg, b, e,
+                                                0u, (unsigned)bits, s));
+    int rc = ensure_tmp(V, bytes);
+    if (rc) return rc;
+    bytes = V->buf[commit_hip_vox::TMP].bytes;
+    HIP_TRY(rocprim::segmented_radix_sort_pairs(V->buf[commit_hip_vox::TMP].p, bytes, ki, ko, vi, vo, (unsigned)n,
+                                                (unsigned)nseg, b, e, 0u, (unsigned)bits, s));
+    return 0;
+}
+
+int sort_pairs_u32(commit_hip_vox *V, const uint32_t *ki, uint32_t *ko, const float *vi, float *vo, size_t n,
+                   int bits, hipStream_t s)
+{
+    if (n == 0) return 0;
+    size_t bytes = 0;
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, ki, ko, vi, vo, (unsigned)n, 0u, (unsigned)bits, s));
+    int rc = ensure_tmp(V, bytes);
+    if (rc) return rc;
+    bytes = V->buf[commit_hip_vox::TMP].bytes;
+    HIP_TRY(rocprim::radix_sort_pairs(V->buf[commit_hip_vox::TMP].p, bytes, ki, ko, vi, vo, (unsigned)n, 0u,
+                                      (unsigned)bits, s));
+    return 0;
+}
+
+template <class T>
+int upload(commit_hip_vox *V, int i, const T *host, size_t count, hipStream_t s)
+{
+    int rc = V->need(i, count * sizeof(T));
+    if (rc) return rc;
+    if (count) HIP_TRY(hipMemcpyAsync(V->buf[i].p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int commit_hip_vox_create(commit_hip_vox **out, int device)
+{
+    if (!out) return fail(COMMIT_HIP_EINVAL, "commit_hip_vox_create: null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(COMMIT_HIP_EINVAL, "device %d out of range (%d devices)", device, ndev);
+    commit_hip_vox *V = new (std::nothrow) commit_hip_vox();
+    if (!V) return fail(COMMIT_HIP_ENOMEM, "host allocation failed");
+    V->device = device;
+    *out = V;
+    return 0;
+}
+
+extern "C" int commit_hip_vox_destroy(commit_hip_vox *V)
+{
+    if (!V) return 0;
+    (void)hipSetDevice(V->device);
+    delete V;
+    return 0;
+}
+
+extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers, const int64_t *fib_ptr,
+                                         int n_count, int64_t n_points, int inputs_on_device, int Nx, int Ny, int Nz,
+                                         float Px, float Py, float Pz, float fiber_shiftX, float fiber_shiftY,
+                                         float fiber_shiftZ, float min_seg_len, float min_fiber_len,
+                                         float max_fiber_len, const float *ptrPEAKS, int Np, float vf_THR, int ECix,
+                                         int ECiy, int ECiz, const float *ptrMASK, const float *ptrISO, int c,
+                                         const double *ptrPeaksAffine, int nReplicas, const double *ptrBlurRho,
+                                         const double *ptrBlurAngle, const double *ptrBlurWeights,
+                                         const uint8_t *ptrBlurApplyTo, const short *ptrHashTable, void *stream)
+{
+    using B = commit_hip_vox;
+    if (!V) return fail(COMMIT_HIP_EINVAL, "null voxeliser handle");
+    V->valid = false;
+    if (n_count < 0 || n_points < 0) return fail(COMMIT_HIP_EINVAL, "negative streamline or point count");
+    if (Nx <= 0 || Ny <= 0 || Nz <= 0 || Nx >= 65536 || Ny >= 65536 || Nz >= 65536)
+        return fail(COMMIT_HIP_EINVAL, "grid %d x %d x %d outside 1..65535 per axis", Nx, Ny, Nz);
+    if (!(Px > 0) || !(Py > 0) || !(Pz > 0)) return fail(COMMIT_HIP_EINVAL, "voxel sizes must be positive");
+    const uint64_t nvox = (uint64_t)Nx * Ny * Nz;
+    if (nvox > 0xFFFFFFFFull) return fail(COMMIT_HIP_EUNSUPPORTED, "grid of %llu voxels: IC_v is 32-bit", (unsigned long long)nvox);
+    if (!ptrHashTable) return fail(COMMIT_HIP_EINVAL, "null orientation hash table");
+    if (nReplicas < 1 || !ptrBlurRho || !ptrBlurAngle || !ptrBlurWeights)
+        return fail(COMMIT_HIP_EINVAL, "need at least one replica with its rho, angle and weight");
+    if (ptrPEAKS && (Np <= 0 || !ptrPeaksAffine)) return fail(COMMIT_HIP_EINVAL, "peaks need Np > 0 and an affine");
+    if (n_count > 0 && (!fibers || !fib_ptr)) return fail(COMMIT_HIP_EINVAL, "null streamline arrays");
+    HIP_TRY(hipSetDevice(V->device));
+    hipStream_t s = (hipStream_t)stream;
+
+    // streamline offsets: checked on the host (MAX_FIB_LEN, :171 and :856)
+    std::vector<int64_t> hptr((size_t)n_count + 1, 0);
+    if (n_count > 0) {
+        if (inputs_on_device) {
+            HIP_TRY(hipMemcpyAsync(hptr.data(), fib_ptr, ((size_t)n_count + 1) * sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        } else {
+            std::memcpy(hptr.data(), fib_ptr, ((size_t)n_count + 1) * sizeof(int64_t));
+        }
+    }
+    if (hptr[0] != 0 || hptr[(size_t)n_count] != n_points)
+        return fail(COMMIT_HIP_EINVAL, "fib_ptr must run from 0 to n_points (%lld)", (long long)n_points);
+    for (int f = 0; f < n_count; f++) {
+        const int64_t np = hptr[(size_t)f + 1] - hptr[(size_t)f];
+        if (np < 0) return fail(COMMIT_HIP_EINVAL, "fib_ptr decreases at streamline %d", f);
+        if (np >= kMaxFibLen)
+            return fail(COMMIT_HIP_EUNSUPPORTED, "streamline %d has %lld points (the reference's limit is %d)", f,
+                        (long long)np, kMaxFibLen - 1);
+    }
+    int maxo = 0;
+    for (int i = 0; i < 181 * 181; i++) {
+        if (ptrHashTable[i] < 0) return fail(COMMIT_HIP_EINVAL, "negative orientation in the hash table");
+        if (ptrHashTable[i] > maxo) maxo = ptrHashTable[i];
+    }
+
+    int rc;
+    const float *d_pts;
+    const int64_t *d_ptr;
+    if (inputs_on_device) {
+        d_pts = fibers;
+        d_ptr = fib_ptr;
+    } else {
+        if ((rc = upload(V, B::PTS, fibers, (size_t)n_points * 3, s))) return rc;
+        if ((rc = upload(V, B::FPTR, fib_ptr, (size_t)n_count + 1, s))) return rc;
+        d_pts = V->at<float>(B::PTS);
+        d_ptr = V->at<int64_t>(B::FPTR);
+    }
+    if ((rc = upload(V, B::HASH, ptrHashTable, 181 * 181, s))) return rc;
+    // replicas: rho, sin(angle/2), cos(angle/2), weight (:614-619)
+    std::vector<double> rep(4 * (size_t)nReplicas);
+    for (int k = 0; k < nReplicas; k++) {
+        rep[k] = ptrBlurRho[k];
+        rep[nReplicas + k] = std::sin(ptrBlurAngle[k] / 2.0);
+        rep[2 * nReplicas + k] = std::cos(ptrBlurAngle[k] / 2.0);
+        rep[3 * nReplicas + k] = ptrBlurWeights[k];
+    }
+    if ((rc = upload(V, B::REP, rep.data(), rep.size(), s))) return rc;
+    const float *d_mask = nullptr, *d_iso = nullptr;
+    if (ptrMASK) {
+        if ((rc = upload(V, B::MASK, ptrMASK, nvox, s))) return rc;
+        d_mask = V->at<float>(B::MASK);
+    }
+    if (ptrISO) {
+        if ((rc = upload(V, B::ISO, ptrISO, nvox, s))) return rc;
+        d_iso = V->at<float>(B::ISO);
+    }
+    const uint8_t *d_apply = nullptr;
+    if (ptrBlurApplyTo && nReplicas > 1) {
+        if ((rc = upload(V, B::APPLY, ptrBlurApplyTo, (size_t)n_count, s))) return rc;
+        d_apply = V->at<uint8_t>(B::APPLY);
+    }
+
+    VoxArgs a;
+    a.X = Nx; a.Y = Ny; a.Z = Nz;
+    a.px = Px; a.py = Py; a.pz = Pz;
+    a.sx = fiber_shiftX * Px; a.sy = fiber_shiftY * Py; a.sz = fiber_shiftZ * Pz;
+    a.min_seg = min_seg_len; a.min_fib = min_fiber_len; a.max_fib = max_fiber_len;
+    a.intersect = c > 0;
+    a.n_rep = nReplicas;
+    a.nvox = nvox;
+    a.mask = d_mask;
+    a.hash = V->at<short>(B::HASH);
+    const double *d_rep = V->at<double>(B::REP);
+    a.rho = d_rep;
+    a.sinh = d_rep + nReplicas;
+    a.cosh = d_rep + 2 * nReplicas;
+    a.wt = d_rep + 3 * nReplicas;
+    a.apply = d_apply;
+
+    const size_t nF = (size_t)n_count;
+    if ((rc = V->need(B::CNT, (nF + 1) * 4)) || (rc = V->need(B::FLEN, nF * 4)) ||
+        (rc = V->need(B::FLENTOT, nF * 4)) || (rc = V->need(B::OFF, (nF + 1) * 8)) ||
+        (rc = V->need(B::STATUS, 4)) || (rc = V->need(B::MCNT, (nF + 1) * 4)) ||
+        (rc = V->need(B::MEND, nF * 8)) || (rc = V->need(B::KEPT, (nF + 1) * 4)) ||
+        (rc = V->need(B::KIDX, (nF + 1) * 8)) || (rc = V->need(B::SPOS, (nF + 1) * 8)) ||
+        (rc = V->need(B::NORM, nF * 4)))
+        return rc;
+    const bool multi = nReplicas > 1;
+    if (multi && (rc = V->need(B::PBUF, nF * (size_t)nReplicas * sizeof(D3)))) return rc;
+    uint32_t *d_status = V->at<uint32_t>(B::STATUS);
+    uint32_t *d_cnt = V->at<uint32_t>(B::CNT);
+    uint64_t *d_off = V->at<uint64_t>(B::OFF);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_status, 1, 0u);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_cnt + nF, 1, 0u);
+    D3 *d_P = multi ? V->at<D3>(B::PBUF) : nullptr;
+    const unsigned gF = grid_for(nF ? nF : 1);
+    if (nF) {
+        if (multi)
+            k_vox_walk<false, true><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, nullptr, nullptr, nullptr,
+                                                             d_cnt, V->at<float>(B::FLEN), V->at<float>(B::FLENTOT),
+                                                             d_status);
+        else
+            k_vox_walk<false, false><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, nullptr, nullptr,
+                                                              nullptr, d_cnt, V->at<float>(B::FLEN),
+                                                              V->at<float>(B::FLENTOT), d_status);
+        HIP_TRY(hipGetLastError());
+    }
+    if ((rc = exscan_u32(V, d_cnt, d_off, nF + 1, s))) return rc;
+    uint64_t npieces = 0;
+    uint32_t hstatus = 0;
+    HIP_TRY(hipMemcpyAsync(&npieces, d_off + nF, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&hstatus, d_status, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (hstatus) return fail(COMMIT_HIP_EUNSUPPORTED, "a polyline step crossed more voxels than its cap allows");
+    if (npieces >= 0xFFFFFFFFull)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "%llu pieces exceed the 32-bit sort size", (unsigned long long)npieces);
+    V->n_pieces = (int64_t)npieces;
+    const size_t np = (size_t)npieces;
+    if ((rc = V->need(B::K1, np * 8)) || (rc = V->need(B::V1, np * 8)) || (rc = V->need(B::K2, np * 8)) ||
+        (rc = V->need(B::V2, np * 8)) || (rc = V->need(B::L1, np * 4)))
+        return rc;
+    uint64_t *K1 = V->at<uint64_t>(B::K1), *K2 = V->at<uint64_t>(B::K2);
+    double *V1 = V->at<double>(B::V1), *V2 = V->at<double>(B::V2);
+    float *L1 = V->at<float>(B::L1);
+    if (nF) {
+        if (multi)
+            k_vox_walk<true, true><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, d_off, K1, V1, nullptr,
+                                                            nullptr, nullptr, d_status);
+        else
+            k_vox_walk<true, false><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, d_off, K1, V1,
+                                                             nullptr, nullptr, nullptr, d_status);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(&hstatus, d_status, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (hstatus) return fail(COMMIT_HIP_EDEVICE, "the recording pass differs from the counting pass");
+    const int kbits = bit_length((uint64_t)(maxo + 1) * nvox - 1);
+    if ((rc = segsort(V, K1, K2, V1, V2, np, n_count, d_off, d_off + 1, kbits > 0 ? kbits : 1, s))) return rc;
+    // merged segments: keys to K1, float lengths to L1, voxel keys into V1's memory
+    uint64_t *VK = (uint64_t *)V1;
+    uint32_t *d_mcnt = V->at<uint32_t>(B::MCNT), *d_kept = V->at<uint32_t>(B::KEPT);
+    uint64_t *d_mend = V->at<uint64_t>(B::MEND);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_mcnt + nF, 1, 0u);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_kept + nF, 1, 0u);
+    if (nF) {
+        k_vox_merge<<<gF, kVoxBlock, 0, s>>>(d_off, n_count, K2, V2, nvox, V->at<float>(B::FLEN), min_fiber_len,
+                                             max_fiber_len, K1, L1, VK, d_mcnt, d_mend, d_kept);
+        HIP_TRY(hipGetLastError());
+    }
+    // per-voxel order inside each kept streamline: voxel keys to K2, lengths to V2's memory
+    float *L2 = (float *)V2;
+    const int vbits = bit_length(nvox - 1);
+    if ((rc = segsort(V, (const uint64_t *)VK, K2, (const float *)L1, L2, np, n_count, d_off, d_mend,
+                      vbits > 0 ? vbits : 1, s)))
+        return rc;
+    if (nF) {
+        k_vox_norm<<<gF, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, K2, L2, V->at<float>(B::NORM));
+        HIP_TRY(hipGetLastError());
+    }
+    uint64_t *d_kidx = V->at<uint64_t>(B::KIDX), *d_spos = V->at<uint64_t>(B::SPOS);
+    if ((rc = exscan_u32(V, d_kept, d_kidx, nF + 1, s))) return rc;
+    if ((rc = exscan_u32(V, d_mcnt, d_spos, nF + 1, s))) return rc;
+    uint64_t tot[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&tot[0], d_kidx + nF, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&tot[1], d_spos + nF, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const size_t n_kept = (size_t)tot[0], n_ic = (size_t)tot[1];
+    if ((rc = V->need(B::IC_F, n_ic * 4)) || (rc = V->need(B::IC_V, n_ic * 4)) || (rc = V->need(B::IC_O, n_ic * 2)) ||
+        (rc = V->need(B::IC_LEN, n_ic * 4)) || (rc = V->need(B::TRK_NORM, n_kept * 4)) ||
+        (rc = V->need(B::TRK_LEN, n_kept * 4)) || (rc = V->need(B::TRK_LENTOT, n_kept * 4)) ||
+        (rc = V->need(B::TRK_KEPT, nF)) || (rc = V->need(B::TDI_V, n_ic * 4)) || (rc = V->need(B::TDI_L, n_ic * 4)) ||
+        (rc = V->need(B::TDI, nvox * 4)) || (rc = V->need(B::HAS, nvox)))
+        return rc;
+    if (nF) {
+        k_vox_write<<<gF, kVoxBlock, 0, s>>>(
+            d_off, n_count, d_kept, d_kidx, d_spos, d_mcnt, K1, L1, nvox, V->at<float>(B::NORM), V->at<float>(B::FLEN),
+            V->at<float>(B::FLENTOT), V->at<uint32_t>(B::IC_F), V->at<uint32_t>(B::IC_V), V->at<uint16_t>(B::IC_O),
+            V->at<float>(B::IC_LEN), V->at<float>(B::TRK_NORM), V->at<float>(B::TRK_LEN), V->at<float>(B::TRK_LENTOT),
+            V->at<uint8_t>(B::TRK_KEPT));
+        HIP_TRY(hipGetLastError());
+    }
+    // TDI and the voxels holding IC segments
+    k_fill_f32<<<grid_for(nvox), kVoxBlock, 0, s>>>(V->at<float>(B::TDI), nvox, 0.0f);
+    k_fill_u8<<<grid_for(nvox), kVoxBlock, 0, s>>>(V->at<uint8_t>(B::HAS), nvox, 0);
+    if ((rc = sort_pairs_u32(V, V->at<uint32_t>(B::IC_V), V->at<uint32_t>(B::TDI_V), V->at<float>(B::IC_LEN),
+                             V->at<float>(B::TDI_L), n_ic, vbits > 0 ? vbits : 1, s)))
+        return rc;
+    if (n_ic) {
+        k_vox_tdi<<<grid_for(n_ic), kVoxBlock, 0, s>>>(V->at<uint32_t>(B::TDI_V), V->at<float>(B::TDI_L), n_ic,
+                                                       V->at<float>(B::TDI), V->at<uint8_t>(B::HAS));
+        HIP_TRY(hipGetLastError());
+    }
+    // EC
+    size_t n_ec = 0;
+    if (ptrPEAKS) {
+        if ((rc = upload(V, B::PEAKS, ptrPEAKS, nvox * 3 * (size_t)Np, s))) return rc;
+        if ((rc = V->need(B::ECNT, (nvox + 1) * 4)) || (rc = V->need(B::EOFF, (nvox + 1) * 8))) return rc;
+        EcArgs e;
+        e.X = Nx; e.Y = Ny; e.Z = Nz; e.Np = Np;
+        e.vf_thr = vf_THR;
+        e.fx = ECix; e.fy = ECiy; e.fz = ECiz;
+        for (int i = 0; i < 9; i++) e.A[i] = ptrPeaksAffine[i];
+        e.peaks = V->at<float>(B::PEAKS);
+        e.hash = a.hash;
+        k_fill_u32<<<1, kVoxBlock, 0, s>>>(V->at<uint32_t>(B::ECNT) + nvox, 1, 0u);
+        k_vox_ec<false><<<grid_for(nvox), kVoxBlock, 0, s>>>(e, nvox, V->at<uint8_t>(B::HAS), V->at<uint32_t>(B::ECNT),
+                                                             nullptr, nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+        if ((rc = exscan_u32(V, V->at<uint32_t>(B::ECNT), V->at<uint64_t>(B::EOFF), nvox + 1, s))) return rc;
+        uint64_t t = 0;
+        HIP_TRY(hipMemcpyAsync(&t, V->at<uint64_t>(B::EOFF) + nvox, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        n_ec = (size_t)t;
+        if ((rc = V->need(B::EC_V, n_ec * 4)) || (rc = V->need(B::EC_O, n_ec * 2))) return rc;
+        k_vox_ec<true><<<grid_for(nvox), kVoxBlock, 0, s>>>(e, nvox, V->at<uint8_t>(B::HAS), nullptr,
+                                                            V->at<uint64_t>(B::EOFF), V->at<uint32_t>(B::EC_V),
+                                                            V->at<uint16_t>(B::EC_O));
+        HIP_TRY(hipGetLastError());
+    }
+    // ISO
+    if ((rc = V->need(B::IFLAG, (nvox + 1) * 4)) || (rc = V->need(B::IOFF, (nvox + 1) * 8))) return rc;
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(V->at<uint32_t>(B::IFLAG) + nvox, 1, 0u);
+    k_vox_iso<false><<<grid_for(nvox), kVoxBlock, 0, s>>>(Nx, Ny, Nz, nvox, d_iso, d_mask, V->at<uint8_t>(B::HAS),
+                                                          V->at<uint32_t>(B::IFLAG), nullptr, nullptr);
+    HIP_TRY(hipGetLastError());
+    if ((rc = exscan_u32(V, V->at<uint32_t>(B::IFLAG), V->at<uint64_t>(B::IOFF), nvox + 1, s))) return rc;
+    uint64_t ti = 0;
+    HIP_TRY(hipMemcpyAsync(&ti, V->at<uint64_t>(B::IOFF) + nvox, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const size_t n_iso = (size_t)ti;
+    if ((rc = V->need(B::ISO_V, n_iso * 4))) return rc;
+    k_vox_iso<true><<<grid_for(nvox), kVoxBlock, 0, s>>>(Nx, Ny, Nz, nvox, d_iso, d_mask, V->at<uint8_t>(B::HAS),
+                                                         V->at<uint32_t>(B::IFLAG), V->at<uint64_t>(B::IOFF),
+                                                         V->at<uint32_t>(B::ISO_V));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    V->n_ic = (int64_t)n_ic;
+    V->n_kept = (int64_t)n_kept;
+    V->n_ec = (int64_t)n_ec;
+    V->n_iso = (int64_t)n_iso;
+    V->n_count = n_count;
+    V->nvox = nvox;
+    V->valid = true;
+    return 0;
+}
+
+extern "C" int commit_hip_vox_sizes(const commit_hip_vox *V, int64_t *n_ic, int64_t *n_kept, int64_t *n_ec,
+                                    int64_t *n_iso, int64_t *n_pieces)
+{
+    if (!V) return fail(COMMIT_HIP_EINVAL, "null voxeliser handle");
+    if (!V->valid) return fail(COMMIT_HIP_EINVAL, "no successful commit_hip_trk2dictionary on this handle");
+    if (n_ic) *n_ic = V->n_ic;
+    if (n_kept) *n_kept = V->n_kept;
+    if (n_ec) *n_ec = V->n_ec;
+    if (n_iso) *n_iso = V->n_iso;
+    if (n_pieces) *n_pieces = V->n_pieces;
+    return 0;
+}
+
+extern "C" int commit_hip_vox_fetch(commit_hip_vox *V, uint32_t *IC_f, uint32_t *IC_v, uint16_t *IC_o,
+                                    float *IC_len, float *TRK_norm, float *TRK_len, float *TRK_lenTot,
+                                    uint8_t *TRK_kept, uint32_t *EC_v, uint16_t *EC_o, uint32_t *ISO_v, float *tdi)
+{
+    using B = commit_hip_vox;
+    if (!V) return fail(COMMIT_HIP_EINVAL, "null voxeliser handle");
+    if (!V->valid) return fail(COMMIT_HIP_EINVAL, "no successful commit_hip_trk2dictionary on this handle");
+    HIP_TRY(hipSetDevice(V->device));
+    struct Item {
+        void *dst;
+        int buf;
+        size_t bytes;
+    } items[] = {
+        {IC_f, B::IC_F, (size_t)V->n_ic * 4},        {IC_v, B::IC_V, (size_t)V->n_ic * 4},
+        {IC_o, B::IC_O, (size_t)V->n_ic * 2},        {IC_len, B::IC_LEN, (size_t)V->n_ic * 4},
+        {TRK_norm, B::TRK_NORM, (size_t)V->n_kept * 4}, {TRK_len, B::TRK_LEN, (size_t)V->n_kept * 4},
+        {TRK_lenTot, B::TRK_LENTOT, (size_t)V->n_kept * 4}, {TRK_kept, B::TRK_KEPT, (size_t)V->n_count},
+        {EC_v, B::EC_V, (size_t)V->n_ec * 4},        {EC_o, B::EC_O, (size_t)V->n_ec * 2},
+        {ISO_v, B::ISO_V, (size_t)V->n_iso * 4},     {tdi, B::TDI, (size_t)V->nvox * 4},
+    };
+    for (const Item &it : items)
+        if (it.dst && it.bytes) HIP_TRY(hipMemcpy(it.dst, V->buf[it.buf].p, it.bytes, hipMemcpyDeviceToHost));
+    return 0;
+}

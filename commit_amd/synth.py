@@ -56,6 +56,9 @@ def _lib():
         lib.synth_stable_argsort_u32.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
                                                  ctypes.c_void_p, ctypes.c_void_p]
         lib.synth_stable_argsort_u32.restype = ctypes.c_int
+        lib.synth_streamlines.argtypes = [ctypes.POINTER(_Params), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+        lib.synth_streamlines.restype = ctypes.c_int
         _synth = lib
     return _synth
 
@@ -201,6 +204,37 @@ def tractogram(seed, nF, dim, pixdim=1.0, mask=None, mode=0, len_mm=(20.0, 200.0
     raw["pixdim"] = float(pixdim)
     raw["dirs"] = dirs
     return raw
+
+
+def streamlines(seed, nF, dim, pixdim=1.0, mask=None, mode=1, len_mm=(20.0, 200.0), step_vox=0.5,
+                nthreads=None):
+    """Seeded random cubic B-spline streamlines as polylines: (fibers float32
+    [n_points, 3] in mm, voxmm space; fib_ptr int64 [nF+1]).  Streamline f is
+    the curve of ``tractogram``'s fibre f before any redraw, sampled every
+    ``step_vox`` voxels; mask and lengths are left to the voxeliser."""
+    p = _Params()
+    p.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    p.nF = int(nF)
+    p.dim[0], p.dim[1], p.dim[2] = (int(d) for d in dim)
+    p.pixdim = float(pixdim)
+    mflat = None
+    if mask is not None:
+        mflat = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).ravel(order="F"))
+        p.mask = mflat.ctypes.data
+    p.mode = int(mode)
+    p.len_min_mm, p.len_max_mm = float(len_mm[0]), float(len_mm[1])
+    p.step_vox = float(step_vox)
+    p.nthreads = int(nthreads or _threads())
+    counts = np.zeros(int(nF), np.int64)
+    lib = _lib()
+    if lib.synth_streamlines(ctypes.byref(p), counts.ctypes.data, None, None) != 0:
+        raise RuntimeError("streamline generation failed")
+    fib_ptr = np.zeros(int(nF) + 1, np.int64)
+    np.cumsum(counts, out=fib_ptr[1:])
+    fibers = np.empty((int(fib_ptr[-1]), 3), np.float32)
+    if lib.synth_streamlines(ctypes.byref(p), counts.ctypes.data, fib_ptr.ctypes.data, fibers.ctypes.data) != 0:
+        raise RuntimeError("streamline generation failed")
+    return fibers, fib_ptr
 
 
 def add_peaks_and_iso(raw):

@@ -404,6 +404,66 @@ int commit_hip_At_multi_momentum(commit_hip_plan *plan, const commit_hip_multi_l
 int commit_hip_fista_multi_roles(const commit_hip_plan *plan, const commit_hip_multi_layout *L, float *r0, float *r1,
                                  int32_t *sel, const commit_hip_fista_col *col, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Tractogram voxeliser: streamlines -> dictionary, on the GPU.
+ * ---------------------------------------------------------------------- */
+
+/* A voxeliser handle keeps its device buffers between runs (they grow on
+ * demand) and the dictionary of its last successful run.  Calls on one handle
+ * must be serialised by the caller. */
+typedef struct commit_hip_vox commit_hip_vox;
+int commit_hip_vox_create(commit_hip_vox **vox, int device);
+int commit_hip_vox_destroy(commit_hip_vox *vox);
+
+/*
+ * Replaces trk2dictionary() (trk2dictionary_c.cpp:101-258, called from
+ * trk2dictionary.pyx:535-541) for streamlines already read into memory: the
+ * IC segments (ICSegments + fiberForwardModel + segmentForwardModel +
+ * rayBoxIntersection, :444-848), the EC peaks (ECSegments, :266-383), the ISO
+ * voxels (ISOcompartments, :386-436), the kept-streamline renumbering of
+ * IC_f (trk2dictionary.pyx:551-563) and the TDI map (compute_tdi, pyx:46-54).
+ * Arguments keep the reference's names and meanings, except:
+ *   fibers      float32 [n_points][3] streamline coordinates in voxmm, as
+ *               read_fiberTRK / read_fiberTCK return them (:851-911), and
+ *   fib_ptr     int64 [n_count+1] first point of each streamline, in place of
+ *               the file name, data offset, scalars and properties; both are
+ *               HOST pointers, or DEVICE pointers when inputs_on_device = 1;
+ *   fiber_shift in voxel units (the reference scales it by the voxel size);
+ *   ptrTDI, path_out, threads_count, verbose are dropped: the results stay in
+ *               the handle (commit_hip_vox_sizes / _fetch);
+ *   ptrBlurApplyTo is uint8 per streamline (NULL: blur all);
+ *   stream      hipStream_t of the work (the call returns when it is done).
+ * Masks and peaks are HOST float32 arrays in C order [Nx][Ny][Nz](...), as the
+ * reference receives them; ptrPEAKS / ptrMASK / ptrISO may be NULL.
+ * ptrPeaksAffine: 3x3 row-major (the identity unless peaks_use_affine).
+ * Returns COMMIT_HIP_EUNSUPPORTED for a streamline of >= 10000 points (the
+ * reference's MAX_FIB_LEN, where it generates no dictionary), for a grid of
+ * more than 2^32 voxels, or when a polyline step crosses more voxels than a
+ * cap of 64 + 12 * length / (smallest voxel size) (the reference has no cap).
+ * Every output is bit-for-bit that of the reference's arithmetic.
+ */
+int commit_hip_trk2dictionary(commit_hip_vox *vox, const float *fibers, const int64_t *fib_ptr, int n_count,
+                              int64_t n_points, int inputs_on_device, int Nx, int Ny, int Nz, float Px, float Py,
+                              float Pz, float fiber_shiftX, float fiber_shiftY, float fiber_shiftZ,
+                              float min_seg_len, float min_fiber_len, float max_fiber_len, const float *ptrPEAKS,
+                              int Np, float vf_THR, int ECix, int ECiy, int ECiz, const float *ptrMASK,
+                              const float *ptrISO, int c, const double *ptrPeaksAffine, int nReplicas,
+                              const double *ptrBlurRho, const double *ptrBlurAngle, const double *ptrBlurWeights,
+                              const uint8_t *ptrBlurApplyTo, const short *ptrHashTable, void *stream);
+
+/* Sizes of the last run: IC segments, kept streamlines, EC peaks, ISO voxels,
+ * and the pieces recorded before aggregation (any pointer may be NULL). */
+int commit_hip_vox_sizes(const commit_hip_vox *vox, int64_t *n_ic, int64_t *n_kept, int64_t *n_ec,
+                         int64_t *n_iso, int64_t *n_pieces);
+
+/* Copies the last run's dictionary to HOST arrays (NULL skips one): the
+ * contents of dictionary_{IC_f, IC_v, IC_o, IC_len, TRK_norm, TRK_len,
+ * TRK_lenTot, TRK_kept, EC_v, EC_o, ISO_v}.dict (trk2dictionary.pyx:565-612)
+ * and the TDI map, float32 [Nx*Ny*Nz] with x fastest (pyx:616-620). */
+int commit_hip_vox_fetch(commit_hip_vox *vox, uint32_t *IC_f, uint32_t *IC_v, uint16_t *IC_o, float *IC_len,
+                         float *TRK_norm, float *TRK_len, float *TRK_lenTot, uint8_t *TRK_kept, uint32_t *EC_v,
+                         uint16_t *EC_o, uint32_t *ISO_v, float *tdi);
+
 /* Thread-local message describing the last failure of this thread. */
 const char *commit_hip_last_error(void);
 

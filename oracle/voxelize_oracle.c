@@ -1,0 +1,533 @@
+/*
+ * ORACLE — test infrastructure only.  Nothing under commit_amd/ links, loads
+ * or calls this file; tests/ use it as the checker of the device voxeliser and
+ * bench.py times it as the CPU baseline of that path.
+ *
+ * CPU restatement of the reference's tractogram voxeliser, the IC/EC/ISO
+ * export of trk2dictionary() for streamlines already read into memory:
+ *   vo_fiber          fiberForwardModel       trk2dictionary_c.cpp:584-735
+ *   vo_segment        segmentForwardModel     trk2dictionary_c.cpp:741-789
+ *   vo_raybox         rayBoxIntersection      trk2dictionary_c.cpp:795-848
+ *   vo_run (IC part)  ICSegments per fibre    trk2dictionary_c.cpp:444-577
+ *                     (std::map<segKey,float> aggregation in (o,z,y,x) key
+ *                     order, :24-41; TRK_norm from the per-voxel sums, :530-537)
+ *   vo_run (EC part)  ECSegments              trk2dictionary_c.cpp:266-383
+ *   vo_run (ISO part) ISOcompartments         trk2dictionary_c.cpp:386-436
+ *   vo_run (TDI)      compute_tdi             trk2dictionary.pyx:46-54
+ * plus the kept-fibre renumbering of the per-thread IC_f files
+ * (trk2dictionary.pyx:551-563): IC_f is the index among kept streamlines.
+ *
+ * Arithmetic follows the reference operation by operation: coordinates are
+ * float sums (point + shift) widened to double, Vector<double>::norm() and
+ * Normalize() go through a float length (Vector.h:47-60), the per-key map
+ * values, FiberLen and FiberLenTot are floats accumulated from doubles, and
+ * the fibre norm is a float sum of squared per-voxel float sums.  Build with
+ * -ffp-contract=off (no fused multiply-adds), as the reference's own build
+ * (plain -O2/-O3 for x86-64, no FMA) evaluates it.
+ *
+ * One deviation: the voxel-crossing loop of one polyline step (the reference's
+ * `while(1)`, :687-729) is capped at vo_max_cross(len) iterations; a step that
+ * needs more sets the run's status to 1.  Real streamlines never come close.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double x, y, z; } v3;
+
+typedef struct {
+    int dim[3];
+    float pixdim[3];
+    float shift[3];               /* fiber_shift in voxel units (scaled by pixdim here, :124-126) */
+    float min_seg_len, min_fiber_len, max_fiber_len;
+    int do_intersect;
+    const float *mask;            /* C order [Nx][Ny][Nz] or NULL (:777) */
+    const float *iso;             /* C order [Nx][Ny][Nz] or NULL (:413) */
+    const float *peaks;           /* C order [Nx][Ny][Nz][3*Np] or NULL (:287) */
+    int Np;
+    float vf_thr;
+    int flip[3];                  /* ECix, ECiy, ECiz: +1 or -1 */
+    const double *peaks_affine;   /* 3x3 row-major, or NULL with no peaks */
+    int n_rep;
+    const double *rho, *angle, *weight;
+    const uint8_t *apply_blur;    /* per streamline; NULL = all */
+    const short *hash;            /* 181*181 */
+    int nthreads;
+} vo_params;
+
+typedef struct {
+    /* IC, in streamline order (kept streamlines only) */
+    uint32_t *IC_f, *IC_v;
+    uint16_t *IC_o;
+    float *IC_len;
+    int64_t n_ic;
+    float *TRK_norm, *TRK_len, *TRK_lenTot;   /* per kept streamline */
+    uint8_t *TRK_kept;                        /* per input streamline */
+    int64_t n_kept;
+    uint32_t *EC_v;
+    uint16_t *EC_o;
+    int64_t n_ec;
+    uint32_t *ISO_v;
+    int64_t n_iso;
+    float *tdi;                               /* [Nx*Ny*Nz], x fastest (pyx:48-53) */
+    int status;
+} vo_result;
+
+/* Vector<double>::norm (Vector.h:47-50): double sqrt returned as float */
+static inline float vnorm(v3 a) { return (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+/* Vector<double>::Normalize (Vector.h:54-61) */
+static inline v3 vnormalize(v3 a)
+{
+    float len = (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    if (len == 0) return a;
+    a.x = a.x / len; a.y = a.y / len; a.z = a.z / len;
+    return a;
+}
+
+/* cap of the voxel-crossing loop for a sub-step of length len (mm) */
+static inline long vo_max_cross(double len, const float *pix)
+{
+    double m = pix[0] < pix[1] ? pix[0] : pix[1];
+    if (pix[2] < m) m = pix[2];
+    double c = 3.0 * len / m;
+    if (!(c < 1e7)) c = 1e7;
+    return 64 + 4 * (long)c;
+}
+
+typedef struct { uint64_t key; uint64_t seq; double w; } piece;
+
+typedef struct {
+    const vo_params *p;
+    piece *pc;
+    size_t n, cap;
+    uint64_t seq;
+    float flen, flentot;
+    int status;
+} fibre_ctx;
+
+static void push_piece(fibre_ctx *c, uint64_t key, double w)
+{
+    if (c->n == c->cap) {
+        c->cap = c->cap ? 2 * c->cap : 256;
+        c->pc = (piece *)realloc(c->pc, c->cap * sizeof(piece));
+    }
+    c->pc[c->n].key = key;
+    c->pc[c->n].seq = c->seq++;
+    c->pc[c->n].w = w;
+    c->n++;
+}
+
+/* round(), as the reference uses on the degree angles (:785-786) */
+static inline int hash_index(const short *hash, double colat, double lon)
+{
+    int ox = (int)round(colat / M_PI * 180.0);
+    int oy = (int)round(lon / M_PI * 180.0);
+    long idx = (long)ox * 181 + oy;
+    if (idx < 0) idx = 0;               /* the reference reads before the table here */
+    if (idx >= 181 * 181) idx = 181 * 181 - 1;
+    return hash[idx];
+}
+
+/* segmentForwardModel (trk2dictionary_c.cpp:741-789) */
+static void vo_segment(fibre_ctx *c, v3 P1, v3 P2, int k, double w)
+{
+    const vo_params *p = c->p;
+    v3 dir;
+    dir.y = P2.y - P1.y;
+    if (dir.y >= 0) {
+        dir.x = P2.x - P1.x;
+        dir.z = P2.z - P1.z;
+    } else {
+        dir.x = P1.x - P2.x;
+        dir.y = P1.y - P2.y;
+        dir.z = P1.z - P2.z;
+    }
+    double len = vnorm(dir);
+    if (w * len <= p->min_seg_len) return;
+    dir = vnormalize(dir);
+
+    int vx = (int)floor(0.5 * (P1.x + P2.x) / p->pixdim[0]);
+    int vy = (int)floor(0.5 * (P1.y + P2.y) / p->pixdim[1]);
+    int vz = (int)floor(0.5 * (P1.z + P2.z) / p->pixdim[2]);
+    if (vx >= p->dim[0] || vx < 0 || vy >= p->dim[1] || vy < 0 || vz >= p->dim[2] || vz < 0) return;
+    if (p->mask && p->mask[vz + (size_t)p->dim[2] * (vy + (size_t)p->dim[1] * vx)] == 0) return;
+
+    double lon = atan2(dir.y, dir.x);
+    double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
+    unsigned short o = (unsigned short)hash_index(p->hash, colat, lon);
+    uint64_t nvox = (uint64_t)p->dim[0] * p->dim[1] * p->dim[2];
+    uint64_t v = (uint64_t)vx + (uint64_t)p->dim[0] * ((uint64_t)vy + (uint64_t)p->dim[1] * vz);
+    push_piece(c, (uint64_t)o * nvox + v, w * len);
+    c->flentot = (float)((double)c->flentot + w * len);
+    if (k == 0) c->flen = (float)((double)c->flen + len);
+}
+
+/* rayBoxIntersection (trk2dictionary_c.cpp:795-848) */
+static int vo_raybox(v3 o, v3 d, v3 vmin, v3 vmax, double *t)
+{
+    double tmin, tmax, tymin, tymax, tzmin, tzmax;
+    v3 inv = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    if (inv.x >= 0) { tmin = (vmin.x - o.x) * inv.x; tmax = (vmax.x - o.x) * inv.x; }
+    else            { tmin = (vmax.x - o.x) * inv.x; tmax = (vmin.x - o.x) * inv.x; }
+    if (inv.y >= 0) { tymin = (vmin.y - o.y) * inv.y; tymax = (vmax.y - o.y) * inv.y; }
+    else            { tymin = (vmax.y - o.y) * inv.y; tymax = (vmin.y - o.y) * inv.y; }
+    if ((tmin > tymax) || (tymin > tmax)) return 0;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    if (inv.z >= 0) { tzmin = (vmin.z - o.z) * inv.z; tzmax = (vmax.z - o.z) * inv.z; }
+    else            { tzmin = (vmax.z - o.z) * inv.z; tzmax = (vmin.z - o.z) * inv.z; }
+    if ((tmin > tzmax) || (tzmin > tmax)) return 0;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    *t = tmin;
+    if (*t <= 0) *t = tmax;
+    return 1;
+}
+
+/* fiberForwardModel (trk2dictionary_c.cpp:584-735) for one streamline of
+ * `pts` points (x,y,z interleaved floats). */
+static void vo_fiber(fibre_ctx *c, const float *fib, int pts, int do_blur, v3 *P,
+                     const double *sinh_, const double *cosh_)
+{
+    const vo_params *p = c->p;
+    const float sx = p->shift[0] * p->pixdim[0], sy = p->shift[1] * p->pixdim[1], sz = p->shift[2] * p->pixdim[2];
+    c->flen = 0.0f;
+    c->flentot = 0.0f;
+    c->n = 0;
+    c->seq = 0;
+    if (pts <= 2) return;
+
+    v3 S1 = {(double)(fib[0] + sx), (double)(fib[1] + sy), (double)(fib[2] + sz)};
+    v3 dir2 = {(double)(fib[3] + sx) - S1.x, (double)(fib[4] + sy) - S1.y, (double)(fib[5] + sz) - S1.z};
+    dir2 = vnormalize(dir2);
+    v3 n = {dir2.y - dir2.z, dir2.z - dir2.x, dir2.x - dir2.y};
+    n = vnormalize(n);
+    for (int k = 0; k < p->n_rep; k++) {
+        if (!do_blur && k > 0) continue;
+        double R = p->rho[k];
+        double w = sinh_[k];
+        v3 q = {dir2.x * w, dir2.y * w, dir2.z * w};
+        w = cosh_[k];
+        v3 qxn = {2.0 * (q.y * n.z - q.z * n.y), 2.0 * (q.z * n.x - q.x * n.z), 2.0 * (q.x * n.y - q.y * n.x)};
+        v3 qq = {q.y * qxn.z - q.z * qxn.y, q.z * qxn.x - q.x * qxn.z, q.x * qxn.y - q.y * qxn.x};
+        v3 nr = {n.x + w * qxn.x + qq.x, n.y + w * qxn.y + qq.y, n.z + w * qxn.z + qq.z};
+        nr = vnormalize(nr);
+        P[k].x = S1.x + R * nr.x;
+        P[k].y = S1.y + R * nr.y;
+        P[k].z = S1.z + R * nr.z;
+    }
+
+    for (int i = 1; i < pts; i++) {
+        v3 S2 = {(double)(fib[3 * i] + sx), (double)(fib[3 * i + 1] + sy), (double)(fib[3 * i + 2] + sz)};
+        v3 dir1 = {S2.x - (double)(fib[3 * i - 3] + sx), S2.y - (double)(fib[3 * i - 2] + sy),
+                   S2.z - (double)(fib[3 * i - 1] + sz)};
+        dir1 = vnormalize(dir1);
+        if (i == pts - 1) {
+            dir2 = dir1;
+        } else {
+            dir2.x = (double)(fib[3 * i + 3] + sx) - S2.x;
+            dir2.y = (double)(fib[3 * i + 4] + sy) - S2.y;
+            dir2.z = (double)(fib[3 * i + 5] + sz) - S2.z;
+            dir2 = vnormalize(dir2);
+        }
+        v3 nn = {0.5 * (dir1.x + dir2.x), 0.5 * (dir1.y + dir2.y), 0.5 * (dir1.z + dir2.z)};
+        double dot = dir1.x * nn.x + dir1.y * nn.y + dir1.z * nn.z;
+        nn.x /= dot; nn.y /= dot; nn.z /= dot;
+
+        for (int k = 0; k < p->n_rep; k++) {
+            if (!do_blur && k > 0) continue;
+            if (p->weight[k] < 1e-3) continue;
+            v3 Po = P[k];
+            double len = (S2.x - Po.x) * nn.x + (S2.y - Po.y) * nn.y + (S2.z - Po.z) * nn.z;
+            if (!(len > 0)) continue;
+            P[k].x += dir1.x * len;
+            P[k].y += dir1.y * len;
+            P[k].z += dir1.z * len;
+            if (!p->do_intersect) {
+                vo_segment(c, Po, P[k], k, p->weight[k]);
+                continue;
+            }
+            v3 S1m = Po, S2m = P[k];
+            long cap = -1;
+            for (long it = 0;; it++) {
+                double dx = S2m.x - S1m.x, dy = S2m.y - S1m.y, dz = S2m.z - S1m.z;
+                double sl = sqrt(dx * dx + dy * dy + dz * dz);
+                if (sl <= p->min_seg_len) break;
+                if (cap < 0) cap = vo_max_cross(sl, p->pixdim);
+                if (it >= cap) { c->status = 1; break; }
+                if (floor(S1m.x / p->pixdim[0]) == floor(S2m.x / p->pixdim[0]) &&
+                    floor(S1m.y / p->pixdim[1]) == floor(S2m.y / p->pixdim[1]) &&
+                    floor(S1m.z / p->pixdim[2]) == floor(S2m.z / p->pixdim[2])) {
+                    vo_segment(c, S1m, S2m, k, p->weight[k]);
+                    break;
+                }
+                v3 vmin, vmax;
+                vmin.x = floor((S1m.x + 1e-6 * dir1.x) / p->pixdim[0]) * p->pixdim[0];
+                vmin.y = floor((S1m.y + 1e-6 * dir1.y) / p->pixdim[1]) * p->pixdim[1];
+                vmin.z = floor((S1m.z + 1e-6 * dir1.z) / p->pixdim[2]) * p->pixdim[2];
+                vmax.x = vmin.x + p->pixdim[0];
+                vmax.y = vmin.y + p->pixdim[1];
+                vmax.z = vmin.z + p->pixdim[2];
+                double t;
+                if (vo_raybox(S1m, dir1, vmin, vmax, &t) && t > 0 && t < sl) {
+                    v3 Pi = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
+                    vo_segment(c, S1m, Pi, k, p->weight[k]);
+                    S1m = Pi;
+                } else {
+                    vo_segment(c, S1m, S2m, k, p->weight[k]);
+                    break;
+                }
+            }
+        }
+    }
+}
+
+static int cmp_piece(const void *a, const void *b)
+{
+    const piece *x = (const piece *)a, *y = (const piece *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->seq > y->seq) - (x->seq < y->seq);
+}
+
+typedef struct { uint64_t v; uint64_t pos; } vkey;
+
+static int cmp_vkey(const void *a, const void *b)
+{
+    const vkey *x = (const vkey *)a, *y = (const vkey *)b;
+    if (x->v != y->v) return x->v < y->v ? -1 : 1;
+    return (x->pos > y->pos) - (x->pos < y->pos);
+}
+
+/* per-streamline output before renumbering */
+typedef struct {
+    uint64_t *key;
+    float *len;
+    int64_t m;
+    float norm, flen, flentot;
+    uint8_t kept;
+} fibre_out;
+
+typedef struct {
+    const vo_params *p;
+    const float *pts;
+    const int64_t *ptr;
+    int f0, f1;
+    fibre_out *out;
+    int status;
+} vo_job;
+
+static void *vo_worker(void *arg)
+{
+    vo_job *j = (vo_job *)arg;
+    const vo_params *p = j->p;
+    fibre_ctx c;
+    memset(&c, 0, sizeof c);
+    c.p = p;
+    v3 *P = (v3 *)malloc((size_t)(p->n_rep > 0 ? p->n_rep : 1) * sizeof(v3));
+    double *sh = (double *)malloc((size_t)(p->n_rep > 0 ? p->n_rep : 1) * sizeof(double));
+    double *ch = (double *)malloc((size_t)(p->n_rep > 0 ? p->n_rep : 1) * sizeof(double));
+    for (int k = 0; k < p->n_rep; k++) {            /* :614-619 */
+        sh[k] = sin(p->angle[k] / 2.0);
+        ch[k] = cos(p->angle[k] / 2.0);
+    }
+    uint64_t nvox = (uint64_t)p->dim[0] * p->dim[1] * p->dim[2];
+    vkey *vk = NULL;
+    size_t vkcap = 0;
+    for (int f = j->f0; f < j->f1; f++) {
+        int npt = (int)(j->ptr[f + 1] - j->ptr[f]);
+        int blur = p->apply_blur ? (p->apply_blur[f] != 0) : 1;
+        vo_fiber(&c, j->pts + 3 * j->ptr[f], npt, blur, P, sh, ch);
+        fibre_out *o = &j->out[f];
+        memset(o, 0, sizeof *o);
+        o->flen = c.flen;
+        o->flentot = c.flentot;
+        if (c.n == 0) continue;
+        /* std::map<segKey,float>: ascending (o,z,y,x), values accumulated in
+         * emission order into a float (:782) */
+        qsort(c.pc, c.n, sizeof(piece), cmp_piece);
+        o->key = (uint64_t *)malloc(c.n * sizeof(uint64_t));
+        o->len = (float *)malloc(c.n * sizeof(float));
+        int64_t m = 0;
+        for (size_t i = 0; i < c.n; i++) {
+            if (i == 0 || c.pc[i].key != c.pc[i - 1].key) {
+                o->key[m] = c.pc[i].key;
+                o->len[m] = 0.0f;
+                m++;
+            }
+            o->len[m - 1] = (float)((double)o->len[m - 1] + c.pc[i].w);
+        }
+        o->m = m;
+        if (!(c.flen > p->min_fiber_len && c.flen < p->max_fiber_len)) continue;   /* :505 */
+        o->kept = 1;
+        /* FiberNorm: per-voxel float sums visited in map order, then the float
+         * sum of their squares in (z,y,x) order (:524-537) */
+        if ((size_t)m > vkcap) {
+            vkcap = (size_t)m * 2;
+            vk = (vkey *)realloc(vk, vkcap * sizeof(vkey));
+        }
+        for (int64_t i = 0; i < m; i++) { vk[i].v = o->key[i] % nvox; vk[i].pos = (uint64_t)i; }
+        qsort(vk, (size_t)m, sizeof(vkey), cmp_vkey);
+        float fn = 0.0f;
+        int64_t i = 0;
+        while (i < m) {
+            float s = 0.0f;
+            int64_t k = i;
+            while (k < m && vk[k].v == vk[i].v) { s = s + o->len[vk[k].pos]; k++; }
+            fn = (float)((double)fn + (double)s * (double)s);
+            i = k;
+        }
+        o->norm = (float)sqrt((double)fn);
+    }
+    j->status = c.status;
+    free(c.pc);
+    free(vk);
+    free(P);
+    free(sh);
+    free(ch);
+    return NULL;
+}
+
+/* Runs the whole export.  Returns 0, or 1 when a crossing loop hit its cap,
+ * or -1 on allocation failure.  Arrays in `r` are malloc'ed; vo_free frees them. */
+int vo_run(const vo_params *p, const float *pts, const int64_t *fib_ptr, int n_count, vo_result *r)
+{
+    memset(r, 0, sizeof *r);
+    uint64_t nvox = (uint64_t)p->dim[0] * p->dim[1] * p->dim[2];
+    fibre_out *out = (fibre_out *)calloc((size_t)(n_count > 0 ? n_count : 1), sizeof(fibre_out));
+    int nt = p->nthreads < 1 ? 1 : (p->nthreads > 255 ? 255 : p->nthreads);
+    if (nt > n_count) nt = n_count > 0 ? n_count : 1;
+    vo_job *jobs = (vo_job *)calloc((size_t)nt, sizeof(vo_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nt, sizeof(pthread_t));
+    for (int t = 0; t < nt; t++) {
+        jobs[t].p = p;
+        jobs[t].pts = pts;
+        jobs[t].ptr = fib_ptr;
+        jobs[t].f0 = (int)((int64_t)n_count * t / nt);
+        jobs[t].f1 = (int)((int64_t)n_count * (t + 1) / nt);
+        jobs[t].out = out;
+        if (pthread_create(&th[t], NULL, vo_worker, &jobs[t]) != 0) { vo_worker(&jobs[t]); th[t] = 0; }
+    }
+    for (int t = 0; t < nt; t++) if (th[t]) pthread_join(th[t], NULL);
+    for (int t = 0; t < nt; t++) if (jobs[t].status) r->status = 1;
+
+    int64_t n_ic = 0, n_kept = 0;
+    for (int f = 0; f < n_count; f++)
+        if (out[f].kept) { n_ic += out[f].m; n_kept++; }
+    r->n_ic = n_ic;
+    r->n_kept = n_kept;
+    r->IC_f = (uint32_t *)malloc((size_t)(n_ic ? n_ic : 1) * 4);
+    r->IC_v = (uint32_t *)malloc((size_t)(n_ic ? n_ic : 1) * 4);
+    r->IC_o = (uint16_t *)malloc((size_t)(n_ic ? n_ic : 1) * 2);
+    r->IC_len = (float *)malloc((size_t)(n_ic ? n_ic : 1) * 4);
+    r->TRK_norm = (float *)malloc((size_t)(n_kept ? n_kept : 1) * 4);
+    r->TRK_len = (float *)malloc((size_t)(n_kept ? n_kept : 1) * 4);
+    r->TRK_lenTot = (float *)malloc((size_t)(n_kept ? n_kept : 1) * 4);
+    r->TRK_kept = (uint8_t *)malloc((size_t)(n_count ? n_count : 1));
+    r->tdi = (float *)calloc((size_t)(nvox ? nvox : 1), 4);
+    uint8_t *has_ic = (uint8_t *)calloc((size_t)(nvox ? nvox : 1), 1);
+    if (!out || !r->IC_f || !r->IC_v || !r->IC_o || !r->IC_len || !r->TRK_norm || !r->TRK_len ||
+        !r->TRK_lenTot || !r->TRK_kept || !r->tdi || !has_ic)
+        return -1;
+    int64_t s = 0, kf = 0;
+    for (int f = 0; f < n_count; f++) {
+        fibre_out *o = &out[f];
+        r->TRK_kept[f] = o->kept;
+        if (o->kept) {
+            for (int64_t i = 0; i < o->m; i++) {
+                uint64_t v = o->key[i] % nvox;
+                r->IC_f[s] = (uint32_t)kf;
+                r->IC_v[s] = (uint32_t)v;
+                r->IC_o[s] = (uint16_t)(o->key[i] / nvox);
+                r->IC_len[s] = o->len[i];
+                r->tdi[v] = r->tdi[v] + o->len[i];        /* compute_tdi, float32 (pyx:50-52) */
+                has_ic[v] = 1;
+                s++;
+            }
+            r->TRK_norm[kf] = o->norm;
+            r->TRK_len[kf] = o->flen;
+            r->TRK_lenTot[kf] = o->flentot;
+            kf++;
+        }
+        free(o->key);
+        free(o->len);
+    }
+    free(out);
+    free(jobs);
+    free(th);
+
+    const int X = p->dim[0], Y = p->dim[1], Z = p->dim[2];
+    /* EC (:266-383): voxels in (z, y, x) loop order, peaks in file order */
+    int64_t cap = 1024, n_ec = 0;
+    r->EC_v = (uint32_t *)malloc((size_t)cap * 4);
+    r->EC_o = (uint16_t *)malloc((size_t)cap * 2);
+    if (p->peaks && p->Np > 0) {
+        float *norms = (float *)malloc((size_t)p->Np * sizeof(float));
+        const double *A = p->peaks_affine;
+        for (int iz = 0; iz < Z; iz++)
+            for (int iy = 0; iy < Y; iy++)
+                for (int ix = 0; ix < X; ix++) {
+                    uint64_t vl = (uint64_t)ix + (uint64_t)X * ((uint64_t)iy + (uint64_t)Y * iz);
+                    if (!has_ic[vl]) continue;
+                    const float *base = p->peaks + 3 * (size_t)p->Np * ((size_t)iz + (size_t)Z * ((size_t)iy + (size_t)Y * ix));
+                    float peakMax = -1;
+                    for (int id = 0; id < p->Np; id++) {
+                        v3 d = {base[3 * id], base[3 * id + 1], base[3 * id + 2]};
+                        norms[id] = vnorm(d);
+                        if (norms[id] > peakMax) peakMax = norms[id];
+                    }
+                    if (!(peakMax > 0)) continue;
+                    for (int id = 0; id < p->Np; id++) {
+                        if (norms[id] == 0 || norms[id] < p->vf_thr * peakMax) continue;
+                        const float *q = base + 3 * id;
+                        v3 d;
+                        d.x = q[0] * A[0] + q[1] * A[1] + q[2] * A[2];
+                        d.y = q[0] * A[3] + q[1] * A[4] + q[2] * A[5];
+                        d.z = q[0] * A[6] + q[1] * A[7] + q[2] * A[8];
+                        d.x *= p->flip[0];
+                        d.y *= p->flip[1];
+                        d.z *= p->flip[2];
+                        if (d.y < 0) { d.x = -d.x; d.y = -d.y; d.z = -d.z; }
+                        double colat = atan2(sqrt(d.x * d.x + d.y * d.y), d.z);
+                        double lon = atan2(d.y, d.x);
+                        if (n_ec == cap) {
+                            cap *= 2;
+                            r->EC_v = (uint32_t *)realloc(r->EC_v, (size_t)cap * 4);
+                            r->EC_o = (uint16_t *)realloc(r->EC_o, (size_t)cap * 2);
+                        }
+                        r->EC_v[n_ec] = (uint32_t)vl;
+                        r->EC_o[n_ec] = (uint16_t)hash_index(p->hash, colat, lon);
+                        n_ec++;
+                    }
+                }
+        free(norms);
+    }
+    r->n_ec = n_ec;
+    /* ISO (:386-436) */
+    int64_t n_iso = 0;
+    r->ISO_v = (uint32_t *)malloc((size_t)(nvox ? nvox : 1) * 4);
+    for (int iz = 0; iz < Z; iz++)
+        for (int iy = 0; iy < Y; iy++)
+            for (int ix = 0; ix < X; ix++) {
+                size_t c = (size_t)iz + (size_t)Z * ((size_t)iy + (size_t)Y * ix);
+                if (p->iso && p->iso[c] == 0) continue;
+                if (p->mask && p->mask[c] == 0) continue;
+                uint64_t vl = (uint64_t)ix + (uint64_t)X * ((uint64_t)iy + (uint64_t)Y * iz);
+                if (!has_ic[vl]) continue;
+                r->ISO_v[n_iso++] = (uint32_t)vl;
+            }
+    r->n_iso = n_iso;
+    free(has_ic);
+    return r->status;
+}
+
+void vo_free(vo_result *r)
+{
+    free(r->IC_f); free(r->IC_v); free(r->IC_o); free(r->IC_len);
+    free(r->TRK_norm); free(r->TRK_len); free(r->TRK_lenTot); free(r->TRK_kept);
+    free(r->EC_v); free(r->EC_o); free(r->ISO_v); free(r->tdi);
+    memset(r, 0, sizeof *r);
+}
