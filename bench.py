@@ -71,10 +71,12 @@ def main():
     ap.add_argument("--rhs", type=int, default=16, help="right-hand sides of the batched leg (0: skip)")
     ap.add_argument("--fit-iters", type=int, default=500,
                     help="FISTA iterations of the mixed-precision fit leg (0: skip); batched-fit mode: max_iter")
-    ap.add_argument("--mode", default="headline", choices=("headline", "batched-fit", "autograd", "products"),
+    ap.add_argument("--mode", default="headline",
+                    choices=("headline", "batched-fit", "autograd", "products", "voxelize"),
                     help="headline: the operator pairs/s line; batched-fit: 8 tractograms x 16 lambdas; "
                          "autograd: differentiable operator on 200k fibres; products: the single-vector "
-                         "products alone with a fixed launch option (--slab), for counter and trace runs")
+                         "products alone with a fixed launch option (--slab), for counter and trace runs; "
+                         "voxelize: the tractogram voxeliser on 1M streamlines")
     ap.add_argument("--slab", type=int, default=2, help="products mode: slab option of the plan (0, 1, 2)")
     ap.add_argument("--residual", default="combine", choices=("combine", "product"),
                     help="batched-fit mode: momentum residual from r(x) and r(x_prev) inside A^T, or a second A X")
@@ -82,6 +84,10 @@ def main():
                     help="only run three plain 256 MiB device copies (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the operator runs on a single MI355X; see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
         return
@@ -94,10 +100,9 @@ def main():
     if args.mode == "products":
         products_mode(args)
         return
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the operator runs on a single MI355X; see the sharding notes)", file=sys.stderr)
-        sys.exit(2)
+    if args.mode == "voxelize":
+        voxelize_mode(args)
+        return
 
     import torch
 
@@ -570,6 +575,92 @@ def products_mode(args):
         print(f"[bench] products {name}: {ms}", file=sys.stderr, flush=True)
     A.set_precision(_lib.FP64)
     print(json.dumps(out), flush=True)
+
+
+def voxelize_mode(args):
+    """Tractogram voxelisation (commit_hip_trk2dictionary) of the full-brain
+    tractogram: 1M streamlines sampled every 0.625 mm (~1.8e8 points) on the
+    145x174x145 grid at 1.25 mm with the ~600k-voxel mask, streamlines of
+    20-250 mm kept.  Points and offsets resident in HBM; a step is one whole
+    export (walks, sorts, merges, norms, TDI, EC and ISO).  CPU baseline: the
+    oracle's C restatement on the host cores, the same tractogram."""
+    import ctypes
+
+    import torch
+    from commit_amd import synth
+    from commit_amd import trk2dictionary as t2d
+    dev = torch.device("cuda", 0)
+    cfg = synth.WORKLOADS["brain"]
+    dim, pix = cfg["dim"], (cfg["pixdim"],) * 3
+    nF = args.fibres or cfg["nF"]
+    mask = synth.ellipsoid_mask(dim, cfg["mask"][1]).astype(np.float32)
+    t0 = time.time()
+    pts, ptr = synth.streamlines(1002, nF, dim, cfg["pixdim"], mask=mask, mode=1, step_vox=0.5)
+    print(f"[bench] {nF} streamlines, {pts.shape[0]} points in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    H = t2d.default_hash_table(500)
+    tp = torch.from_numpy(pts).to(dev)
+    tq = torch.from_numpy(ptr).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    v = t2d.Voxeliser(0)
+    kw = dict(mask=torch.from_numpy(mask).to(dev), min_fiber_len=20.0, max_fiber_len=250.0)
+
+    def step():
+        v.run(tp, tq, dim, pix, H, stream=stream.cuda_stream, **kw)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    ev1.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    ms = ev0.elapsed_time(ev1) / args.steps
+    s = v.sizes()
+    d = v.fetch()
+    v.close()
+    npts = int(pts.shape[0])
+    # compulsory bytes of one export: the points and offsets read once, the
+    # dictionary written once (IC 14 B per segment, 12 B + 1 B per kept and
+    # input streamline, EC 6 B, ISO 4 B, TDI 4 B per voxel)
+    nvox = int(np.prod(dim))
+    alg = (12 * npts + 8 * (nF + 1) + 14 * s["n_ic"] + 12 * s["n_kept"] + nF + 6 * s["n_ec"] + 4 * s["n_iso"]
+           + 4 * nvox + 4 * nvox)
+    print(f"[bench] voxelise: {ms:.2f} ms per export (wall {wall * 1e3:.2f} ms), {s}", file=sys.stderr, flush=True)
+    cpu = None
+    if args.cpu_pairs:
+        from oracle import voxelize as vo
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        n = max(1, min(cores, 255))
+        t0 = time.perf_counter()
+        o = vo.trk2dictionary(pts, ptr, dim, pix, H, n_threads=n, mask=mask, min_fiber_len=20.0,
+                              max_fiber_len=250.0)
+        tc = time.perf_counter() - t0
+        same = all(np.array_equal(np.asarray(o[k]).view(np.uint8), np.asarray(d[k]).view(np.uint8))
+                   for k in ("IC_f", "IC_v", "IC_o", "IC_len", "TRK_norm", "TRK_len", "TRK_kept", "ISO_v", "tdi"))
+        print(f"[bench] cpu voxelise: {tc:.2f}s on {n} threads; outputs bitwise equal: {same}", file=sys.stderr,
+              flush=True)
+        cpu = {"value": round(nF / tc, 1), "unit": "streamlines/s", "cores": n, "kind": "port",
+               "sample": f"the full tractogram once ({nF} streamlines), oracle C restatement of "
+                         f"trk2dictionary_c.cpp, {n} threads; outputs bitwise equal to the GPU's: {same}"}
+    line = {"metric": "tractogram voxelisation streamlines/s (1M streamlines, 145x174x145 at 1.25 mm)",
+            "value": round(nF / (ms * 1e-3), 1), "unit": "streamlines/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+            "dtype": "f64 geometry, f32 lengths", "data": "synthetic",
+            "config": {"workload": f"{nF} streamlines, {npts} points, {s['n_pieces']} pieces, {s['n_ic']} "
+                                   f"segments, {s['n_kept']} kept", "wall_ms_per_step": round(wall * 1e3, 3)},
+            "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": alg, "scope": "whole export"},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
 
 
 def copy_calibration():

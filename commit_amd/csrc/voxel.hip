@@ -185,10 +185,12 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                                                         D3 *__restrict__ Pbuf, const uint64_t *__restrict__ off,
                                                         uint64_t *__restrict__ keys, double *__restrict__ vals,
                                                         uint32_t *__restrict__ cnt, float *__restrict__ flen,
-                                                        float *__restrict__ flentot, uint32_t *__restrict__ status)
+                                                        float *__restrict__ flentot, uint32_t *__restrict__ status,
+                                                        const uint32_t *__restrict__ order)
 {
-    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
-    if (f >= nF) return;
+    const int t = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (t >= nF) return;
+    const int f = (int)order[t];
     Recorder<EMIT> rec(a, EMIT ? keys + off[f] : nullptr, EMIT ? vals + off[f] : nullptr,
                        EMIT ? (uint32_t)(off[f + 1] - off[f]) : 0u);
     const int64_t p0 = fptr[f];
@@ -296,8 +298,24 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
     if (capped || (EMIT && rec.n != rec.limit)) *status = 1;
 }
 
+// Walk order: the streamlines of one wavefront take similar time when they
+// have similar point counts, so threads take them by descending count
+// (a stable radix sort of these keys; ties keep streamline order).
+__global__ __launch_bounds__(kVoxBlock) void k_vox_order_keys(const int64_t *__restrict__ fptr, int nF,
+                                                              uint32_t *__restrict__ key, uint32_t *__restrict__ idx)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f >= nF) return;
+    const int64_t n = fptr[f + 1] - fptr[f];
+    key[f] = (uint32_t)(16383 - (n < 16383 ? n : 16383));
+    idx[f] = (uint32_t)f;
+}
+
 // Equal keys of a streamline's sorted pieces summed into a float in recording
-// order; m[f] merged segments stored from the streamline's first slot.
+// order; m[f] merged segments stored from the streamline's first slot.  One
+// wavefront per streamline: each lane of a 64-piece chunk that starts a run
+// of equal keys adds the run in order and stores it at its rank among the
+// run heads.
 __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restrict__ off, int nF,
                                                          const uint64_t *__restrict__ skey,
                                                          const double *__restrict__ sval, uint64_t nvox,
@@ -306,32 +324,28 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restr
                                                          uint64_t *__restrict__ vkey, uint32_t *__restrict__ mcnt,
                                                          uint64_t *__restrict__ mend, uint32_t *__restrict__ kept)
 {
-    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    const int f = blockIdx.x * (kVoxBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
     if (f >= nF) return;
     const uint64_t b = off[f], e = off[f + 1];
     uint64_t m = 0;
-    float acc = 0.0f;
-    uint64_t cur = 0;
-    for (uint64_t i = b; i < e; i++) {
-        const uint64_t k = skey[i];
-        if (i == b || k != cur) {
-            if (i != b) {
-                mkey[b + m] = cur;
-                mlen[b + m] = acc;
-                vkey[b + m] = cur % nvox;
-                m++;
-            }
-            cur = k;
-            acc = 0.0f;
+    for (uint64_t base = b; base < e; base += kWave) {
+        const uint64_t i = base + lane;
+        const bool valid = i < e;
+        const uint64_t k = valid ? skey[i] : 0;
+        const bool head = valid && (i == b || skey[i - 1] != k);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+            float acc = 0.0f;
+            for (uint64_t j = i; j < e && skey[j] == k; j++) acc = (float)((double)acc + sval[j]);
+            const uint64_t pos = b + m + __popcll(heads & ((1ull << lane) - 1ull));
+            mkey[pos] = k;
+            mlen[pos] = acc;
+            vkey[pos] = k % nvox;
         }
-        acc = (float)((double)acc + sval[i]);
+        m += __popcll(heads);
     }
-    if (e > b) {
-        mkey[b + m] = cur;
-        mlen[b + m] = acc;
-        vkey[b + m] = cur % nvox;
-        m++;
-    }
+    if (lane != 0) return;
     const float fl = flen[f];
     const uint32_t kf = (m > 0 && fl > min_fib && fl < max_fib) ? 1u : 0u;
     mcnt[f] = kf ? (uint32_t)m : 0u;
@@ -369,40 +383,82 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_write(
     uint16_t *__restrict__ IC_o, float *__restrict__ IC_len, float *__restrict__ TRK_norm,
     float *__restrict__ TRK_len, float *__restrict__ TRK_lenTot, uint8_t *__restrict__ TRK_kept)
 {
-    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    const int f = blockIdx.x * (kVoxBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
     if (f >= nF) return;
     const uint32_t kf = kept[f];
-    TRK_kept[f] = (uint8_t)kf;
+    if (lane == 0) TRK_kept[f] = (uint8_t)kf;
     if (!kf) return;
     const uint64_t kk = kidx[f], s = spos[f], b = off[f];
     const uint32_t m = mcnt[f];
-    for (uint32_t i = 0; i < m; i++) {
+    for (uint32_t i = lane; i < m; i += kWave) {
         const uint64_t key = mkey[b + i];
+        const uint64_t o = key / nvox;
         IC_f[s + i] = (uint32_t)kk;
-        IC_v[s + i] = (uint32_t)(key % nvox);
-        IC_o[s + i] = (uint16_t)(key / nvox);
+        IC_v[s + i] = (uint32_t)(key - o * nvox);
+        IC_o[s + i] = (uint16_t)o;
         IC_len[s + i] = mlen[b + i];
     }
+    if (lane != 0) return;
     TRK_norm[kk] = norm[f];
     TRK_len[kk] = flen[f];
     TRK_lenTot[kk] = flentot[f];
 }
 
-// compute_tdi (trk2dictionary.pyx:46-54) from the segments sorted by voxel
-// (stable, so each voxel's lengths are in segment order); marks the voxels
-// holding an IC segment.
-__global__ __launch_bounds__(kVoxBlock) void k_vox_tdi(const uint32_t *__restrict__ vs,
-                                                       const float *__restrict__ ls, uint64_t n,
-                                                       float *__restrict__ tdi, uint8_t *__restrict__ has)
+// Each voxel's range [rs[v], re[v]) in the segments sorted by voxel (stable,
+// so each voxel's lengths are in segment order); voxels without segments keep
+// rs = re = 0.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_runs(const uint32_t *__restrict__ vs, uint64_t n,
+                                                        uint32_t *__restrict__ rs, uint32_t *__restrict__ re)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = vs[i];
-    if (i > 0 && vs[i - 1] == v) return;
+    if (i == 0 || vs[i - 1] != v) rs[v] = (uint32_t)i;
+    if (i == n - 1 || vs[i + 1] != v) re[v] = (uint32_t)(i + 1);
+}
+
+// compute_tdi (trk2dictionary.pyx:46-54): a wavefront per voxel loads its
+// lengths 64 at a time and adds them one by one in segment order (the float
+// sum of the reference's loop); marks the voxels holding an IC segment.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_tdi(const uint32_t *__restrict__ rs,
+                                                       const uint32_t *__restrict__ re,
+                                                       const float *__restrict__ ls, uint64_t nvox,
+                                                       float *__restrict__ tdi, uint8_t *__restrict__ has)
+{
+    const uint64_t v = (uint64_t)blockIdx.x * (kVoxBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (v >= nvox) return;
+    const uint32_t b = rs[v], e = re[v];
+    if (b == e) return;
     float s = 0.0f;
-    for (uint64_t j = i; j < n && vs[j] == v; j++) s = s + ls[j];
-    tdi[v] = s;
-    has[v] = 1;
+    for (uint32_t base = b; base < e; base += kWave) {
+        const float x = base + lane < e ? ls[base + lane] : 0.0f;
+        const int cnt = (int)(e - base < (uint32_t)kWave ? e - base : (uint32_t)kWave);
+        for (int k = 0; k < cnt; k++)
+            s = s + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+    }
+    if (lane == 0) {
+        tdi[v] = s;
+        has[v] = 1;
+    }
+}
+
+// fib_ptr checks on the device: res[0] bit 0 a decreasing offset, bit 1 a
+// streamline of kMaxFibLen points or more, bit 2 bad first or last offset;
+// res[1] the first streamline of bit 1.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_check(const int64_t *__restrict__ fptr, int nF, int64_t npts,
+                                                         uint32_t *__restrict__ res)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f == 0 && (fptr[0] != 0 || fptr[nF] != npts)) atomicOr(&res[0], 4u);
+    if (f >= nF) return;
+    const int64_t n = fptr[f + 1] - fptr[f];
+    if (n < 0) atomicOr(&res[0], 1u);
+    if (n >= kMaxFibLen) {
+        atomicOr(&res[0], 2u);
+        atomicMin(&res[1], (uint32_t)f);
+    }
 }
 
 __global__ __launch_bounds__(kVoxBlock) void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
@@ -529,7 +585,7 @@ struct commit_hip_vox {
         PTS, FPTR, MASK, ISO, PEAKS, HASH, REP, APPLY, PBUF, CNT, FLEN, FLENTOT, OFF, K1, V1, K2, V2, L1,
         MCNT, MEND, KEPT, KIDX, SPOS, NORM, STATUS, TMP,
         IC_F, IC_V, IC_O, IC_LEN, TRK_NORM, TRK_LEN, TRK_LENTOT, TRK_KEPT, TDI_V, TDI_L, TDI, HAS,
-        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, NBUF
+        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, OKEY, OIDX, OKEY2, ORDER, NBUF
     };
     Buf buf[NBUF];
     int64_t n_ic = 0, n_kept = 0, n_ec = 0, n_iso = 0, n_count = 0, n_pieces = 0;
@@ -593,8 +649,9 @@ int segsort(commit_hip_vox *V, const K *ki, K *ko, const Vt *vi, Vt *vo, size_t 
     return 0;
 }
 
-int sort_pairs_u32(commit_hip_vox *V, const uint32_t *ki, uint32_t *ko, const float *vi, float *vo, size_t n,
-                   int bits, hipStream_t s)
+template <class Vt>
+int sort_pairs_u32(commit_hip_vox *V, const uint32_t *ki, uint32_t *ko, const Vt *vi, Vt *vo, size_t n, int bits,
+                   hipStream_t s)
 {
     if (n == 0) return 0;
     size_t bytes = 0;
@@ -667,26 +724,8 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     HIP_TRY(hipSetDevice(V->device));
     hipStream_t s = (hipStream_t)stream;
 
-    // streamline offsets: checked on the host (MAX_FIB_LEN, :171 and :856)
-    std::vector<int64_t> hptr((size_t)n_count + 1, 0);
-    if (n_count > 0) {
-        if (inputs_on_device) {
-            HIP_TRY(hipMemcpyAsync(hptr.data(), fib_ptr, ((size_t)n_count + 1) * sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-        } else {
-            std::memcpy(hptr.data(), fib_ptr, ((size_t)n_count + 1) * sizeof(int64_t));
-        }
-    }
-    if (hptr[0] != 0 || hptr[(size_t)n_count] != n_points)
+    if (!inputs_on_device && n_count > 0 && (fib_ptr[0] != 0 || fib_ptr[n_count] != n_points))
         return fail(COMMIT_HIP_EINVAL, "fib_ptr must run from 0 to n_points (%lld)", (long long)n_points);
-    for (int f = 0; f < n_count; f++) {
-        const int64_t np = hptr[(size_t)f + 1] - hptr[(size_t)f];
-        if (np < 0) return fail(COMMIT_HIP_EINVAL, "fib_ptr decreases at streamline %d", f);
-        if (np >= kMaxFibLen)
-            return fail(COMMIT_HIP_EUNSUPPORTED, "streamline %d has %lld points (the reference's limit is %d)", f,
-                        (long long)np, kMaxFibLen - 1);
-    }
     int maxo = 0;
     for (int i = 0; i < 181 * 181; i++) {
         if (ptrHashTable[i] < 0) return fail(COMMIT_HIP_EINVAL, "negative orientation in the hash table");
@@ -700,10 +739,29 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
         d_pts = fibers;
         d_ptr = fib_ptr;
     } else {
-        if ((rc = upload(V, B::PTS, fibers, (size_t)n_points * 3, s))) return rc;
-        if ((rc = upload(V, B::FPTR, fib_ptr, (size_t)n_count + 1, s))) return rc;
+        if (n_count > 0) {
+            if ((rc = upload(V, B::PTS, fibers, (size_t)n_points * 3, s))) return rc;
+            if ((rc = upload(V, B::FPTR, fib_ptr, (size_t)n_count + 1, s))) return rc;
+        }
         d_pts = V->at<float>(B::PTS);
         d_ptr = V->at<int64_t>(B::FPTR);
+    }
+    // streamline offsets (MAX_FIB_LEN, trk2dictionary_c.cpp:171 and :856)
+    if ((rc = V->need(B::STATUS, 16))) return rc;
+    if (n_count > 0) {
+        uint32_t *st = V->at<uint32_t>(B::STATUS);
+        k_fill_u32<<<1, kVoxBlock, 0, s>>>(st, 1, 0u);
+        k_fill_u32<<<1, kVoxBlock, 0, s>>>(st + 1, 1, 0xFFFFFFFFu);
+        k_vox_check<<<grid_for((size_t)n_count + 1), kVoxBlock, 0, s>>>(d_ptr, n_count, n_points, st);
+        HIP_TRY(hipGetLastError());
+        uint32_t chk[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(chk, st, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (chk[0] & 4u) return fail(COMMIT_HIP_EINVAL, "fib_ptr must run from 0 to n_points (%lld)", (long long)n_points);
+        if (chk[0] & 1u) return fail(COMMIT_HIP_EINVAL, "fib_ptr decreases");
+        if (chk[0] & 2u)
+            return fail(COMMIT_HIP_EUNSUPPORTED, "streamline %u has %d points or more (the reference's limit is %d)",
+                        chk[1], kMaxFibLen, kMaxFibLen - 1);
     }
     if ((rc = upload(V, B::HASH, ptrHashTable, 181 * 181, s))) return rc;
     // replicas: rho, sin(angle/2), cos(angle/2), weight (:614-619)
@@ -716,11 +774,15 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     }
     if ((rc = upload(V, B::REP, rep.data(), rep.size(), s))) return rc;
     const float *d_mask = nullptr, *d_iso = nullptr;
-    if (ptrMASK) {
+    if (ptrMASK && inputs_on_device) {
+        d_mask = ptrMASK;
+    } else if (ptrMASK) {
         if ((rc = upload(V, B::MASK, ptrMASK, nvox, s))) return rc;
         d_mask = V->at<float>(B::MASK);
     }
-    if (ptrISO) {
+    if (ptrISO && inputs_on_device) {
+        d_iso = ptrISO;
+    } else if (ptrISO) {
         if ((rc = upload(V, B::ISO, ptrISO, nvox, s))) return rc;
         d_iso = V->at<float>(B::ISO);
     }
@@ -750,7 +812,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     const size_t nF = (size_t)n_count;
     if ((rc = V->need(B::CNT, (nF + 1) * 4)) || (rc = V->need(B::FLEN, nF * 4)) ||
         (rc = V->need(B::FLENTOT, nF * 4)) || (rc = V->need(B::OFF, (nF + 1) * 8)) ||
-        (rc = V->need(B::STATUS, 4)) || (rc = V->need(B::MCNT, (nF + 1) * 4)) ||
+        (rc = V->need(B::MCNT, (nF + 1) * 4)) ||
         (rc = V->need(B::MEND, nF * 8)) || (rc = V->need(B::KEPT, (nF + 1) * 4)) ||
         (rc = V->need(B::KIDX, (nF + 1) * 8)) || (rc = V->need(B::SPOS, (nF + 1) * 8)) ||
         (rc = V->need(B::NORM, nF * 4)))
@@ -764,15 +826,25 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_cnt + nF, 1, 0u);
     D3 *d_P = multi ? V->at<D3>(B::PBUF) : nullptr;
     const unsigned gF = grid_for(nF ? nF : 1);
+    const unsigned gW = (unsigned)((nF + kVoxBlock / kWave - 1) / (kVoxBlock / kWave));   // a wavefront per streamline
+    if ((rc = V->need(B::OKEY, nF * 4)) || (rc = V->need(B::OIDX, nF * 4)) || (rc = V->need(B::OKEY2, nF * 4)) ||
+        (rc = V->need(B::ORDER, nF * 4)))
+        return rc;
+    const uint32_t *d_order = V->at<uint32_t>(B::ORDER);
     if (nF) {
+        k_vox_order_keys<<<gF, kVoxBlock, 0, s>>>(d_ptr, n_count, V->at<uint32_t>(B::OKEY), V->at<uint32_t>(B::OIDX));
+        HIP_TRY(hipGetLastError());
+        if ((rc = sort_pairs_u32(V, (const uint32_t *)V->at<uint32_t>(B::OKEY), V->at<uint32_t>(B::OKEY2),
+                                 (const uint32_t *)V->at<uint32_t>(B::OIDX), V->at<uint32_t>(B::ORDER), nF, 14, s)))
+            return rc;
         if (multi)
             k_vox_walk<false, true><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, nullptr, nullptr, nullptr,
                                                              d_cnt, V->at<float>(B::FLEN), V->at<float>(B::FLENTOT),
-                                                             d_status);
+                                                             d_status, d_order);
         else
             k_vox_walk<false, false><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, nullptr, nullptr,
                                                               nullptr, d_cnt, V->at<float>(B::FLEN),
-                                                              V->at<float>(B::FLENTOT), d_status);
+                                                              V->at<float>(B::FLENTOT), d_status, d_order);
         HIP_TRY(hipGetLastError());
     }
     if ((rc = exscan_u32(V, d_cnt, d_off, nF + 1, s))) return rc;
@@ -795,10 +867,10 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     if (nF) {
         if (multi)
             k_vox_walk<true, true><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, d_off, K1, V1, nullptr,
-                                                            nullptr, nullptr, d_status);
+                                                            nullptr, nullptr, d_status, d_order);
         else
             k_vox_walk<true, false><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, d_off, K1, V1,
-                                                             nullptr, nullptr, nullptr, d_status);
+                                                             nullptr, nullptr, nullptr, d_status, d_order);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpyAsync(&hstatus, d_status, 4, hipMemcpyDeviceToHost, s));
@@ -813,7 +885,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_mcnt + nF, 1, 0u);
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_kept + nF, 1, 0u);
     if (nF) {
-        k_vox_merge<<<gF, kVoxBlock, 0, s>>>(d_off, n_count, K2, V2, nvox, V->at<float>(B::FLEN), min_fiber_len,
+        k_vox_merge<<<gW, kVoxBlock, 0, s>>>(d_off, n_count, K2, V2, nvox, V->at<float>(B::FLEN), min_fiber_len,
                                              max_fiber_len, K1, L1, VK, d_mcnt, d_mend, d_kept);
         HIP_TRY(hipGetLastError());
     }
@@ -842,7 +914,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
         (rc = V->need(B::TDI, nvox * 4)) || (rc = V->need(B::HAS, nvox)))
         return rc;
     if (nF) {
-        k_vox_write<<<gF, kVoxBlock, 0, s>>>(
+        k_vox_write<<<gW, kVoxBlock, 0, s>>>(
             d_off, n_count, d_kept, d_kidx, d_spos, d_mcnt, K1, L1, nvox, V->at<float>(B::NORM), V->at<float>(B::FLEN),
             V->at<float>(B::FLENTOT), V->at<uint32_t>(B::IC_F), V->at<uint32_t>(B::IC_V), V->at<uint16_t>(B::IC_O),
             V->at<float>(B::IC_LEN), V->at<float>(B::TRK_NORM), V->at<float>(B::TRK_LEN), V->at<float>(B::TRK_LENTOT),
@@ -856,21 +928,28 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
                              V->at<float>(B::TDI_L), n_ic, vbits > 0 ? vbits : 1, s)))
         return rc;
     if (n_ic) {
-        k_vox_tdi<<<grid_for(n_ic), kVoxBlock, 0, s>>>(V->at<uint32_t>(B::TDI_V), V->at<float>(B::TDI_L), n_ic,
-                                                       V->at<float>(B::TDI), V->at<uint8_t>(B::HAS));
+        // run bounds per voxel into the (free) EC count and ISO flag buffers
+        if ((rc = V->need(B::ECNT, (nvox + 1) * 4)) || (rc = V->need(B::IFLAG, (nvox + 1) * 4))) return rc;
+        uint32_t *rs = V->at<uint32_t>(B::ECNT), *re = V->at<uint32_t>(B::IFLAG);
+        k_fill_u32<<<grid_for(nvox), kVoxBlock, 0, s>>>(rs, nvox, 0u);
+        k_fill_u32<<<grid_for(nvox), kVoxBlock, 0, s>>>(re, nvox, 0u);
+        k_vox_runs<<<grid_for(n_ic), kVoxBlock, 0, s>>>(V->at<uint32_t>(B::TDI_V), n_ic, rs, re);
+        const unsigned gV = (unsigned)((nvox + kVoxBlock / kWave - 1) / (kVoxBlock / kWave));
+        k_vox_tdi<<<gV, kVoxBlock, 0, s>>>(rs, re, V->at<float>(B::TDI_L), nvox, V->at<float>(B::TDI),
+                                           V->at<uint8_t>(B::HAS));
         HIP_TRY(hipGetLastError());
     }
     // EC
     size_t n_ec = 0;
     if (ptrPEAKS) {
-        if ((rc = upload(V, B::PEAKS, ptrPEAKS, nvox * 3 * (size_t)Np, s))) return rc;
+        if (!inputs_on_device && (rc = upload(V, B::PEAKS, ptrPEAKS, nvox * 3 * (size_t)Np, s))) return rc;
         if ((rc = V->need(B::ECNT, (nvox + 1) * 4)) || (rc = V->need(B::EOFF, (nvox + 1) * 8))) return rc;
         EcArgs e;
         e.X = Nx; e.Y = Ny; e.Z = Nz; e.Np = Np;
         e.vf_thr = vf_THR;
         e.fx = ECix; e.fy = ECiy; e.fz = ECiz;
         for (int i = 0; i < 9; i++) e.A[i] = ptrPeaksAffine[i];
-        e.peaks = V->at<float>(B::PEAKS);
+        e.peaks = inputs_on_device ? ptrPEAKS : V->at<float>(B::PEAKS);
         e.hash = a.hash;
         k_fill_u32<<<1, kVoxBlock, 0, s>>>(V->at<uint32_t>(B::ECNT) + nvox, 1, 0u);
         k_vox_ec<false><<<grid_for(nvox), kVoxBlock, 0, s>>>(e, nvox, V->at<uint8_t>(B::HAS), V->at<uint32_t>(B::ECNT),

@@ -262,7 +262,8 @@ class Voxeliser:
             vf_THR=0.1, flip_peaks=(False, False, False), peaks_affine=None, blur=None, blur_apply_to=None,
             stream=None, n_points=None):
         """Voxelise.  fibers/fib_ptr are numpy arrays, or torch CUDA tensors
-        (float32 [n_points, 3] and int64 [n+1]) already in device memory."""
+        (float32 [n_points, 3] and int64 [n+1]) already in device memory; then
+        mask, iso_mask and peaks are used (or copied) as device tensors too."""
         on_dev = hasattr(fibers, "data_ptr")
         if on_dev:
             n_count = int(fib_ptr.numel()) - 1
@@ -283,12 +284,15 @@ class Voxeliser:
             rho, ang, wts = np.zeros(1), np.zeros(1), np.ones(1)
         else:
             rho, ang, wts = (np.ascontiguousarray(b, np.float64) for b in blur[:3])
-        mask = None if mask is None else np.ascontiguousarray(mask, np.float32)
-        iso_mask = None if iso_mask is None else np.ascontiguousarray(iso_mask, np.float32)
+        as_f32 = (lambda a: a.float().contiguous()) if on_dev else (lambda a: np.ascontiguousarray(a, np.float32))
+        istensor = lambda a: hasattr(a, "data_ptr")
+        mask = None if mask is None else (as_f32(mask) if istensor(mask) else np.ascontiguousarray(mask, np.float32))
+        iso_mask = None if iso_mask is None else (as_f32(iso_mask) if istensor(iso_mask) else
+                                                  np.ascontiguousarray(iso_mask, np.float32))
         Np = 0
         aff = None
         if peaks is not None:
-            peaks = np.ascontiguousarray(peaks, np.float32)
+            peaks = as_f32(peaks) if istensor(peaks) else np.ascontiguousarray(peaks, np.float32)
             if peaks.ndim != 4 or peaks.shape[3] % 3:
                 raise ValueError("PEAKS dataset must have 3*k volumes")
             Np = peaks.shape[3] // 3
@@ -303,13 +307,24 @@ class Voxeliser:
                 raise ValueError(f"{name} shape {a.shape} does not match the grid {tuple(dim)}")
         if peaks is not None and tuple(peaks.shape[:3]) != tuple(dim):
             raise ValueError(f"peaks shape {peaks.shape[:3]} does not match the grid {tuple(dim)}")
+        dptr = _lib.ptr
+        if on_dev:
+            import torch
+            held = []
+
+            def dptr(a):
+                if a is None:
+                    return None
+                t = a if hasattr(a, "data_ptr") else torch.from_numpy(np.ascontiguousarray(a)).to(fibers.device)
+                held.append(t)
+                return ctypes.c_void_p(t.data_ptr())
         f = ctypes.c_float
         rc = self._lib.commit_hip_trk2dictionary(
             self._h, pf, pp, n_count, n_points, 1 if on_dev else 0, dim[0], dim[1], dim[2],
             f(pixdim[0]), f(pixdim[1]), f(pixdim[2]), f(sh[0]), f(sh[1]), f(sh[2]),
-            f(min_seg_len), f(min_fiber_len), f(max_fiber_len), _lib.ptr(peaks), Np, f(vf_THR),
+            f(min_seg_len), f(min_fiber_len), f(max_fiber_len), dptr(peaks), Np, f(vf_THR),
             -1 if flip_peaks[0] else 1, -1 if flip_peaks[1] else 1, -1 if flip_peaks[2] else 1,
-            _lib.ptr(mask), _lib.ptr(iso_mask), 1 if do_intersect else 0, _lib.ptr(aff), int(rho.size),
+            dptr(mask), dptr(iso_mask), 1 if do_intersect else 0, _lib.ptr(aff), int(rho.size),
             _lib.ptr(rho), _lib.ptr(ang), _lib.ptr(wts), _lib.ptr(apply), _lib.ptr(hash_table),
             None if stream is None else ctypes.c_void_p(stream))
         del keep

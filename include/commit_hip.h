@@ -426,15 +426,17 @@ int commit_hip_vox_destroy(commit_hip_vox *vox);
  *   fibers      float32 [n_points][3] streamline coordinates in voxmm, as
  *               read_fiberTRK / read_fiberTCK return them (:851-911), and
  *   fib_ptr     int64 [n_count+1] first point of each streamline, in place of
- *               the file name, data offset, scalars and properties; both are
- *               HOST pointers, or DEVICE pointers when inputs_on_device = 1;
+ *               the file name, data offset, scalars and properties;
+ *   inputs_on_device  0: fibers, fib_ptr, ptrPEAKS, ptrMASK and ptrISO are
+ *               HOST pointers; 1: they are DEVICE pointers;
  *   fiber_shift in voxel units (the reference scales it by the voxel size);
  *   ptrTDI, path_out, threads_count, verbose are dropped: the results stay in
  *               the handle (commit_hip_vox_sizes / _fetch);
  *   ptrBlurApplyTo is uint8 per streamline (NULL: blur all);
  *   stream      hipStream_t of the work (the call returns when it is done).
- * Masks and peaks are HOST float32 arrays in C order [Nx][Ny][Nz](...), as the
- * reference receives them; ptrPEAKS / ptrMASK / ptrISO may be NULL.
+ * Masks and peaks are float32 arrays in C order [Nx][Ny][Nz](...), as the
+ * reference receives them; ptrPEAKS / ptrMASK / ptrISO may be NULL.  The
+ * hash table, replicas and affine are HOST arrays.
  * ptrPeaksAffine: 3x3 row-major (the identity unless peaks_use_affine).
  * Returns COMMIT_HIP_EUNSUPPORTED for a streamline of >= 10000 points (the
  * reference's MAX_FIB_LEN, where it generates no dictionary), for a grid of
