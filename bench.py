@@ -639,6 +639,31 @@ def voxelize_mode(args):
         except AttributeError:
             cores = os.cpu_count() or 1
         n = max(1, min(cores, 255))
+    if args.cpu_pairs and vo.reference_available():
+        # the reference's own trk2dictionary() (built from its source into
+        # oracle/_ref) on a bounded sample: the first 250k streamlines, read
+        # from a .trk file as the reference does, its files read back
+        import shutil
+        import tempfile
+        ns = min(nF, 250_000)
+        wd = tempfile.mkdtemp(prefix="vox_ref_")
+        try:
+            r = vo.reference_trk2dictionary(wd, pts[:ptr[ns]], ptr[:ns + 1], dim, pix, H, n_threads=n, **{
+                "mask": mask, "min_fiber_len": 20.0, "max_fiber_len": 250.0})
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+        m = r["IC_v"].size
+        same = all(np.array_equal(r[k].view(np.uint8), d[k][:m].view(np.uint8))
+                   for k in ("IC_f", "IC_v", "IC_o", "IC_len")) and \
+            all(np.array_equal(r[k].view(np.uint8), d[k][:r[k].size].view(np.uint8))
+                for k in ("TRK_norm", "TRK_len", "TRK_lenTot", "TRK_kept"))
+        print(f"[bench] reference trk2dictionary(): {r['seconds']:.2f}s for {ns} streamlines on {n} threads; "
+              f"its segments equal the GPU's first {m}: {same}", file=sys.stderr, flush=True)
+        cpu = {"value": round(ns / r["seconds"], 1), "unit": "streamlines/s", "cores": n, "kind": "reference",
+               "sample": f"the first {ns} streamlines, the reference's trk2dictionary() compiled from its source "
+                         f"(g++ -O2), {n} threads, reading them from a .trk file and writing its .dict files; "
+                         f"its IC segments and streamline values bitwise equal to the GPU's: {same}"}
+    elif args.cpu_pairs:
         t0 = time.perf_counter()
         o = vo.trk2dictionary(pts, ptr, dim, pix, H, n_threads=n, mask=mask, min_fiber_len=20.0,
                               max_fiber_len=250.0)

@@ -5,16 +5,20 @@ may import this module.  It wraps ``voxelize_oracle.c``, the CPU restatement
 of the reference's tractogram voxeliser (trk2dictionary_c.cpp:101-869 for
 streamlines already in memory; see that file's header for the line map).
 
-Pinning (tests/test_oracle_voxelize.py): straight streamlines through the
-grid against their hand-computed voxel crossings, orientations and norms;
-random smooth streamlines against a dense-sampling estimate of each (voxel,
-orientation) length; the centroid mode against the polyline steps; blur
-replicas of a straight streamline against the offset lines they must follow;
-the EC peak selection, hemisphere flip, axis flips and affine against
-hand-derived orientations; the ISO and TDI definitions.
+Pinning: tests/test_oracle_voxelize_ref.py runs the reference's own
+trk2dictionary() (compiled from the reference tree into oracle/_ref by
+oracle/Makefile; ``reference_trk2dictionary`` below) on the same streamlines
+and checks every dictionary file bitwise equal; tests/test_oracle_voxelize.py
+checks straight streamlines against their hand-computed voxel crossings,
+orientations and norms, random smooth streamlines against a dense-sampling
+estimate of each voxel's length, the centroid mode against the polyline
+steps, blur replicas of a straight streamline against the offset lines they
+must follow, the EC peak rules against hand-derived orientations, and the
+ISO and TDI definitions.
 """
 import ctypes
 import os
+import time
 
 import numpy as np
 
@@ -141,4 +145,122 @@ def trk2dictionary(fibers, fib_ptr, dim, pixdim, hash_table, fiber_shift=(0.0, 0
         "status": int(r.status),
     }
     _lib().vo_free(ctypes.byref(r))
+    return out
+
+
+# ------------------------------------------------ the reference's own code --
+_REF = None
+
+
+def reference_available():
+    return os.path.exists(os.path.join(_HERE, "_ref", "libtrk2dictionary_ref.so"))
+
+
+def _ref_lib():
+    """trk2dictionary() of the reference (trk2dictionary_c.cpp:101-258),
+    compiled from the reference tree by oracle/Makefile into _ref/."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libtrk2dictionary_ref.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (make -C oracle, with the reference tree present)")
+        with open(os.path.join(_HERE, "_ref", "trk2dictionary.sym")) as f:
+            sym = f.read().split()[0]
+        lib = ctypes.CDLL(path)
+        fn = getattr(lib, sym)
+        I, F, C = ctypes.c_int, ctypes.c_float, ctypes.c_char_p
+        fn.argtypes = [C, I, I, I, I, F, F, F, I, I, I, F, F, F, F, F, F, P, I, F, I, I, I, P, P,
+                       ctypes.POINTER(ctypes.c_void_p), C, I, P, I, P, P, P, P, P, P, I, I]
+        fn.restype = I
+        _REF = fn
+    return _REF
+
+
+def _write_trk(path, fibers, fib_ptr, dim, pixdim):
+    hdr = bytearray(1000)
+    hdr[0:6] = b"TRACK\0"
+    np.frombuffer(hdr, "<i2", 3, 6)[:] = np.asarray(dim, np.int16)
+    np.frombuffer(hdr, "<f4", 3, 12)[:] = np.asarray(pixdim, np.float32)
+    np.frombuffer(hdr, "<i4", 1, 988)[0] = len(fib_ptr) - 1
+    np.frombuffer(hdr, "<i4", 1, 992)[0] = 2
+    np.frombuffer(hdr, "<i4", 1, 996)[0] = 1000
+    with open(path, "wb") as f:
+        f.write(bytes(hdr))
+        for i in range(len(fib_ptr) - 1):
+            pts = np.ascontiguousarray(fibers[fib_ptr[i]:fib_ptr[i + 1]], np.float32)
+            f.write(np.int32(pts.shape[0]).tobytes())
+            f.write(pts.astype("<f4").tobytes())
+
+
+def reference_trk2dictionary(workdir, fibers, fib_ptr, dim, pixdim, hash_table, fiber_shift=(0.0, 0.0, 0.0),
+                             min_seg_len=1e-3, min_fiber_len=0.0, max_fiber_len=250.0, do_intersect=True,
+                             mask=None, iso_mask=None, peaks=None, vf_THR=0.1, flip_peaks=(False, False, False),
+                             peaks_affine=None, blur=None, blur_apply_to=None, n_threads=1):
+    """Run the reference's own trk2dictionary() on these streamlines (written
+    to a .trk file in workdir) and read back what it writes, concatenated and
+    renumbered as trk2dictionary.pyx:551-612 does.  Returns the same keys as
+    trk2dictionary() above, except 'tdi': 'has_ic' marks the voxels its
+    per-thread TDI buffers saw (the presence test of :291-302 and :420-428)."""
+    os.makedirs(workdir, exist_ok=True)
+    fibers = np.ascontiguousarray(fibers, np.float32).reshape(-1, 3)
+    fib_ptr = np.ascontiguousarray(fib_ptr, np.int64)
+    n = fib_ptr.size - 1
+    trk = os.path.join(workdir, "t.trk")
+    _write_trk(trk, fibers, fib_ptr, dim, pixdim)
+    temp = os.path.join(workdir, "temp")
+    os.makedirs(temp, exist_ok=True)
+    nvox = int(np.prod([int(d) for d in dim]))
+    nt = max(1, min(int(n_threads), n if n > 0 else 1))
+    tdis = [np.zeros(nvox, np.float64) for _ in range(nt)]
+    tptr = (ctypes.c_void_p * nt)(*[t.ctypes.data for t in tdis])
+    if blur is None:
+        rho, ang, wts = np.zeros(1), np.zeros(1), np.ones(1)
+    else:
+        rho, ang, wts = (np.ascontiguousarray(b, np.float64) for b in blur)
+    apply = np.ones(max(n, 1), np.uint8) if blur_apply_to is None else \
+        np.ascontiguousarray(blur_apply_to, np.uint8)
+    mask = None if mask is None else np.ascontiguousarray(mask, np.float32)
+    iso_mask = None if iso_mask is None else np.ascontiguousarray(iso_mask, np.float32)
+    Np = 0
+    aff = None
+    if peaks is not None:
+        peaks = np.ascontiguousarray(peaks, np.float32)
+        Np = peaks.shape[3] // 3
+        aff = np.ascontiguousarray(np.eye(3) if peaks_affine is None else peaks_affine, np.float64)
+    sh = np.broadcast_to(np.asarray(fiber_shift, np.float32), (3,))
+    hash_table = np.ascontiguousarray(hash_table, np.int16)
+    t0 = time.perf_counter()
+    ret = _ref_lib()(trk.encode(), 1000, int(dim[0]), int(dim[1]), int(dim[2]), float(pixdim[0]), float(pixdim[1]),
+                     float(pixdim[2]), n, 0, 0, float(sh[0]), float(sh[1]), float(sh[2]), float(min_seg_len),
+                     float(min_fiber_len), float(max_fiber_len), _ptr(peaks), Np, float(vf_THR),
+                     -1 if flip_peaks[0] else 1, -1 if flip_peaks[1] else 1, -1 if flip_peaks[2] else 1,
+                     _ptr(mask), _ptr(iso_mask), tptr, temp.encode(), 1 if do_intersect else 0, _ptr(aff),
+                     int(rho.size), _ptr(rho), _ptr(ang), _ptr(wts), _ptr(apply), None, _ptr(hash_table), nt, 0)
+    seconds = time.perf_counter() - t0
+    if ret != 1:
+        raise RuntimeError("the reference's trk2dictionary() generated no dictionary")
+    rd = lambda name, dt: np.fromfile(name, dtype=dt)
+    out = {}
+    parts = {k: [] for k in ("IC_f", "IC_v", "IC_o", "IC_len", "TRK_norm", "TRK_len", "TRK_lenTot", "TRK_kept")}
+    dts = {"IC_f": "<u4", "IC_v": "<u4", "IC_o": "<u2", "IC_len": "<f4", "TRK_norm": "<f4", "TRK_len": "<f4",
+           "TRK_lenTot": "<f4", "TRK_kept": "u1"}
+    discarded = 0
+    for j in range(nt):
+        for k in parts:
+            a = rd(os.path.join(temp, f"dictionary_{k}_{j}.dict"), dts[k])
+            if k == "IC_f" and j > 0:
+                a = (a.astype(np.int64) - discarded).astype(np.uint32)
+            parts[k].append(a)
+        discarded += int(np.count_nonzero(parts["TRK_kept"][j] == 0))
+    for k in parts:
+        out[k] = np.concatenate(parts[k]).astype(np.dtype(dts[k]).newbyteorder("="))
+    out["EC_v"] = rd(os.path.join(workdir, "dictionary_EC_v.dict"), "<u4").astype(np.uint32)
+    out["EC_o"] = rd(os.path.join(workdir, "dictionary_EC_o.dict"), "<u2").astype(np.uint16)
+    out["ISO_v"] = rd(os.path.join(workdir, "dictionary_ISO_v.dict"), "<u4").astype(np.uint32)
+    X, Y, Z = (int(d) for d in dim)
+    has = np.zeros((X, Y, Z), bool)
+    for t in tdis:
+        has |= t.reshape(X, Y, Z) != 0          # C order [x][y][z] (:540)
+    out["has_ic"] = has.ravel(order="F")          # x fastest, as the linear voxel ids
+    out["seconds"] = seconds                      # the trk2dictionary() call alone
     return out
