@@ -421,23 +421,27 @@ __global__ __launch_bounds__(kBlock) void k_res_problems(const double *__restric
 // entries for each IC RF, then its EC peaks for each EC RF, then its ISO
 // entries for each ISO RF.  They are taken as one list, 16 per M-tile, so the
 // few EC and ISO rows fill the free rows of the voxel's last IC tile instead
-// of a tile each.  The 16 LUT rows of a tile (4NT 16-byte chunks each) go
-// straight from memory into a wavefront-private LDS tile (global_load_lds_dwordx4:
-// lane l of a load writes 16 bytes at base + 16 l), chunk-major: slot cc*16 + r
-// holds chunk cc of row r, so each load covers 64 contiguous bytes of 16 rows
-// and the 16 lanes of one operand read hit 16 consecutive slots (no bank
-// conflicts).  A tile's operands are read into registers, the next tile is
-// requested into the same LDS, and the matrix-core work of the current tile
-// runs under that load.  The row records (LUT row, output slot, scale) come
-// from registers: each lane holds the records of one row of a 64-row window,
-// the next window loads four tiles ahead, and a tile's row pointers and slots
-// are shuffled out of the window, so issuing the next tile's copy waits on no
-// record load.
-// yb: this lane's B operand, Y_v[g*4NT + t][rhs] for t < 4NT.
+// of a tile each.  The K order interleaves the lane groups: MFMA step 4u + j
+// takes sample 16u + 4g + j from lane group g, so a lane's A operand for
+// steps 4u..4u+3 is chunk 4u + g (4 floats) of its tile row q, and one
+// 16-byte load per u, issued by all 64 lanes, covers 64 contiguous bytes of
+// each of the 16 rows.  The A operand goes straight from memory into
+// registers: as each chunk's 4 MFMAs are issued, the same registers are
+// reloaded with the next tile's chunk, so every load has a whole tile of
+// matrix-core work to arrive in.  (Before, tiles were staged through a
+// wavefront-private LDS tile that could hold one tile only: a tile waited for
+// its rows, requested when the previous tile's MFMAs began.)  The row records
+// (LUT row, output slot, scale) come from registers: each lane holds the
+// records of one row of a 64-row window, the next window loads four tiles
+// ahead, and a tile's row pointers and slots are shuffled out of the window.
+// yb: this lane's B operand, Y_v[16u + 4g + j][rhs] at yb[4u + j].
 //   IC row (RF i, entry k) : z[(i*n + k)*16 + q]                        = len * dot
 //   EC row (RF i, peak k)  : X[(nIC*nF + i*nE + col)*R + rhs]            = dot
 //   ISO row (RF i, entry k): X[(nIC*nF + nEC*nE + i*nI + col)*R + rhs]   = dot
 
+
+// Sample of K step t (< 4NT) of lane group g in A^T.
+__device__ __forceinline__ int at_sample(int t, int g) { return 16 * (t >> 2) + 4 * g + (t & 3); }
 
 // Output slot and scale of the 4 D rows (e0+4g+k) a tile produces for this lane.
 struct Aux {
@@ -567,18 +571,15 @@ __device__ __forceinline__ void rec_win2(const DevDict &d, const RecA &A0, const
     W1 = rec_win_b(d, A1, o1);
 }
 
-// Request the LUT rows of tile e0 (rows past the end repeat the last row);
-// W holds the records of rows [wb, wb + 64) and e0 lies in that window.
-template <int NT>
-__device__ __forceinline__ void at_request_w(const DevDict &d, const VRows &vr, uint32_t e0, const RecWin &W,
-                                             uint32_t wb, int lane, float *T)
+// This lane's A operand base in tile e0: row q = lane % 16 of the tile (rows
+// past the end repeat the last row), chunk g = lane / 16; W holds the records
+// of rows [wb, wb + 64) and e0 lies in that window.
+__device__ __forceinline__ const float *at_row(const DevDict &d, const VRows &vr, uint32_t e0, const RecWin &W,
+                                               uint32_t wb, int lane)
 {
-    const uint32_t r = min(e0 + (uint32_t)(lane & 15), vr.total - 1);   // row lane%16 of the tile
+    const uint32_t r = min(e0 + (uint32_t)(lane & 15), vr.total - 1);
     const uint32_t c = (uint32_t)__shfl((int)W.code, (int)(r - wb));
-    const float *src = row_code(d, c, 0) + 4 * (lane >> 4);
-#pragma unroll
-    for (int u = 0; u < NT; u++)                                  // chunks 4u .. 4u+3 of every row
-        __builtin_amdgcn_global_load_lds(src + 16 * u, (lds_void *)(T + 256 * u), 16, 0, 0);
+    return row_code(d, c, 0) + 4 * (lane >> 4);
 }
 
 __device__ __forceinline__ Aux at_aux_w(const VRows &vr, uint32_t e0, int g, const RecWin &W, uint32_t wb)
@@ -595,52 +596,52 @@ __device__ __forceinline__ Aux at_aux_w(const VRows &vr, uint32_t e0, int g, con
 
 template <int NT>
 __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d, const VRows &vr, int lane,
-                                          float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok, float *T,
+                                          float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok,
                                           RecWin W0, RecWin W1)
 {
     const int g = lane >> 4, q = lane & 15;
     if (vr.total == 0) return;
     // record windows: W0 rows [wb, wb+64), W1 the next 64 (loaded a window ahead)
     uint32_t wb = 0;
-    at_request_w<NT>(d, vr, 0, W0, wb, lane, T);
-    Aux aux = at_aux_w(vr, 0, g, W0, wb), auxp = aux;
-    f4 pend = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    uint32_t pe0 = vr.total;                             // total: nothing pending
-    for (uint32_t e0 = 0; e0 < vr.total; e0 += 16) {
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");     // tile e0 is in LDS
-        __builtin_amdgcn_wave_barrier();
-        float4 a[NT];
+    float4 a[NT];
+    // the B operand (and its weights) are final before the first tile's loads
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const float *src = at_row(d, vr, 0, W0, wb, lane);
 #pragma unroll
-        for (int tb = 0; tb < NT; tb++)                      // chunk g*NT + tb of row q
-            a[tb] = *reinterpret_cast<const float4 *>(T + 4 * ((g * NT + tb) * 16 + q));
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // operands in registers
-        __builtin_amdgcn_wave_barrier();
-        if (pe0 < vr.total) at_store(pend, auxp, pe0, vr.total, g, q, z, Xr, R, rhs_ok);
-        Aux auxn = aux;
+        for (int u = 0; u < NT; u++) a[u] = *reinterpret_cast<const float4 *>(src + 16 * u);
+    }
+    Aux aux = at_aux_w(vr, 0, g, W0, wb);
+    for (uint32_t e0 = 0; e0 < vr.total; e0 += 16) {
         const uint32_t e1 = e0 + 16;
+        // the next tile's rows; after the last tile the reloads read row 0 of
+        // the IC table (a valid, cached address) and are never used
+        const float *nsrc = d.wmr + 4 * g;
+        Aux auxn = aux;
         if (e1 < vr.total) {
             if (e1 == wb + kWave) {                          // next tile opens the next window
                 W0 = W1;
                 wb += kWave;
                 W1 = rec_win(d, vr, wb + kWave, lane);
             }
-            at_request_w<NT>(d, vr, e1, W0, wb, lane, T);
+            nsrc = at_row(d, vr, e1, W0, wb, lane);
             auxn = at_aux_w(vr, e1, g, W0, wb);
         }
         f4 acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
 #pragma unroll
-        for (int tb = 0; tb < NT; tb++) {
-            acc0 = mfma4(a[tb].x, yb[4 * tb + 0], acc0);
-            acc1 = mfma4(a[tb].y, yb[4 * tb + 1], acc1);
-            acc0 = mfma4(a[tb].z, yb[4 * tb + 2], acc0);
-            acc1 = mfma4(a[tb].w, yb[4 * tb + 3], acc1);
+        for (int u = 0; u < NT; u++) {
+            acc0 = mfma4(a[u].x, yb[4 * u + 0], acc0);
+            acc1 = mfma4(a[u].y, yb[4 * u + 1], acc1);
+            acc0 = mfma4(a[u].z, yb[4 * u + 2], acc0);
+            acc1 = mfma4(a[u].w, yb[4 * u + 3], acc1);
+            a[u] = *reinterpret_cast<const float4 *>(nsrc + 16 * u);
+            // keep each reload behind its chunk's MFMAs: hoisted loads would
+            // need a second register set for the A operand
+            __builtin_amdgcn_sched_barrier(0);
         }
-        pend = acc0 + acc1;
-        pe0 = e0;
-        auxp = aux;
+        at_store(acc0 + acc1, aux, e0, vr.total, g, q, z, Xr, R, rhs_ok);
         aux = auxn;
     }
-    if (pe0 < vr.total) at_store(pend, auxp, pe0, vr.total, g, q, z, Xr, R, rhs_ok);
 }
 
 // The momentum point's residual of the batched fits, formed while A^T loads
@@ -659,8 +660,10 @@ struct MomArgs {
 // (sel, vseg and col also go to the kernel as separate restrict arguments, so
 // that the wave-uniform loads of sel and vseg[v] can be scalar loads)
 
+// Two wavefronts per SIMD: the register budget (256) holds the B operand,
+// the A operand of one tile and the records, with no LDS.
 template <int NT, bool MOM>
-__global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__restrict__ Y,
+__global__ __launch_bounds__(kBlock, 2) void k_At_mm(DevDict d, const float *__restrict__ Y,
                                                   const float *__restrict__ sqrtw, float *__restrict__ z,
                                                   float *__restrict__ X, int R, int rg,
                                                   const uint32_t *__restrict__ uptr,
@@ -669,46 +672,49 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
                                                   const int32_t *__restrict__ vseg,
                                                   const commit_hip_fista_col *__restrict__ col)
 {
-    __shared__ __align__(16) float tiles[kWavesPerBlock][256 * NT];   // 16 rows x 16NT floats per wave
+    // MOM: r(prev_x) of the voxel and its sqrtw row, staged per wavefront
     constexpr int kW = (16 * NT + kWave - 1) / kWave * kWave;             // whole 64-lane copies of nS <= 16 NT
-    __shared__ float wst[MOM ? kWavesPerBlock : 1][MOM ? kW : 1];          // MOM: the voxel's sqrtw row
+    __shared__ __align__(16) float tiles[MOM ? kWavesPerBlock : 1][MOM ? 256 * NT : 1];
+    __shared__ float wst[MOM ? kWavesPerBlock : 1][MOM ? kW : 1];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int v = wave_index(d.xcd_remap);
     if (v >= d.nVOX) return;
-    float *T = tiles[w];
     const int g = lane >> 4, q = lane & 15;
     const int rhs = rg * kTileR + q;
     bool rhs_ok = rhs < R;
-    // B operand: samples past nS and RHS columns past R read sample 0 / column
-    // 0 of this voxel and are zeroed, so the loads are unconditional and in
-    // flight together.  The row records follow (both phases of both windows
-    // together).  Requesting the records with the operand measured no faster
-    // in the plain form and 3% slower in the momentum form.
+    // B operand: samples past nS and RHS columns past R read the voxel's last
+    // sample / column 0 and are zeroed, so the loads are unconditional and in
+    // flight together.  For a fixed K step the 64 lanes read 4 samples x 16
+    // consecutive columns.  The row records follow (both phases of both
+    // windows together).
     float yb[4 * NT];
     const size_t row0 = (size_t)v * d.nS;
     const int rc = rhs_ok ? rhs : 0;
+    // at_sample(t, g) < nS  <=>  16 (t >> 2) + (t & 3) < L: one register, not
+    // one sample index per K step
+    const int L = d.nS - 4 * g;
     const VRows vr = vrows_at(d, v, uptr, uhead);
     RecWin W0, W1;
     if constexpr (!MOM) {
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++) {
-            const int s = g * 4 * NT + t;
-            yb[t] = Y[(row0 + (s < d.nS ? s : 0)) * R + rc];
+            const int s = at_sample(t, g);
+            yb[t] = Y[(row0 + (s < d.nS ? s : d.nS - 1)) * R + rc];
         }
         if (sqrtw) {
             float wv[4 * NT];
 #pragma unroll
             for (int t = 0; t < 4 * NT; t++) {
-                const int s = g * 4 * NT + t;
-                wv[t] = sqrtw[row0 + (s < d.nS ? s : 0)];
+                const int s = at_sample(t, g);
+                wv[t] = sqrtw[row0 + (s < d.nS ? s : d.nS - 1)];
             }
 #pragma unroll
             for (int t = 0; t < 4 * NT; t++) yb[t] *= wv[t];
         }
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++)
-            if (!(g * 4 * NT + t < d.nS && rhs_ok)) yb[t] = 0.0f;
+            if (!(16 * (t >> 2) + (t & 3) < L && rhs_ok)) yb[t] = 0.0f;
         const RecA A0 = rec_win_a(d, vr, 0, lane), A1 = rec_win_a(d, vr, kWave, lane);
         rec_win2(d, A0, A1, W0, W1);
     } else {
@@ -717,12 +723,12 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
         const commit_hip_fista_col &c = col[(size_t)vseg[v] * R + rc];
         const int adv = c.advance;                                 // requested before the rows
         const double coef = c.coef;
-        // r(prev_x) of the voxel (nS * R <= 256 NT floats) goes to this
-        // wavefront's LDS tile, which is free until the first LUT tile, and
-        // its sqrtw row to a small LDS row; r(x) goes to registers.
-        const int cnt = d.nS * R, gs = g * 4 * NT;
+        // r(prev_x) of the voxel (nS * R <= 256 NT floats) and its sqrtw row
+        // go to this wavefront's LDS by DMA (no registers held); r(x) goes to
+        // registers.
+        const int cnt = d.nS * R;
         const float *src = rp + row0 * R;
-        float *Wl = wst[w];
+        float *T = tiles[w], *Wl = wst[w];
         if ((cnt & 3) == 0) {          // 16-byte copies (the rows start 16-byte aligned): 2% faster than 4-byte
             for (int j = 0; j * 4 * kWave < cnt; j++)
                 __builtin_amdgcn_global_load_lds(src + min(j * 4 * kWave + 4 * lane, cnt - 4),
@@ -736,13 +742,13 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
             for (int j = 0; j * kWave < d.nS; j++)
                 __builtin_amdgcn_global_load_lds(sqrtw + row0 + min(j * kWave + lane, d.nS - 1),
                                                  (lds_void *)(Wl + kWave * j), 4, 0, 0);
-        // Samples gs + t >= nS (t >= lim, last lane group only) read clamped
-        // rows and stale LDS and are zeroed below.  The later stages index
-        // with t and lim only, so no per-sample index outlives the loads.
-        const int lim = d.nS - gs;
-        const float *px = rx + row0 * R + rc + (size_t)gs * R;
+        // Samples past nS (last lane group only) read the last sample's row
+        // and stale LDS and are zeroed below.
 #pragma unroll
-        for (int t = 0; t < 4 * NT; t++) yb[t] = px[(size_t)min(t, lim - 1) * R];
+        for (int t = 0; t < 4 * NT; t++) {
+            const int s = at_sample(t, g);
+            yb[t] = rx[(row0 + (s < d.nS ? s : d.nS - 1)) * R + rc];
+        }
         // r(prev_x) and sqrtw are in LDS.  The builtin (vmcnt 0, the other
         // counters unconstrained), unlike inline asm, also tells the
         // compiler's wait insertion that every load so far is done.
@@ -751,17 +757,17 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
         __builtin_amdgcn_wave_barrier();
         rhs_ok = rhs_ok && adv != 0;
         const double a = 1.0 + coef, b = -coef;
-        const float *Tq = T + gs * R + rc;                         // (gs + t) * R + rc < 256 NT
-        const float *Wg = Wl + gs;                                 // gs + t < 16 NT <= kW
+        const float *Tq = T + rc;                                  // sample s: Tq[s * R] (s * R + rc < 256 NT)
         double ss = 0.0;
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++) {
-            const double u = a * (double)yb[t] + b * (double)Tq[t * R];   // as commit_hip_residual_combine
-            const bool in = t < lim;
+            const bool in = 16 * (t >> 2) + (t & 3) < L;
+            const int s = in ? at_sample(t, g) : 0;
+            const double u = a * (double)yb[t] + b * (double)Tq[s * R];   // as commit_hip_residual_combine
             ss = fma(in ? u : 0.0, u, ss);
-            yb[t] = in && rhs_ok ? (sqrtw ? (float)u * Wg[t] : (float)u) : 0.0f;
+            yb[t] = in && rhs_ok ? (sqrtw ? (float)u * Wl[s] : (float)u) : 0.0f;
         }
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");        // T is read before the first tile lands
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __builtin_amdgcn_wave_barrier();
         ss = swap32_add(ss, ss);           // lanes q, q+16, q+32, q+48 hold column q
         ss = swap16_add(ss, ss);
@@ -769,7 +775,7 @@ __global__ __launch_bounds__(kBlock) void k_At_mm(DevDict d, const float *__rest
         const RecA A0 = rec_win_a(d, vr, 0, lane), A1 = rec_win_a(d, vr, kWave, lane);
         rec_win2(d, A0, A1, W0, W1);
     }
-    mm_vox_At<NT>(yb, d, vr, lane, z, X + rhs, R, rhs_ok, T, W0, W1);
+    mm_vox_At<NT>(yb, d, vr, lane, z, X + rhs, R, rhs_ok, W0, W1);
 }
 
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
