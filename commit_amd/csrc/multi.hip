@@ -443,28 +443,6 @@ __global__ __launch_bounds__(kBlock) void k_res_problems(const double *__restric
 // Sample of K step t (< 4NT) of lane group g in A^T.
 __device__ __forceinline__ int at_sample(int t, int g) { return 16 * (t >> 2) + 4 * g + (t & 3); }
 
-// Output slot and scale of the 4 D rows (e0+4g+k) a tile produces for this lane.
-struct Aux {
-    uint32_t slot[4];
-    float scale[4];
-};
-
-// Write the D tile of rows e0.. (deferred by one tile, so that the wait for
-// the next LUT tile does not also wait for these stores).
-__device__ __forceinline__ void at_store(const f4 &acc, const Aux &a, uint32_t e0, uint32_t total, int g, int q,
-                                         float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok)
-{
-#pragma unroll
-    for (int k = 0; k < 4; k++) {                        // D row 4g+k = row e0+4g+k, column q
-        if (e0 + 4 * g + k < total) {
-            if (a.scale[k] >= 0.0f)
-                z[(size_t)a.slot[k] * kTileR + q] = acc[k];                 // (setup_operator.py:412-428)
-            else if (rhs_ok)
-                Xr[(size_t)a.slot[k] * R] = acc[k];                         // (setup_operator.py:444-563)
-        }
-    }
-}
-
 // Record of row r (< vr.total) of the voxel's A^T list: LUT row code (table
 // in the top 2 bits: 0 IC, 1 EC, 2 ISO), output slot and a sign (IC: slot
 // i*n + h of z for head entry h, 1; EC/ISO: x column, -1).  It is read in two
@@ -526,107 +504,36 @@ __device__ __forceinline__ void at_rec_b(const DevDict &d, const RecA &a, uint32
     scale = a.kind == 0 ? 1.0f : -1.0f;
 }
 
-// Records of 64 consecutive rows, one per lane.
-struct RecWin {
-    uint32_t code, slot;
-    float scale;
-};
+// One row record of the voxel's A^T list (see commit_hip_plan::mm_rec).
+__device__ __forceinline__ uint64_t rec_pack(uint32_t code, uint32_t slot) { return (uint64_t)code | ((uint64_t)slot << 32); }
 
-__device__ __forceinline__ RecA rec_win_a(const DevDict &d, const VRows &vr, uint32_t b, int lane)
-{
-    RecA a{0u, 0u, 0u, 0u, 0u, 3};
-    if (b < vr.total) a = at_rec_a(d, vr, b + (uint32_t)lane);
-    return a;
-}
-
-__device__ __forceinline__ RecWin rec_win_b(const DevDict &d, const RecA &a, uint32_t o)
-{
-    RecWin w{0u, 0u, 0.0f};
-    if (a.kind != 3) at_rec_b(d, a, o, w.code, w.slot, w.scale);
-    return w;
-}
-
-__device__ __forceinline__ RecWin rec_win(const DevDict &d, const VRows &vr, uint32_t b, int lane)
-{
-    const RecA a = rec_win_a(d, vr, b, lane);
-    return rec_win_b(d, a, rec_o(d, a));
-}
-
-// The first two windows of a voxel, each phase's loads of both issued together.
-__device__ __forceinline__ void rec_o2(const DevDict &d, const RecA &A0, const RecA &A1, uint32_t &o0, uint32_t &o1)
-{
-    o0 = o1 = 0u;
-    if (d.n > 0 && d.nIC > 0) {                    // one block: both addresses, then both loads
-        const uint32_t k0 = A0.kind == 0 ? A0.h : 0u, k1 = A1.kind == 0 ? A1.h : 0u;
-        o0 = d.ic.o[k0];
-        o1 = d.ic.o[k1];
-    }
-}
-
-__device__ __forceinline__ void rec_win2(const DevDict &d, const RecA &A0, const RecA &A1, RecWin &W0, RecWin &W1)
-{
-    uint32_t o0, o1;
-    rec_o2(d, A0, A1, o0, o1);
-    W0 = rec_win_b(d, A0, o0);
-    W1 = rec_win_b(d, A1, o1);
-}
-
-// This lane's A operand base in tile e0: row q = lane % 16 of the tile (rows
-// past the end repeat the last row), chunk g = lane / 16; W holds the records
-// of rows [wb, wb + 64) and e0 lies in that window.
-__device__ __forceinline__ const float *at_row(const DevDict &d, const VRows &vr, uint32_t e0, const RecWin &W,
-                                               uint32_t wb, int lane)
-{
-    const uint32_t r = min(e0 + (uint32_t)(lane & 15), vr.total - 1);
-    const uint32_t c = (uint32_t)__shfl((int)W.code, (int)(r - wb));
-    return row_code(d, c, 0) + 4 * (lane >> 4);
-}
-
-__device__ __forceinline__ Aux at_aux_w(const VRows &vr, uint32_t e0, int g, const RecWin &W, uint32_t wb)
-{
-    Aux a;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint32_t r = min(e0 + 4 * g + k, vr.total - 1);
-        a.slot[k] = (uint32_t)__shfl((int)W.slot, (int)(r - wb));
-        a.scale[k] = __shfl(W.scale, (int)(r - wb));
-    }
-    return a;
-}
-
+// A^T of one voxel over its rows [rb, rb + total) of the flat record list.
+// Lane (g, q) takes row e0 + q of tile e0 for its A operand (chunk g of each
+// 64-byte group) and D rows e0 + 4g + k for its outputs.  The records of the
+// next two tiles are always in flight (one independent 8-byte load per lane
+// per tile, clamped to the last row), so no load inside the tile loop waits
+// on another.
 template <int NT>
-__device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d, const VRows &vr, int lane,
-                                          float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok,
-                                          RecWin W0, RecWin W1)
+__device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDict &d,
+                                          const uint64_t *__restrict__ rec, uint32_t rb, uint32_t total, int lane,
+                                          float *__restrict__ z, float *__restrict__ Xr, int R, bool rhs_ok)
 {
     const int g = lane >> 4, q = lane & 15;
-    if (vr.total == 0) return;
-    // record windows: W0 rows [wb, wb+64), W1 the next 64 (loaded a window ahead)
-    uint32_t wb = 0;
+    if (total == 0) return;
+    const uint64_t *rr = rec + rb;
+    const uint32_t last = total - 1;
+    uint64_t r0 = rr[min((uint32_t)q, last)], r1 = rr[min(16u + q, last)];
     float4 a[NT];
-    // the B operand (and its weights) are final before the first tile's loads
-    __builtin_amdgcn_sched_barrier(0);
     {
-        const float *src = at_row(d, vr, 0, W0, wb, lane);
+        const float *src = row_code(d, (uint32_t)r0, 0) + 4 * g;
 #pragma unroll
         for (int u = 0; u < NT; u++) a[u] = *reinterpret_cast<const float4 *>(src + 16 * u);
     }
-    Aux aux = at_aux_w(vr, 0, g, W0, wb);
-    for (uint32_t e0 = 0; e0 < vr.total; e0 += 16) {
-        const uint32_t e1 = e0 + 16;
+    for (uint32_t e0 = 0; e0 < total; e0 += 16) {
+        const uint64_t r2 = rr[min(e0 + 32u + q, last)];              // tile e0 + 32's records
         // the next tile's rows; after the last tile the reloads read row 0 of
         // the IC table (a valid, cached address) and are never used
-        const float *nsrc = d.wmr + 4 * g;
-        Aux auxn = aux;
-        if (e1 < vr.total) {
-            if (e1 == wb + kWave) {                          // next tile opens the next window
-                W0 = W1;
-                wb += kWave;
-                W1 = rec_win(d, vr, wb + kWave, lane);
-            }
-            nsrc = at_row(d, vr, e1, W0, wb, lane);
-            auxn = at_aux_w(vr, e1, g, W0, wb);
-        }
+        const float *nsrc = e0 + 16 < total ? row_code(d, (uint32_t)r1, 0) + 4 * g : d.wmr + 4 * g;
         f4 acc0 = f4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
 #pragma unroll
         for (int u = 0; u < NT; u++) {
@@ -639,8 +546,20 @@ __device__ __forceinline__ void mm_vox_At(const float (&yb)[4 * NT], const DevDi
             // need a second register set for the A operand
             __builtin_amdgcn_sched_barrier(0);
         }
-        at_store(acc0 + acc1, aux, e0, vr.total, g, q, z, Xr, R, rhs_ok);
-        aux = auxn;
+        const f4 acc = acc0 + acc1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {                    // D row 4g+k = row e0+4g+k, column q
+            const uint64_t rk = (uint64_t)__shfl((long long)r0, 4 * g + k);
+            const uint32_t slot = (uint32_t)(rk >> 32);
+            if (e0 + 4 * g + k < total) {
+                if ((uint32_t)rk >> 30 == 0u)
+                    z[(size_t)slot * kTileR + q] = acc[k];                  // (setup_operator.py:412-428)
+                else if (rhs_ok)
+                    Xr[(size_t)slot * R] = acc[k];                          // (setup_operator.py:444-563)
+            }
+        }
+        r0 = r1;
+        r1 = r2;
     }
 }
 
@@ -666,8 +585,8 @@ template <int NT, bool MOM>
 __global__ __launch_bounds__(kBlock, 2) void k_At_mm(DevDict d, const float *__restrict__ Y,
                                                   const float *__restrict__ sqrtw, float *__restrict__ z,
                                                   float *__restrict__ X, int R, int rg,
-                                                  const uint32_t *__restrict__ uptr,
-                                                  const uint32_t *__restrict__ uhead, MomArgs ma,
+                                                  const uint32_t *__restrict__ rptr,
+                                                  const uint64_t *__restrict__ rec, MomArgs ma,
                                                   const int32_t *__restrict__ sel,
                                                   const int32_t *__restrict__ vseg,
                                                   const commit_hip_fista_col *__restrict__ col)
@@ -694,8 +613,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_At_mm(DevDict d, const float *__r
     // at_sample(t, g) < nS  <=>  16 (t >> 2) + (t & 3) < L: one register, not
     // one sample index per K step
     const int L = d.nS - 4 * g;
-    const VRows vr = vrows_at(d, v, uptr, uhead);
-    RecWin W0, W1;
+    const uint32_t rb = rptr[v], total = rptr[v + 1] - rb;
     if constexpr (!MOM) {
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++) {
@@ -715,8 +633,6 @@ __global__ __launch_bounds__(kBlock, 2) void k_At_mm(DevDict d, const float *__r
 #pragma unroll
         for (int t = 0; t < 4 * NT; t++)
             if (!(16 * (t >> 2) + (t & 3) < L && rhs_ok)) yb[t] = 0.0f;
-        const RecA A0 = rec_win_a(d, vr, 0, lane), A1 = rec_win_a(d, vr, kWave, lane);
-        rec_win2(d, A0, A1, W0, W1);
     } else {
         const int sl = *sel;
         const float *rx = sl ? ma.r0 : ma.r1, *rp = sl ? ma.r1 : ma.r0;
@@ -772,10 +688,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_At_mm(DevDict d, const float *__r
         ss = swap32_add(ss, ss);           // lanes q, q+16, q+32, q+48 hold column q
         ss = swap16_add(ss, ss);
         if (g == 0) ma.part[(size_t)v * kTileR + q] = rhs_ok ? ss : 0.0;
-        const RecA A0 = rec_win_a(d, vr, 0, lane), A1 = rec_win_a(d, vr, kWave, lane);
-        rec_win2(d, A0, A1, W0, W1);
     }
-    mm_vox_At<NT>(yb, d, vr, lane, z, X + rhs, R, rhs_ok, W0, W1);
+    mm_vox_At<NT>(yb, d, rec, rb, total, lane, z, X + rhs, R, rhs_ok);
 }
 
 // Pass 2: X[(f + i nF)*R + rhs] = eval * sum over the fibre's entries (ascending
@@ -859,6 +773,77 @@ __global__ __launch_bounds__(kBlock) void k_mm_fibre_order(const uint32_t *__res
         hpos[k] = headof[e];
         lenF[k] = len[e];
     }
+}
+
+// Rows of each voxel's A^T list: IC heads for each IC RF, EC peaks for each
+// EC RF, ISO entries for each ISO RF (setup_operator.py:370-563).
+__global__ __launch_bounds__(kBlock) void k_mm_rows_count(DevDict d, const uint32_t *__restrict__ uptr,
+                                                          uint32_t *__restrict__ cnt)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= d.nVOX) return;
+    const VRows vr = uptr ? vrows_at(d, v, uptr, nullptr) : vrows(d, v);
+    cnt[v] = vr.total;
+}
+
+// Each voxel's records (LUT row code | output slot << 32), in list order.
+__global__ __launch_bounds__(kBlock) void k_mm_rows_fill(DevDict d, const uint32_t *__restrict__ uptr,
+                                                         const uint32_t *__restrict__ uhead,
+                                                         const uint32_t *__restrict__ rptr,
+                                                         uint64_t *__restrict__ rec)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= d.nVOX) return;
+    const VRows vr = uptr ? vrows_at(d, v, uptr, uhead) : vrows(d, v);
+    uint64_t *out = rec + rptr[v];
+    for (uint32_t r = 0; r < vr.total; r++) {
+        const RecA a = at_rec_a(d, vr, r);
+        uint32_t code, slot;
+        float scale;
+        at_rec_b(d, a, rec_o(d, a), code, slot, scale);
+        out[r] = rec_pack(code, slot);
+    }
+}
+
+int build_mm_rows(commit_hip_plan *p)
+{
+    const DevDict &d = p->d;
+    const int nV = d.nVOX;
+    void *a = nullptr, *b = nullptr;
+    uint32_t *cnt = nullptr;
+    int rc;
+    if ((rc = commit_internal::dev_alloc(p, (size_t)(nV + 1) * sizeof(uint32_t), &a))) return rc;
+    if (hipMalloc((void **)&cnt, (size_t)(nV + 1) * sizeof(uint32_t)) != hipSuccess)
+        return fail(COMMIT_HIP_ENOMEM, "hipMalloc of %zu bytes failed", (size_t)(nV + 1) * sizeof(uint32_t));
+    std::vector<uint32_t> h((size_t)nV + 1, 0u);
+    const unsigned gv = (unsigned)((nV + kBlock - 1) / kBlock);
+    const uint32_t *uptr = d.n > 0 && d.nIC > 0 ? p->mm_uptr : nullptr;
+    if (gv > 0) hipLaunchKernelGGL(k_mm_rows_count, dim3(gv), dim3(kBlock), 0, nullptr, d, uptr, cnt);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpy(h.data(), cnt, (size_t)nV * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(cnt);
+    uint64_t run = 0;
+    if (err == hipSuccess) {
+        for (int v = 0; v < nV; v++) {                     // exclusive scan: first row of each voxel
+            const uint32_t cv = h[v];
+            h[v] = (uint32_t)run;
+            run += cv;
+        }
+        if (run >= 0xFFFFFFFFull) return fail(COMMIT_HIP_EUNSUPPORTED, "%llu A^T rows", (unsigned long long)run);
+        h[nV] = (uint32_t)run;
+        err = hipMemcpy(a, h.data(), (size_t)(nV + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
+    if (err != hipSuccess) return fail(COMMIT_HIP_EDEVICE, "A^T rows: %s", hipGetErrorString(err));
+    if ((rc = commit_internal::dev_alloc(p, (size_t)std::max<uint64_t>(run, 1u) * sizeof(uint64_t), &b))) return rc;
+    if (gv > 0)
+        hipLaunchKernelGGL(k_mm_rows_fill, dim3(gv), dim3(kBlock), 0, nullptr, d, uptr, p->mm_uhead,
+                           static_cast<uint32_t *>(a), static_cast<uint64_t *>(b));
+    err = hipGetLastError();
+    if (err == hipSuccess) err = hipStreamSynchronize(nullptr);
+    if (err != hipSuccess) return fail(COMMIT_HIP_EDEVICE, "A^T rows: %s", hipGetErrorString(err));
+    p->mm_rptr = static_cast<uint32_t *>(a);
+    p->mm_rec = static_cast<uint64_t *>(b);             // set last: a failed build is redone by the next call
+    return 0;
 }
 
 // Builds the head lists on the null stream and waits for them; mm_uhead is set
@@ -1013,6 +998,7 @@ int launch_At_mm(commit_hip_plan *plan, const float *y, const float *w, float *x
         plan->zmulti = static_cast<float *>(zp);
     }
     if (!plan->mm_uhead && d.n > 0 && d.nIC > 0 && (rc = build_mm_heads(plan))) return rc;
+    if (!plan->mm_rec && (rc = build_mm_rows(plan))) return rc;
     const int nrg = (nrhs + kTileR - 1) / kTileR;
     const unsigned grid = (unsigned)((d.nVOX + kWavesPerBlock - 1) / kWavesPerBlock);
     const size_t total = (size_t)d.nF * d.nIC * kTileR;
@@ -1023,7 +1009,7 @@ int launch_At_mm(commit_hip_plan *plan, const float *y, const float *w, float *x
 #define CASE(NT)                                                                                            \
     case NT:                                                                                                \
         hipLaunchKernelGGL((k_At_mm<NT, MOM>), dim3(grid), dim3(kBlock), 0, st, d, y, w, plan->zmulti, x, nrhs, \
-                           rg, plan->mm_uptr, plan->mm_uhead, ma, ma.sel, ma.vseg, ma.col);                 \
+                           rg, plan->mm_rptr, plan->mm_rec, ma, ma.sel, ma.vseg, ma.col);                   \
         break;
                 CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
                 CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)

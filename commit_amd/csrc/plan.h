@@ -85,6 +85,11 @@ struct commit_hip_plan {
     uint32_t *mm_uhead = nullptr;  // [heads] entry index of each head, voxel-major
     uint32_t *mm_hpos = nullptr;   // [n] head of each entry, in fibre order (as fib_perm)
     float *mm_lenF = nullptr;      // [n] length of each entry, in fibre order
+    // the rows of each voxel's A^T product as one flat list (IC heads for each
+    // IC RF, EC peaks for each EC RF, ISO entries for each ISO RF): record =
+    // LUT row code (low 32 bits) | output slot (high 32 bits); built on first use
+    uint32_t *mm_rptr = nullptr;   // [nVOX+1] first row of each voxel
+    uint64_t *mm_rec = nullptr;    // [rows]
     // orientation-major lists of the IC, EC and ISO entries for the LUT gradient
     // (lut_grad.hip; built on first use)
     commit_internal::OmList lg_om[3];
