@@ -215,12 +215,20 @@ __device__ __forceinline__ void mm_vox_A(f4 (&acc)[NH], const DevDict &d, const 
         const int cnt = (int)min((uint32_t)kWave, vr.total - base);
         const int nsteps = (cnt + 3) >> 2;
         const uint32_t cc = code;
-        float a0[NH], a1[NH];
-#define ROW(t) row_code(d, (uint32_t)__shfl((int)cc, 4 * (t) + g), lane_off)
-        // the first two LUT rows are requested before the coefficient
-        // gathers, so that both are in flight together
-        load_run<NH>(a0, ROW(0));
-        if (nsteps > 1) load_run<NH>(a1, ROW(1));
+        // a ring of four LUT row buffers: step t's rows are requested three
+        // steps before its MFMAs; requests past the chunk's last step repeat
+        // its rows (cached, never used), so every load is unconditional and
+        // each wait covers one step's rows only
+        float b0[NH], b1[NH], b2[NH], b3[NH];
+#define ROW(t) row_code(d, (uint32_t)__shfl((int)cc, 4 * min((t), nsteps - 1) + g), lane_off)
+        // the first four steps' rows are requested before the coefficient
+        // gathers, so that all are in flight together
+        load_run<NH>(b0, ROW(0));
+        load_run<NH>(b1, ROW(1));
+        load_run<NH>(b2, ROW(2));
+        load_run<NH>(b3, ROW(3));
+        // the scheduler would otherwise sink each row load to its first use
+        __builtin_amdgcn_sched_barrier(0);
         // coefficients scale * x of this lane's entries, all 16 gathers in flight
         float xv[16];
         {
@@ -240,15 +248,16 @@ __device__ __forceinline__ void mm_vox_A(f4 (&acc)[NH], const DevDict &d, const 
         const bool more = base + kWave < vr.total;
         EntA en;
         if (more) en = entry_a(d, vr, base + kWave + lane);     // combined after this chunk's products
-#pragma unroll
-        for (int t = 0; t < 16; t += 2) {
-            a_mma<NH>(acc, a0, xv[t]);
-            if (t + 1 >= nsteps) break;
-            if (t + 2 < nsteps) load_run<NH>(a0, ROW(t + 2));
-            a_mma<NH>(acc, a1, xv[t + 1]);
-            if (t + 2 >= nsteps) break;
-            if (t + 3 < nsteps) load_run<NH>(a1, ROW(t + 3));
-        }
+        // the 16 steps written out (buffer t % 4, static indices throughout)
+#define STEP(B, T)                                                                                                 \
+    if ((T) >= nsteps) goto chunk_done;                                                                            \
+    a_mma<NH>(acc, B, xv[T]);                                                                                      \
+    load_run<NH>(B, ROW((T) + 4));                                                                                 \
+    __builtin_amdgcn_sched_barrier(0);
+        STEP(b0, 0) STEP(b1, 1) STEP(b2, 2) STEP(b3, 3) STEP(b0, 4) STEP(b1, 5) STEP(b2, 6) STEP(b3, 7)
+        STEP(b0, 8) STEP(b1, 9) STEP(b2, 10) STEP(b3, 11) STEP(b0, 12) STEP(b1, 13) STEP(b2, 14) STEP(b3, 15)
+    chunk_done:
+#undef STEP
 #undef ROW
         if (more) entry_b<EV>(d, en, eval, code, xcol, scale);
     }
