@@ -14,8 +14,9 @@
 // LDS.  The plan keeps each voxel's IC entries sorted by orientation
 // (stable), so a voxel's entries of one slab are a contiguous range; their
 // offsets are a [nVOX][nsl+1] table.  Voxels are dealt to the wavefronts of a
-// work-group longest-first onto the least-loaded wavefront, so the wavefronts
-// reach each slab barrier together; work-groups are numbered heaviest first.
+// work-group by their work in each slab (distinct orientations there), so the
+// wavefronts reach each slab barrier together; work-groups are numbered
+// heaviest first.
 // EC and ISO (a few entries per voxel) use rows from global memory after the
 // slab walk: the items of all of a wavefront's voxels are decoded lane-parallel
 // and each voxel's rows are fetched together.
@@ -34,6 +35,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "../../include/commit_hip.h"
@@ -614,46 +616,160 @@ __global__ void k_slab_ofs(const uint32_t *__restrict__ ptr, const uint16_t *__r
     }
 }
 
+// Work of voxel v in slab s, in half head-slots: the (voxel, slab) range
+// [ofs[v][s], ofs[v][s+1]) holds nu distinct orientations (heads), applied in
+// groups of 8 with a tail of at most 4 (as slab_at_chunk runs them), plus a
+// fixed start-up per non-empty range and half a slot per entry.
+__global__ void k_slab_cost(const uint32_t *__restrict__ ofs, const uint16_t *__restrict__ o, int nVOX, int nsl,
+                            uint32_t *__restrict__ cost)
+{
+    const size_t total = (size_t)nVOX * nsl;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t v = t / nsl;
+        const int s = (int)(t - v * nsl);
+        const uint32_t b = ofs[v * (nsl + 1) + s], e = ofs[v * (nsl + 1) + s + 1];
+        uint32_t nu = 0;
+        for (uint32_t k = b; k < e; k++) nu += (k == b || o[k] != o[k - 1]) ? 1u : 0u;
+        const uint32_t g = nu > 4 ? (nu - 4 + 7) / 8 : 0u;
+        const uint32_t slots = 8 * g + (8 * g < nu ? 4u : 0u);
+        cost[t] = 2 * slots + (e > b ? 16u : 0u) + (e - b);
+    }
+}
+
 }  // namespace
 
 namespace commit_internal {
 
 namespace {
 
+// One work-group's voxels dealt to its W wavefronts (V each).  A wavefront
+// runs the slabs in order with a barrier after each, so a work-group takes
+// about the sum over slabs of its slowest wavefront's work in that slab;
+// c[i*nsl + s] is voxel i's work in slab s.  Voxels go longest first to the
+// wavefront (with room) that raises that sum least, ties to the lighter one;
+// then one pass of pairwise swaps between wavefronts keeps each swap that
+// lowers it.  Returns the sum; slot[i] = wavefront * V + place.
+uint64_t deal_group(const uint32_t *c, int nv, int nsl, int W, int V, int *slot)
+{
+    std::vector<int> order(nv), wave(nv);
+    std::vector<uint64_t> tot(nv, 0), L((size_t)W * nsl, 0), mx(nsl);
+    std::vector<int> used(W, 0);
+    for (int i = 0; i < nv; i++)
+        for (int s = 0; s < nsl; s++) tot[i] += c[(size_t)i * nsl + s];
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tot[a] > tot[b]; });
+    for (int i : order) {
+        const uint32_t *ci = c + (size_t)i * nsl;
+        for (int s = 0; s < nsl; s++) {
+            mx[s] = 0;
+            for (int w = 0; w < W; w++) mx[s] = std::max(mx[s], L[(size_t)w * nsl + s]);
+        }
+        int best = -1;
+        uint64_t best_inc = 0, best_row = 0;
+        for (int w = 0; w < W; w++) {
+            if (used[w] >= V) continue;
+            uint64_t inc = 0, row = 0;
+            for (int s = 0; s < nsl; s++) {
+                const uint64_t nl = L[(size_t)w * nsl + s] + ci[s];
+                inc += nl > mx[s] ? nl - mx[s] : 0;
+                row += nl;
+            }
+            if (best < 0 || inc < best_inc || (inc == best_inc && row < best_row)) {
+                best = w;
+                best_inc = inc;
+                best_row = row;
+            }
+        }
+        wave[i] = best;
+        used[best]++;
+        for (int s = 0; s < nsl; s++) L[(size_t)best * nsl + s] += ci[s];
+    }
+    // per slab, the three heaviest wavefronts: the largest load outside any
+    // two wavefronts is among them
+    std::vector<int> top((size_t)nsl * 3);
+    auto rank = [&]() {
+        for (int s = 0; s < nsl; s++) {
+            int *t = &top[(size_t)s * 3];
+            t[0] = t[1] = t[2] = -1;
+            for (int w = 0; w < W; w++) {
+                const uint64_t v = L[(size_t)w * nsl + s];
+                if (t[0] < 0 || v > L[(size_t)t[0] * nsl + s]) { t[2] = t[1]; t[1] = t[0]; t[0] = w; }
+                else if (t[1] < 0 || v > L[(size_t)t[1] * nsl + s]) { t[2] = t[1]; t[1] = w; }
+                else if (t[2] < 0 || v > L[(size_t)t[2] * nsl + s]) t[2] = w;
+            }
+        }
+    };
+    auto objective = [&]() {
+        uint64_t m = 0;
+        for (int s = 0; s < nsl; s++) {
+            uint64_t v = 0;
+            for (int w = 0; w < W; w++) v = std::max(v, L[(size_t)w * nsl + s]);
+            m += v;
+        }
+        return m;
+    };
+    rank();
+    uint64_t cur = objective();
+    for (int i = 0; i < nv; i++)
+        for (int j = i + 1; j < nv; j++) {
+            const int wi = wave[i], wj = wave[j];
+            if (wi == wj) continue;
+            const uint32_t *ci = c + (size_t)i * nsl, *cj = c + (size_t)j * nsl;
+            uint64_t val = 0;
+            for (int s = 0; s < nsl && val < cur; s++) {
+                uint64_t other = 0;
+                for (int k = 0; k < 3; k++) {
+                    const int w = top[(size_t)s * 3 + k];
+                    if (w >= 0 && w != wi && w != wj) {
+                        other = L[(size_t)w * nsl + s];
+                        break;
+                    }
+                }
+                const uint64_t li = L[(size_t)wi * nsl + s] - ci[s] + cj[s];
+                const uint64_t lj = L[(size_t)wj * nsl + s] - cj[s] + ci[s];
+                val += std::max(other, std::max(li, lj));
+            }
+            if (val >= cur) continue;
+            for (int s = 0; s < nsl; s++) {
+                L[(size_t)wi * nsl + s] += (uint64_t)cj[s] - ci[s];
+                L[(size_t)wj * nsl + s] += (uint64_t)ci[s] - cj[s];
+            }
+            std::swap(wave[i], wave[j]);
+            cur = val;
+            rank();
+        }
+    std::fill(used.begin(), used.end(), 0);
+    for (int i = 0; i < nv; i++) slot[i] = wave[i] * V + used[wave[i]]++;
+    return cur;
+}
+
 // Voxels dealt to the W wavefronts of each work-group (W*V consecutive voxels
-// per group), longest first onto the least-loaded wavefront with room.
-int build_vlist(commit_hip_plan *p, const std::vector<uint32_t> &ptr, int W, int V, int32_t **out, unsigned *blocks)
+// per group) by deal_group; work-groups numbered heaviest first.  cost:
+// [nVOX][nsl] per-slab work (k_slab_cost).
+int build_vlist(commit_hip_plan *p, const std::vector<uint32_t> &cost, int nsl, int W, int V, int32_t **out,
+                unsigned *blocks)
 {
     const int nVOX = p->d.nVOX, per = W * V;
     const size_t nb = ((size_t)nVOX + per - 1) / per;
     std::vector<int32_t> vl(nb * per, -1);
-    std::vector<int> order(per);
-    std::vector<uint64_t> load(W);
-    std::vector<int> used(W);
-    std::vector<uint64_t> cost(nb);
-    for (size_t b = 0; b < nb; b++) {
-        const int v0 = (int)(b * per), nv = std::min(per, nVOX - v0);
-        std::iota(order.begin(), order.begin() + nv, v0);
-        std::stable_sort(order.begin(), order.begin() + nv, [&](int a, int c) {
-            return ptr[a + 1] - ptr[a] > ptr[c + 1] - ptr[c];
+    std::vector<uint64_t> gcost(nb);
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++)
+        pool.emplace_back([&, t]() {
+            std::vector<int> slot(per);
+            for (size_t b = t; b < nb; b += nth) {
+                const int v0 = (int)(b * per), nv = std::min(per, nVOX - v0);
+                gcost[b] = deal_group(cost.data() + (size_t)v0 * nsl, nv, nsl, W, V, slot.data());
+                for (int i = 0; i < nv; i++) vl[b * per + (size_t)slot[i]] = v0 + i;
+            }
         });
-        std::fill(load.begin(), load.end(), 0);
-        std::fill(used.begin(), used.end(), 0);
-        for (int r = 0; r < nv; r++) {
-            int best = -1;
-            for (int w = 0; w < W; w++)
-                if (used[w] < V && (best < 0 || load[w] < load[best])) best = w;
-            const int v = order[r];
-            vl[b * per + (size_t)best * V + used[best]++] = v;
-            load[best] += ptr[v + 1] - ptr[v];
-        }
-        cost[b] = *std::max_element(load.begin(), load.end());
-    }
+    for (auto &th : pool) th.join();
     // work-groups are dispatched in index order: the heaviest first, so that
     // the last ones to start are short and the launch ends evenly
     std::vector<uint32_t> bo(nb);
     std::iota(bo.begin(), bo.end(), 0u);
-    std::stable_sort(bo.begin(), bo.end(), [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
+    std::stable_sort(bo.begin(), bo.end(), [&](uint32_t a, uint32_t c) { return gcost[a] > gcost[c]; });
     std::vector<int32_t> sorted(vl.size());
     for (size_t b = 0; b < nb; b++)
         std::copy(vl.begin() + (size_t)bo[b] * per, vl.begin() + (size_t)(bo[b] + 1) * per, sorted.begin() + b * per);
@@ -682,11 +798,7 @@ int slab_configure(commit_hip_plan *p, int mode)
     const int Smax = std::min(d.ndirs, (ldsf - W * kBcastFloats) / d.nSp);
     const int nsl = (d.ndirs + Smax - 1) / Smax;
     const int S = (d.ndirs + nsl - 1) / nsl;           // equal slabs: no short last step
-    std::vector<uint32_t> ptr((size_t)d.nVOX + 1);
-    HIP_TRY(hipMemcpy(ptr.data(), d.ic.ptr, ptr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     int rc;
-    if ((rc = build_vlist(p, ptr, W, kSlabVA, &p->slab_vlist, &p->slab_blocks))) return rc;
-    if ((rc = build_vlist(p, ptr, W, kSlabVAt, &p->slab_vlist_at, &p->slab_blocks_at))) return rc;
     const size_t nofs = (size_t)d.nVOX * (nsl + 1);
     void *mem = nullptr;
     if ((rc = dev_alloc(p, nofs * sizeof(uint32_t), &mem))) return rc;
@@ -694,6 +806,19 @@ int slab_configure(commit_hip_plan *p, int mode)
     const unsigned g = (unsigned)std::min<size_t>((nofs + 255) / 256, 65536);
     hipLaunchKernelGGL(k_slab_ofs, dim3(g), dim3(256), 0, 0, d.ic.ptr, d.ic.o, d.nVOX, nsl, S, p->slab_ofs);
     HIP_TRY(hipGetLastError());
+    // per-slab work of every voxel, for the dealing
+    const size_t ncost = (size_t)d.nVOX * nsl;
+    uint32_t *dcost = nullptr;
+    HIP_TRY(hipMalloc(&dcost, ncost * sizeof(uint32_t)));
+    const unsigned gc = (unsigned)std::min<size_t>((ncost + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_slab_cost, dim3(gc), dim3(256), 0, 0, p->slab_ofs, d.ic.o, d.nVOX, nsl, dcost);
+    std::vector<uint32_t> cost(ncost);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(cost.data(), dcost, ncost * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(dcost);
+    HIP_TRY(e);
+    if ((rc = build_vlist(p, cost, nsl, W, kSlabVA, &p->slab_vlist, &p->slab_blocks))) return rc;
+    if ((rc = build_vlist(p, cost, nsl, W, kSlabVAt, &p->slab_vlist_at, &p->slab_blocks_at))) return rc;
     HIP_TRY(hipDeviceSynchronize());
     p->slab_S = S;
     p->slab_nsl = nsl;
