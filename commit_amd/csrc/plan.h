@@ -122,7 +122,11 @@ struct SlabArgs {
     const uint32_t *ofs;    // [nVOX][nsl+1]
     const int32_t *vlist;   // [blocks][W][V]
     int S, nsl;
+    int rs;                 // LDS row stride of the slab in floats: nS rounded up to 4
 };
+
+// LDS row stride of the slab kernels (16-byte aligned rows, no padding to nSp)
+inline int slab_row_stride(const DevDict &d) { return (d.nS + 3) & ~3; }
 
 // (Re)build the slab tables for mode 0 (off), 1 or 2; the plan's IC entries
 // must be sorted by orientation inside each voxel.

@@ -53,7 +53,8 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 // LDS floats per work-group: 160 KiB shared by the 16 wavefronts of a CU.
 // The last W*kBcastFloats floats hold one broadcast table per wavefront; the
-// rest holds the slab (S rows of nSp floats).
+// rest holds the slab (S rows packed at rs = nS rounded up to 4 floats) and a
+// zeroed pad of nSp - rs floats after it.
 template <int W>
 struct SlabLds {
     static constexpr int floats = 40960 * W / 16;
@@ -79,16 +80,34 @@ __device__ __forceinline__ Bcast<C> *bcast_table(float *L, int w)
     return reinterpret_cast<Bcast<C> *>(L + SlabLds<W>::floats - (w + 1) * kBcastFloats);
 }
 
-// Copy nfloats contiguous floats (a multiple of 64) into L: 1 KiB pieces,
-// wavefront w takes pieces w, w+W, ...; the caller waits and synchronises.
+// Copy rows LUT rows (stride nSp floats in memory, from src) into L packed at
+// stride rs floats (nS rounded up to 4): 1 KiB pieces of the packed slab,
+// 16 bytes per lane, wavefront w takes pieces w, w+W, ...; the caller waits
+// and synchronises.
 template <int W>
-__device__ __forceinline__ void slab_load(float *L, const float *__restrict__ src, int nfloats, int w, int lane)
+__device__ __forceinline__ void slab_load(float *L, const float *__restrict__ src, int rows, int rs, int nSp, int w,
+                                          int lane)
 {
+    const int nfloats = rows * rs;
     const int npieces = (nfloats + 255) / 256;
     for (int pc = w; pc < npieces; pc += W) {
         const int f = pc * 256 + 4 * lane;
-        if (f < nfloats) __builtin_amdgcn_global_load_lds(src + f, (lds_void *)(L + pc * 256), 16, 0, 0);
+        const int r = f / rs;
+        if (f < nfloats)
+            __builtin_amdgcn_global_load_lds(src + (size_t)r * nSp + (f - r * rs), (lds_void *)(L + pc * 256), 16, 0,
+                                             0);
     }
+}
+
+// The lanes of a row's last samples read up to nSp - rs floats past the row
+// (rows are packed at rs floats in LDS): past the slab's last row they read
+// this pad, zeroed once per work-group, so those lanes see finite values.
+// Products of padded samples are discarded (A x stores samples < nS only) or
+// multiplied by a zero y (A^T y).
+template <int W>
+__device__ __forceinline__ void slab_pad_zero(float *L, const SlabArgs &sa, int nSp)
+{
+    for (int k = (int)threadIdx.x; k < nSp - sa.rs; k += W * kWave) L[sa.S * sa.rs + k] = 0.0f;
 }
 
 // One LUT row of the slab in the lane layout (Lanes<NJ, Q>): Lq is the row's
@@ -139,7 +158,7 @@ constexpr int kMergeMin = 12;
 // accumulators, so the FP32 rounding is that of a 4-term sum.
 template <int NJ, typename T>
 __device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T xv, bool ok, const float *Lq,
-                                             const float *Ls, Bcast<T> *bt, int o0, int nSp, int lane)
+                                             const float *Ls, Bcast<T> *bt, int o0, int rs, int lane)
 {
     // IC coefficient l * x * eval                                  (setup_operator.py:63-96)
     double c = (double)xv;
@@ -177,7 +196,7 @@ __device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T 
     const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
     const int dst = nzl ? below : nz + (lane - below);
     bt->c[dst] = (T)c;
-    bt->off[dst] = ((int)r.o - o0) * nSp;
+    bt->off[dst] = ((int)r.o - o0) * rs;
     for (int t = 0; t < nz; t += 4) {
         const int4 of = *reinterpret_cast<const int4 *>(&bt->off[t]);
         float ra[NJ], rb[NJ], rc[NJ], rd[NJ];
@@ -215,7 +234,7 @@ __device__ __forceinline__ void slab_a_chunk(double (&acc)[NJ], const Rec &r, T 
 template <int NJ, int V, typename T>
 __device__ __forceinline__ void slab_a_pass(double (&acc)[V][NJ], const Comp &cp, const uint32_t (&rb)[V],
                                             const uint32_t (&re)[V], const T *__restrict__ xb, const float *Lq,
-                                            const float *Ls, Bcast<T> *bt, int o0, int nSp, int lane, Rec r0, Rec r1)
+                                            const float *Ls, Bcast<T> *bt, int o0, int rs, int lane, Rec r0, Rec r1)
 {
     T x0 = xb[r0.col];
 #pragma unroll
@@ -225,10 +244,10 @@ __device__ __forceinline__ void slab_a_pass(double (&acc)[V][NJ], const Comp &cp
         T x1 = x0;
         if (u + 1 < V) x1 = xb[r1.col];
         if (rb[u] < re[u]) {
-            slab_a_chunk<NJ, T>(acc[u], r0, x0, rb[u] + lane < re[u], Lq, Ls, bt, o0, nSp, lane);
+            slab_a_chunk<NJ, T>(acc[u], r0, x0, rb[u] + lane < re[u], Lq, Ls, bt, o0, rs, lane);
             for (uint32_t base = rb[u] + kWave; base < re[u]; base += kWave) {
                 const Rec r = rec_load(cp, cp.col, base, re[u], lane);
-                slab_a_chunk<NJ, T>(acc[u], r, xb[r.col], base + lane < re[u], Lq, Ls, bt, o0, nSp, lane);
+                slab_a_chunk<NJ, T>(acc[u], r, xb[r.col], base + lane < re[u], Lq, Ls, bt, o0, rs, lane);
             }
         }
         r0 = r1;
@@ -300,6 +319,7 @@ __device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x
     int vox[V];
 #pragma unroll
     for (int u = 0; u < V; u++) vox[u] = __builtin_amdgcn_readfirstlane(vl[u]);
+    slab_pad_zero<W>(L, sa, d.nSp);
     double acc[V][NJ];
 #pragma unroll
     for (int u = 0; u < V; u++)
@@ -321,10 +341,10 @@ __device__ __forceinline__ void a_slab(const DevDict &d, const T *__restrict__ x
             const Rec r0 = rec_load(d.ic, d.ic.col, rb[0], re[0], lane);
             const Rec r1 = rec_load(d.ic, d.ic.col, rb[V > 1 ? 1 : 0], re[V > 1 ? 1 : 0], lane);
             __syncthreads();                                         // previous slab consumed
-            slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows * d.nSp, w, lane);
+            slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows, sa.rs, d.nSp, w, lane);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();                                         // slab in LDS
-            slab_a_pass<NJ, V, T>(acc, d.ic, rb, re, xb, L + 4 * lane, L + lane, bt, o0, d.nSp, lane, r0, r1);
+            slab_a_pass<NJ, V, T>(acc, d.ic, rb, re, xb, L + 4 * lane, L + lane, bt, o0, sa.rs, lane, r0, r1);
         }
     }
     const bool tail = Lanes<NJ, kQuads<T>>::tail(lane, d.nS);
@@ -425,7 +445,7 @@ __device__ __forceinline__ double dot_row(const float (&raw)[NJ], const float (&
 // l * dot.                                            (setup_operator.py:384-428)
 template <int NJ, typename Y>
 __device__ __forceinline__ void slab_at_chunk(const Y (&yv)[NJ], const Rec &r, int cnt, double *__restrict__ zi,
-                                              const float *Lq, const float *Ls, Bcast<double> *bt, int o0, int nSp,
+                                              const float *Lq, const float *Ls, Bcast<double> *bt, int o0, int rs,
                                               int lane)
 {
     constexpr bool Q = kQuads<Y>;
@@ -435,7 +455,7 @@ __device__ __forceinline__ void slab_at_chunk(const Y (&yv)[NJ], const Rec &r, i
     const int nu = __popcll(m);
     const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
     const int run = head ? below : below - 1;                 // this entry's head, counted in head order
-    bt->off[head ? below : nu + (lane - below)] = ((int)r.o - o0) * nSp;
+    bt->off[head ? below : nu + (lane - below)] = ((int)r.o - o0) * rs;
     int4 oa = *reinterpret_cast<const int4 *>(&bt->off[0]);
     int4 ob = *reinterpret_cast<const int4 *>(&bt->off[4]);
     int t0 = 0;
@@ -498,6 +518,7 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     Bcast<double> *bt = bcast_table<W, double>(L, w);
+    slab_pad_zero<W>(L, sa, d.nSp);
     const int32_t *vl = sa.vlist + ((size_t)blockIdx.x * W + w) * V;
     int vox[V];
 #pragma unroll
@@ -536,7 +557,7 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
             }
             Rec r0 = rec_load(d.ic, d.zpos, rb[0], re[0], lane);       // loads while the slab is copied
             __syncthreads();
-            slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows * d.nSp, w, lane);
+            slab_load<W>(L, d.wmr + (size_t)i * lut_rf + (size_t)o0 * d.nSp, rows, sa.rs, d.nSp, w, lane);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             const float *Lq = L + 4 * lane, *Ls = L + lane;
@@ -546,10 +567,10 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
                 if (u + 1 < V) r1 = rec_load(d.ic, d.zpos, rb[u + 1 < V ? u + 1 : 0], re[u + 1 < V ? u + 1 : 0], lane);
                 if (rb[u] < re[u]) {
                     slab_at_chunk<NJ, T>(yv[u], r0, (int)min((uint32_t)kWave, re[u] - rb[u]), zi, Lq, Ls, bt, o0,
-                                         d.nSp, lane);
+                                         sa.rs, lane);
                     for (uint32_t base = rb[u] + kWave; base < re[u]; base += kWave)
                         slab_at_chunk<NJ, T>(yv[u], rec_load(d.ic, d.zpos, base, re[u], lane),
-                                             (int)min((uint32_t)kWave, re[u] - base), zi, Lq, Ls, bt, o0, d.nSp, lane);
+                                             (int)min((uint32_t)kWave, re[u] - base), zi, Lq, Ls, bt, o0, sa.rs, lane);
                 }
                 r0 = r1;
             }
@@ -795,7 +816,8 @@ int slab_configure(commit_hip_plan *p, int mode)
     if (d.n == 0 || d.nIC == 0 || d.nVOX == 0 || d.nSp / kWave > kSlabMaxNJ) return 0;
     const int W = mode == 1 ? 8 : 16;
     const int ldsf = (mode == 1 ? SlabLds<8>::floats : SlabLds<16>::floats);
-    const int Smax = std::min(d.ndirs, (ldsf - W * kBcastFloats) / d.nSp);
+    const int rs = slab_row_stride(d);                // rows packed at nS rounded up to 4 floats
+    const int Smax = std::min(d.ndirs, (ldsf - W * kBcastFloats - (d.nSp - rs)) / rs);
     const int nsl = (d.ndirs + Smax - 1) / Smax;
     const int S = (d.ndirs + nsl - 1) / nsl;           // equal slabs: no short last step
     int rc;
@@ -846,7 +868,7 @@ void slab_release(commit_hip_plan *p)
 
 #define SLAB_SWITCH(KERNEL, kSlabV, VLIST, BLOCKS, ...)                                                           \
     do {                                                                                                          \
-        const SlabArgs sa{p->slab_ofs, VLIST, p->slab_S, p->slab_nsl};                                            \
+        const SlabArgs sa{p->slab_ofs, VLIST, p->slab_S, p->slab_nsl, slab_row_stride(d)};                                            \
         const dim3 grid(BLOCKS);                                                                                  \
         const int NJv = d.nSp / kWave;                                                                            \
         if (p->slab == 1) {                                                                                       \
