@@ -1,13 +1,13 @@
 // Tractogram voxeliser on the GPU: streamlines -> the IC/EC/ISO dictionary
-// that trk2dictionary() exports (trk2dictionary_c.cpp:101-869), for
+// that trk2dictionary() exports (trk2dictionary_c.cpp:111-869), for
 // streamlines already read into memory.
 //
 // Pipeline (one stream, all device-resident):
 //   k_vox_walk<false>   one thread per streamline runs fiberForwardModel
-//                       (:584-735) with its replicas, rayBoxIntersection
-//                       (:795-848) and segmentForwardModel (:741-789), and
+//                       (:584-755) with its replicas, rayBoxIntersection
+//                       (:814-869) and segmentForwardModel (:761-808), and
 //                       counts the pieces it would record, with FiberLen and
-//                       FiberLenTot (float sums in recording order, :782-788)
+//                       FiberLenTot (float sums in recording order, :804-807)
 //   exclusive scan      piece offsets per streamline
 //   k_vox_walk<true>    the same walk again, storing each piece as
 //                       key = o * nvox + v (v = x + X(y + Y z), so ascending
@@ -15,17 +15,17 @@
 //                       and its double value w * len, in recording order
 //   segmented sort      stable radix sort of each streamline's pieces by key
 //   k_vox_merge         equal keys summed into a float in recording order
-//                       (the map's `+=`, :782), kept flag (:503-505)
+//                       (the map's `+=`, :804), kept flag (:510-515)
 //   segmented sort      the merged segments of each streamline by voxel
 //                       (stable, so each voxel's segments stay in map order)
 //   k_vox_norm          TRK_norm: per-voxel float sums, float sum of their
-//                       squares in (z, y, x) order, float square root (:524-537)
+//                       squares in (z, y, x) order, float square root (:530-537)
 //   scans + k_vox_write kept streamlines renumbered (trk2dictionary.pyx:551-563)
 //                       and their segments written in streamline order
-//   radix sort + k_vox_tdi   compute_tdi (pyx:46-54): float sums per voxel in
+//   radix sort + k_vox_tdi   compute_tdi (pyx:50-59): float sums per voxel in
 //                       segment order, and which voxels hold an IC segment
-//   k_vox_ec_*          ECSegments (:266-383), voxels in (z, y, x) order
-//   k_vox_iso_*         ISOcompartments (:386-436)
+//   k_vox_ec_*          ECSegments (:289-397), voxels in (z, y, x) order
+//   k_vox_iso_*         ISOcompartments (:400-437)
 //
 // The arithmetic is the reference's, operation by operation, in IEEE double
 // with its float roundings, and this file is compiled without contraction
@@ -58,7 +58,7 @@ constexpr int kMaxFibLen = 10000;      // MAX_FIB_LEN, trk2dictionary_c.cpp:16
 struct VoxArgs {
     int X, Y, Z;
     float px, py, pz;
-    float sx, sy, sz;                  // shift in mm: fiber_shift * pixdim, float (:124-126)
+    float sx, sy, sz;                  // shift in mm: fiber_shift * pixdim, float (:125-127)
     float min_seg, min_fib, max_fib;
     int intersect;
     int n_rep;
@@ -73,10 +73,10 @@ struct D3 {
     double x, y, z;
 };
 
-// Vector<double>::norm (Vector.h:47-50)
+// Vector<double>::norm (Vector.h:44-47)
 __device__ __forceinline__ float vnorm(D3 a) { return (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 
-// Vector<double>::Normalize (Vector.h:54-61)
+// Vector<double>::Normalize (Vector.h:50-57)
 __device__ __forceinline__ D3 vnormalize(D3 a)
 {
     float len = (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
@@ -87,6 +87,8 @@ __device__ __forceinline__ D3 vnormalize(D3 a)
     return a;
 }
 
+// ptrHashTable[ox*181 + oy] with the angles rounded to degrees as
+// segmentForwardModel (:799-802) and ECSegments (:371-374) round them
 __device__ __forceinline__ int hash_index(const short *hash, double colat, double lon)
 {
     int ox = (int)round(colat / M_PI * 180.0);
@@ -107,7 +109,7 @@ __device__ __forceinline__ long long max_cross(double len, const VoxArgs &a)
     return 64 + 4 * (long long)c;
 }
 
-// rayBoxIntersection (trk2dictionary_c.cpp:795-848)
+// rayBoxIntersection (trk2dictionary_c.cpp:814-869)
 __device__ __forceinline__ bool raybox(D3 o, D3 d, D3 vmin, D3 vmax, double &t)
 {
     double tmin, tmax, tymin, tymax, tzmin, tzmax;
@@ -142,7 +144,7 @@ struct Recorder {
     __device__ Recorder(const VoxArgs &a_, uint64_t *k, double *v, uint32_t lim)
         : a(a_), keys(k), vals(v), limit(lim) {}
 
-    // segmentForwardModel (trk2dictionary_c.cpp:741-789)
+    // segmentForwardModel (trk2dictionary_c.cpp:761-808)
     __device__ void segment(D3 P1, D3 P2, int k, double w)
     {
         D3 dir;
@@ -177,7 +179,7 @@ struct Recorder {
     }
 };
 
-// fiberForwardModel (trk2dictionary_c.cpp:584-735) for streamline f.  MULTI:
+// fiberForwardModel (trk2dictionary_c.cpp:584-755) for streamline f.  MULTI:
 // more than one replica, their positions kept in Pbuf [nF][n_rep].
 template <bool EMIT, bool MULTI>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *__restrict__ pts,
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_runs(const uint32_t *__restri
     if (i == n - 1 || vs[i + 1] != v) re[v] = (uint32_t)(i + 1);
 }
 
-// compute_tdi (trk2dictionary.pyx:46-54): a wavefront per voxel loads its
+// compute_tdi (trk2dictionary.pyx:50-59): a wavefront per voxel loads its
 // lengths 64 at a time and adds them one by one in segment order (the float
 // sum of the reference's loop); marks the voxels holding an IC segment.
 __global__ __launch_bounds__(kVoxBlock) void k_vox_tdi(const uint32_t *__restrict__ rs,
@@ -488,7 +490,7 @@ struct EcArgs {
     const short *hash;
 };
 
-// ECSegments (trk2dictionary_c.cpp:266-383) for voxel v (x fastest, so the
+// ECSegments (trk2dictionary_c.cpp:289-397) for voxel v (x fastest, so the
 // reference's z, y, x loops visit ascending v).  WRITE = false counts.
 template <bool WRITE>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_ec(EcArgs a, uint64_t nvox, const uint8_t *__restrict__ has,
@@ -539,7 +541,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_ec(EcArgs a, uint64_t nvox, c
     if (!WRITE) cnt[v] = c;
 }
 
-// ISOcompartments (trk2dictionary_c.cpp:386-436).  WRITE = false flags.
+// ISOcompartments (trk2dictionary_c.cpp:400-437).  WRITE = false flags.
 template <bool WRITE>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_iso(int X, int Y, int Z, uint64_t nvox,
                                                        const float *__restrict__ iso, const float *__restrict__ mask,
@@ -746,7 +748,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
         d_pts = V->at<float>(B::PTS);
         d_ptr = V->at<int64_t>(B::FPTR);
     }
-    // streamline offsets (MAX_FIB_LEN, trk2dictionary_c.cpp:171 and :856)
+    // streamline offsets (MAX_FIB_LEN, trk2dictionary_c.cpp:195 and :878)
     if ((rc = V->need(B::STATUS, 16))) return rc;
     if (n_count > 0) {
         uint32_t *st = V->at<uint32_t>(B::STATUS);
@@ -764,7 +766,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
                         chk[1], kMaxFibLen, kMaxFibLen - 1);
     }
     if ((rc = upload(V, B::HASH, ptrHashTable, 181 * 181, s))) return rc;
-    // replicas: rho, sin(angle/2), cos(angle/2), weight (:614-619)
+    // replicas: rho, sin(angle/2), cos(angle/2), weight (:615-623)
     std::vector<double> rep(4 * (size_t)nReplicas);
     for (int k = 0; k < nReplicas; k++) {
         rep[k] = ptrBlurRho[k];

@@ -94,8 +94,8 @@ def _trk_header(raw):
 def read_trk(filename):
     """Streamlines of a .trk file as (fibers float32 [n_points, 3] in voxmm,
     fib_ptr int64 [n+1], header), the coordinates read_fiberTRK returns
-    (trk2dictionary_c.cpp:851-873).  A streamline of MAX_FIB_LEN or more
-    points makes the reference generate no dictionary (:171): ValueError."""
+    (trk2dictionary_c.cpp:873-893).  A streamline of MAX_FIB_LEN or more
+    points makes the reference generate no dictionary (:195): ValueError."""
     with open(filename, "rb") as f:
         raw = f.read()
     h = _trk_header(raw[:1000])
@@ -173,7 +173,7 @@ def _tck_header(filename):
 
 def read_tck(filename, toVOXMM):
     """Streamlines of a .tck file mapped to voxmm with the float32 3x4 matrix
-    toVOXMM, evaluated as read_fiberTCK does (trk2dictionary_c.cpp:876-911):
+    toVOXMM, evaluated as read_fiberTCK does (trk2dictionary_c.cpp:896-911):
     ((p0*m0 + p1*m1) + p2*m2) + m3 in float arithmetic."""
     h = _tck_header(filename)
     data = np.fromfile(filename, dtype=h["_dtype"], offset=h["_offset_data"]).astype(np.float32).reshape(-1, 3)

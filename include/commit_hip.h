@@ -416,15 +416,15 @@ int commit_hip_vox_create(commit_hip_vox **vox, int device);
 int commit_hip_vox_destroy(commit_hip_vox *vox);
 
 /*
- * Replaces trk2dictionary() (trk2dictionary_c.cpp:101-258, called from
- * trk2dictionary.pyx:535-541) for streamlines already read into memory: the
+ * Replaces trk2dictionary() (trk2dictionary_c.cpp:111-286, called from
+ * trk2dictionary.pyx:534-539) for streamlines already read into memory: the
  * IC segments (ICSegments + fiberForwardModel + segmentForwardModel +
  * rayBoxIntersection, :444-848), the EC peaks (ECSegments, :266-383), the ISO
  * voxels (ISOcompartments, :386-436), the kept-streamline renumbering of
- * IC_f (trk2dictionary.pyx:551-563) and the TDI map (compute_tdi, pyx:46-54).
+ * IC_f (trk2dictionary.pyx:551-563) and the TDI map (compute_tdi, pyx:50-59).
  * Arguments keep the reference's names and meanings, except:
  *   fibers      float32 [n_points][3] streamline coordinates in voxmm, as
- *               read_fiberTRK / read_fiberTCK return them (:851-911), and
+ *               read_fiberTRK / read_fiberTCK return them (:873-911), and
  *   fib_ptr     int64 [n_count+1] first point of each streamline, in place of
  *               the file name, data offset, scalars and properties;
  *   inputs_on_device  0: fibers, fib_ptr, ptrPEAKS, ptrMASK and ptrISO are

@@ -2,7 +2,7 @@
 
 Only ``tests/`` and the ``cpu_baseline`` leg of ``bench.py --mode voxelize``
 may import this module.  It wraps ``voxelize_oracle.c``, the CPU restatement
-of the reference's tractogram voxeliser (trk2dictionary_c.cpp:101-869 for
+of the reference's tractogram voxeliser (trk2dictionary_c.cpp:111-869 for
 streamlines already in memory; see that file's header for the line map).
 
 Pinning: tests/test_oracle_voxelize_ref.py runs the reference's own
@@ -157,7 +157,7 @@ def reference_available():
 
 
 def _ref_lib():
-    """trk2dictionary() of the reference (trk2dictionary_c.cpp:101-258),
+    """trk2dictionary() of the reference (trk2dictionary_c.cpp:111-286),
     compiled from the reference tree by oracle/Makefile into _ref/."""
     global _REF
     if _REF is None:
@@ -260,7 +260,7 @@ def reference_trk2dictionary(workdir, fibers, fib_ptr, dim, pixdim, hash_table, 
     X, Y, Z = (int(d) for d in dim)
     has = np.zeros((X, Y, Z), bool)
     for t in tdis:
-        has |= t.reshape(X, Y, Z) != 0          # C order [x][y][z] (:540)
+        has |= t.reshape(X, Y, Z) != 0          # C order [x][y][z] (:528)
     out["has_ic"] = has.ravel(order="F")          # x fastest, as the linear voxel ids
     out["seconds"] = seconds                      # the trk2dictionary() call alone
     return out

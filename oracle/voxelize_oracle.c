@@ -5,21 +5,21 @@
  *
  * CPU restatement of the reference's tractogram voxeliser, the IC/EC/ISO
  * export of trk2dictionary() for streamlines already read into memory:
- *   vo_fiber          fiberForwardModel       trk2dictionary_c.cpp:584-735
- *   vo_segment        segmentForwardModel     trk2dictionary_c.cpp:741-789
- *   vo_raybox         rayBoxIntersection      trk2dictionary_c.cpp:795-848
- *   vo_run (IC part)  ICSegments per fibre    trk2dictionary_c.cpp:444-577
+ *   vo_fiber          fiberForwardModel       trk2dictionary_c.cpp:584-755
+ *   vo_segment        segmentForwardModel     trk2dictionary_c.cpp:761-808
+ *   vo_raybox         rayBoxIntersection      trk2dictionary_c.cpp:814-869
+ *   vo_run (IC part)  ICSegments per fibre    trk2dictionary_c.cpp:446-578
  *                     (std::map<segKey,float> aggregation in (o,z,y,x) key
  *                     order, :24-41; TRK_norm from the per-voxel sums, :530-537)
- *   vo_run (EC part)  ECSegments              trk2dictionary_c.cpp:266-383
- *   vo_run (ISO part) ISOcompartments         trk2dictionary_c.cpp:386-436
- *   vo_run (TDI)      compute_tdi             trk2dictionary.pyx:46-54
+ *   vo_run (EC part)  ECSegments              trk2dictionary_c.cpp:289-397
+ *   vo_run (ISO part) ISOcompartments         trk2dictionary_c.cpp:400-437
+ *   vo_run (TDI)      compute_tdi             trk2dictionary.pyx:50-59
  * plus the kept-fibre renumbering of the per-thread IC_f files
  * (trk2dictionary.pyx:551-563): IC_f is the index among kept streamlines.
  *
  * Arithmetic follows the reference operation by operation: coordinates are
  * float sums (point + shift) widened to double, Vector<double>::norm() and
- * Normalize() go through a float length (Vector.h:47-60), the per-key map
+ * Normalize() go through a float length (Vector.h:44-57), the per-key map
  * values, FiberLen and FiberLenTot are floats accumulated from doubles, and
  * the fibre norm is a float sum of squared per-voxel float sums.  Build with
  * -ffp-contract=off (no fused multiply-adds), as the reference's own build
@@ -44,9 +44,9 @@ typedef struct {
     float shift[3];               /* fiber_shift in voxel units (scaled by pixdim here, :124-126) */
     float min_seg_len, min_fiber_len, max_fiber_len;
     int do_intersect;
-    const float *mask;            /* C order [Nx][Ny][Nz] or NULL (:777) */
-    const float *iso;             /* C order [Nx][Ny][Nz] or NULL (:413) */
-    const float *peaks;           /* C order [Nx][Ny][Nz][3*Np] or NULL (:287) */
+    const float *mask;            /* C order [Nx][Ny][Nz] or NULL (:795) */
+    const float *iso;             /* C order [Nx][Ny][Nz] or NULL (:417) */
+    const float *peaks;           /* C order [Nx][Ny][Nz][3*Np] or NULL (:333) */
     int Np;
     float vf_thr;
     int flip[3];                  /* ECix, ECiy, ECiz: +1 or -1 */
@@ -76,10 +76,10 @@ typedef struct {
     int status;
 } vo_result;
 
-/* Vector<double>::norm (Vector.h:47-50): double sqrt returned as float */
+/* Vector<double>::norm (Vector.h:44-47): double sqrt returned as float */
 static inline float vnorm(v3 a) { return (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 
-/* Vector<double>::Normalize (Vector.h:54-61) */
+/* Vector<double>::Normalize (Vector.h:50-57) */
 static inline v3 vnormalize(v3 a)
 {
     float len = (float)sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
@@ -121,7 +121,7 @@ static void push_piece(fibre_ctx *c, uint64_t key, double w)
     c->n++;
 }
 
-/* round(), as the reference uses on the degree angles (:785-786) */
+/* round(), as the reference uses on the degree angles (:801-802) */
 static inline int hash_index(const short *hash, double colat, double lon)
 {
     int ox = (int)round(colat / M_PI * 180.0);
@@ -132,7 +132,7 @@ static inline int hash_index(const short *hash, double colat, double lon)
     return hash[idx];
 }
 
-/* segmentForwardModel (trk2dictionary_c.cpp:741-789) */
+/* segmentForwardModel (trk2dictionary_c.cpp:761-808) */
 static void vo_segment(fibre_ctx *c, v3 P1, v3 P2, int k, double w)
 {
     const vo_params *p = c->p;
@@ -166,7 +166,7 @@ static void vo_segment(fibre_ctx *c, v3 P1, v3 P2, int k, double w)
     if (k == 0) c->flen = (float)((double)c->flen + len);
 }
 
-/* rayBoxIntersection (trk2dictionary_c.cpp:795-848) */
+/* rayBoxIntersection (trk2dictionary_c.cpp:814-869) */
 static int vo_raybox(v3 o, v3 d, v3 vmin, v3 vmax, double *t)
 {
     double tmin, tmax, tymin, tymax, tzmin, tzmax;
@@ -188,7 +188,7 @@ static int vo_raybox(v3 o, v3 d, v3 vmin, v3 vmax, double *t)
     return 1;
 }
 
-/* fiberForwardModel (trk2dictionary_c.cpp:584-735) for one streamline of
+/* fiberForwardModel (trk2dictionary_c.cpp:584-755) for one streamline of
  * `pts` points (x,y,z interleaved floats). */
 static void vo_fiber(fibre_ctx *c, const float *fib, int pts, int do_blur, v3 *P,
                      const double *sinh_, const double *cosh_)
@@ -330,7 +330,7 @@ static void *vo_worker(void *arg)
     v3 *P = (v3 *)malloc((size_t)(p->n_rep > 0 ? p->n_rep : 1) * sizeof(v3));
     double *sh = (double *)malloc((size_t)(p->n_rep > 0 ? p->n_rep : 1) * sizeof(double));
     double *ch = (double *)malloc((size_t)(p->n_rep > 0 ? p->n_rep : 1) * sizeof(double));
-    for (int k = 0; k < p->n_rep; k++) {            /* :614-619 */
+    for (int k = 0; k < p->n_rep; k++) {            /* :615-623 */
         sh[k] = sin(p->angle[k] / 2.0);
         ch[k] = cos(p->angle[k] / 2.0);
     }
@@ -347,7 +347,7 @@ static void *vo_worker(void *arg)
         o->flentot = c.flentot;
         if (c.n == 0) continue;
         /* std::map<segKey,float>: ascending (o,z,y,x), values accumulated in
-         * emission order into a float (:782) */
+         * emission order into a float (:804) */
         qsort(c.pc, c.n, sizeof(piece), cmp_piece);
         o->key = (uint64_t *)malloc(c.n * sizeof(uint64_t));
         o->len = (float *)malloc(c.n * sizeof(float));
@@ -364,7 +364,7 @@ static void *vo_worker(void *arg)
         if (!(c.flen > p->min_fiber_len && c.flen < p->max_fiber_len)) continue;   /* :505 */
         o->kept = 1;
         /* FiberNorm: per-voxel float sums visited in map order, then the float
-         * sum of their squares in (z,y,x) order (:524-537) */
+         * sum of their squares in (z,y,x) order (:530-537) */
         if ((size_t)m > vkcap) {
             vkcap = (size_t)m * 2;
             vk = (vkey *)realloc(vk, vkcap * sizeof(vkey));
@@ -460,7 +460,7 @@ int vo_run(const vo_params *p, const float *pts, const int64_t *fib_ptr, int n_c
     free(th);
 
     const int X = p->dim[0], Y = p->dim[1], Z = p->dim[2];
-    /* EC (:266-383): voxels in (z, y, x) loop order, peaks in file order */
+    /* EC (:289-397): voxels in (z, y, x) loop order, peaks in file order */
     int64_t cap = 1024, n_ec = 0;
     r->EC_v = (uint32_t *)malloc((size_t)cap * 4);
     r->EC_o = (uint16_t *)malloc((size_t)cap * 2);
@@ -506,7 +506,7 @@ int vo_run(const vo_params *p, const float *pts, const int64_t *fib_ptr, int n_c
         free(norms);
     }
     r->n_ec = n_ec;
-    /* ISO (:386-436) */
+    /* ISO (:400-437) */
     int64_t n_iso = 0;
     r->ISO_v = (uint32_t *)malloc((size_t)(nvox ? nvox : 1) * 4);
     for (int iz = 0; iz < Z; iz++)

@@ -1,5 +1,5 @@
 """Device voxeliser (commit_hip_trk2dictionary) against the oracle
-(oracle/voxelize_oracle.c, trk2dictionary_c.cpp:101-869): every output
+(oracle/voxelize_oracle.c, trk2dictionary_c.cpp:111-869): every output
 bit-for-bit, on synthetic streamlines with and without voxel intersection, a
 mask, length limits, a coordinate shift, anisotropic voxels, blur replicas,
 EC peaks and an ISO mask, edge cases, the full-brain tractogram, device-

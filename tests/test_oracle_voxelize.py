@@ -2,8 +2,8 @@
 brute force: straight streamlines with hand-computed voxel crossings,
 orientations and norms; dense sampling of random smooth streamlines; the
 centroid mode; blur replicas along offset lines; the EC peak selection; the
-ISO voxels and the TDI map (trk2dictionary_c.cpp:266-869,
-trk2dictionary.pyx:46-54)."""
+ISO voxels and the TDI map (trk2dictionary_c.cpp:289-869,
+trk2dictionary.pyx:50-59)."""
 import numpy as np
 import pytest
 
@@ -212,7 +212,7 @@ def test_length_filters_renumbering_and_short_streamlines():
 def test_blur_replicas_follow_offset_lines():
     """Replica k starts at R_k * (n rotated by angle_k about the first step's
     direction), n = normalize(d.y - d.z, d.z - d.x, d.x - d.y)
-    (trk2dictionary_c.cpp:596-629): for d = +x, n = (0, -1, 1)/sqrt(2), and a
+    (trk2dictionary_c.cpp:597-641): for d = +x, n = (0, -1, 1)/sqrt(2), and a
     rotation by +90 degrees about x gives (0, -1, -1)/sqrt(2)."""
     dim = (12, 10, 10)
     f = [[0.5, 5.0, 5.0], [6.0, 5.0, 5.0], [11.5, 5.0, 5.0]]

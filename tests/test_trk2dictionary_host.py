@@ -1,6 +1,6 @@
 """Host side of the GPU voxeliser's flow (commit_amd/trk2dictionary.py): the
 TRK and TCK readers and writers, the TCK-to-voxmm matrix
-(trk2dictionary.pyx:406-411, trk2dictionary_c.cpp:876-911) and the blur
+(trk2dictionary.pyx:406-411, trk2dictionary_c.cpp:896-911) and the blur
 replicas (trk2dictionary.pyx:210-239).  No GPU needed."""
 import numpy as np
 import pytest
@@ -21,7 +21,7 @@ def test_trk_round_trip(tmp_path):
 
 
 def test_trk_with_scalars_and_properties(tmp_path):
-    """Per-point scalars and per-streamline properties are skipped (read_fiberTRK, :851-873)."""
+    """Per-point scalars and per-streamline properties are skipped (read_fiberTRK, :873-893)."""
     path = str(tmp_path / "s.trk")
     hdr = bytearray(1000)
     hdr[0:6] = b"TRACK\0"
