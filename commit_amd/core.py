@@ -592,7 +592,7 @@ class Evaluation:
             D = self.DICTIONARY
             confidence_array = cm[D["MASK_ix"], D["MASK_iy"], D["MASK_iz"], :].flatten().astype(np.float32)
             changed = False
-            if np.isnan(confidence_array).any():               # (core.pyx:1434-1437)
+            if np.isnan(confidence_array).any():               # (core.pyx:1433-1437)
                 confidence_array[np.isnan(confidence_array)] = 0.0
                 changed = True
             cmax, cmin = confidence_array.max(), confidence_array.min()

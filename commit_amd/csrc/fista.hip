@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_prox_elem(ProxArgs P, int64_t n, con
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double v = (double)xin[i];
-        if (grad) v = v - step * (double)grad[i];                         // (solvers.py:300)
+        if (grad) v = v - step * (double)grad[i];                         // (solvers.py:299)
         for (int c = 0; c < 3; c++) {
             if (i < P.start[c] || i >= P.end[c]) continue;
             const int r = P.reg[c];
@@ -241,7 +241,7 @@ int prox_impl(const commit_hip_prox_params *p, int64_t n, const T *xin, const T 
     const int r = p->reg[0];
     if (r == COMMIT_HIP_REG_GROUP_LASSO || r == COMMIT_HIP_REG_SPARSE_GROUP_LASSO) {
         const double lam = p->lam_group * scale;
-        if (lam != 0.0 && p->ngroups > 0) {          // proximals.pyx:115
+        if (lam != 0.0 && p->ngroups > 0) {          // proximals.pyx:116-117
             if (!p->group_ptr || !p->group_idx || !p->group_w) return fail(COMMIT_HIP_EINVAL, "group arrays missing");
             if (p->groups_overlap)
                 hipLaunchKernelGGL(k_group_prox_seq<T>, dim3(1), dim3(kBlock), 0, st, p->ngroups, p->group_ptr,

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_mprox_elem(MProx P, Seg G, int64_t n
         const Col &c = col[G.segx[i] * R + r];
         if (c.done) continue;
         const double step = c.step;
-        double v = (double)xhat[t] - step * (double)grad[t];             // (solvers.py:300)
+        double v = (double)xhat[t] - step * (double)grad[t];             // (solvers.py:299)
         for (int k = 0; k < 3; k++) {
             if (i < P.start[k] || i >= P.end[k]) continue;
             const int rg = P.reg[k];
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kGroupBlock) void k_mprox_group(MProx P, Seg G, flo
     const Col *c = col + (size_t)P.gseg[g] * R + r;
     if (c->done) return;
     const double lam = c->lam_group * c->step;
-    if (lam == 0.0) return;                                               // (proximals.pyx:115)
+    if (lam == 0.0) return;                                               // (proximals.pyx:116)
     const double nrm = sqrt(s);
     const double wl = P.gw[g] * lam;
     const double f = (nrm <= wl) ? 0.0 : (nrm - wl) / nrm;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_mresid_stage2(const double *__restri
     }
 }
 
-// Omega of one problem from its sums (solvers.py:202 and the compartment omegas)
+// Omega of one problem from its sums (solvers.py:200 and the compartment omegas)
 __device__ double omega_of(const Col &c, const double *s, const int *reg)
 {
     double om = 0.0;

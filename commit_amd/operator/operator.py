@@ -158,7 +158,7 @@ class LinearOperator:
         self._plan.set_precision(precision)
 
     def set_eval(self, ICeval):
-        """Update DICTIONARY['IC']['eval'] on the device (debias, core.pyx:1481-1487)."""
+        """Update DICTIONARY['IC']['eval'] on the device (debias, core.pyx:1482-1487)."""
         self._plan.set_eval(ICeval)
         self.DICTIONARY["IC"]["eval"] = np.ascontiguousarray(ICeval, dtype=np.uint32)
 

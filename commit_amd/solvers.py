@@ -143,7 +143,7 @@ class DeviceRegularisation:
         return self.out.cpu().numpy() if out is None else None
 
     def omega_from(self, s):
-        """Omega(x) from the sums (solvers.py:202 and the per-compartment omegas)."""
+        """Omega(x) from the sums (solvers.py:200 and the per-compartment omegas)."""
         om = 0.0
         for kind, a, b in self.omega_terms:
             if kind == "l1":

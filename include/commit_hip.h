@@ -65,7 +65,7 @@ int commit_hip_plan_create(commit_hip_plan **plan, int nS, int ndirs, int nF, in
 /* Free every device buffer of the plan. */
 int commit_hip_plan_destroy(commit_hip_plan *plan);
 
-/* Replace ICeval (debias mask, core.pyx:1481-1487); host pointer, nF*nIC entries. */
+/* Replace ICeval (debias mask, core.pyx:1482-1487); host pointer, nF*nIC entries. */
 int commit_hip_plan_set_eval(commit_hip_plan *plan, const uint32_t *ICeval);
 
 /* Sizes: rows = nVOX*nS, cols = nF*nIC + nE*nEC + nV*nISO (operator.pyx:69-70). */

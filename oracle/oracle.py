@@ -23,7 +23,7 @@ CPU restatement of the reference algorithm for the operator + FISTA hot path:
 * ``lambda_max_lasso`` / ``lambda_max_group``  <- set_regularisation  core.pyx:961-976
 * ``group_weights``         <- set_regularisation             core.pyx:1123-1192
 * ``debias_eval`` / ``debias_rows``  <- fit (debias)          core.pyx:1476-1501
-* ``confidence``            <- fit (confidence map)           core.pyx:1434-1457
+* ``confidence``            <- fit (confidence map)           core.pyx:1433-1457
 * ``fit_errors`` / ``compartment_maps``  <- save_results      core.pyx:1614-1756
 * ``iso_weights``           <- set_wLasso_ISO                 core.pyx:828-843
 
@@ -708,7 +708,7 @@ def group_weights(group_idx, kept, nSTR, extra=None, cardinality=True, adaptive=
 
 
 def debias_eval(x, offset):
-    """The debias step's IC column mask (core.pyx:1481-1483): 0 where
+    """The debias step's IC column mask (core.pyx:1482-1485): 0 where
     x < 1e-15."""
     mask = np.ones(offset, dtype=np.uint32)
     mask[x[:offset] < 0.000000000000001] = 0
@@ -729,7 +729,7 @@ def debias_rows(A_masked, x, mask, offset):
 
 
 def confidence(values, rescale):
-    """Confidence values of fit (core.pyx:1434-1457): NaN -> 0; a constant
+    """Confidence values of fit (core.pyx:1433-1457): NaN -> 0; a constant
     map is dropped (None); rescale maps [min, max] to [0, 1].  Returns
     (array or None, whether the values were changed)."""
     a = values.astype(np.float32).copy()

@@ -11,7 +11,7 @@ the oracle's restatements on the same inputs:
   * the debias masks and refit (core.pyx:1476-1508);
   * the error maps, adjusted maps, compartment maps and streamline weights
     written by save_results (core.pyx:1614-1790), with a confidence map that
-    holds NaNs and is rescaled (core.pyx:1434-1457);
+    holds NaNs and is rescaled (core.pyx:1433-1457);
   * set_wLasso_ISO weights and its weighted-lasso fit (core.pyx:768-858)."""
 import os
 import pickle
@@ -229,7 +229,7 @@ def _check_maps(mit, conf_values=None):
         got = _at_mask(mit, os.path.join(out, f"compartment_{name}.nii.gz"))
         assert np.allclose(got, m, rtol=1e-6, atol=1e-12), name
     # streamline weights: renormalised IC coefficients summed over RFs, 0 for dropped streamlines
-    # (core.pyx:1541-1548, 1763-1790)
+    # (core.pyx:1542-1548, 1763-1790)
     w = np.loadtxt(os.path.join(out, "streamline_weights.txt"))
     x_ren = mit.x[:D["IC"]["nSTR"] * K["wmr"].shape[0]] / np.kron(K["wmr_norm"], D["TRK"]["norm"])
     exp = np.zeros(D["TRK"]["kept"].size)

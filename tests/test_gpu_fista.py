@@ -113,7 +113,7 @@ def test_group_penalties(problem, kind):
 
 
 def test_overlapping_groups_prox_order():
-    """Hierarchical groups are applied in the given order (proximals.pyx:115-133)."""
+    """Hierarchical groups are applied in the given order (proximals.pyx:116-134)."""
     rng = np.random.default_rng(8)
     n = 50
     groups = np.empty(3, dtype=object)

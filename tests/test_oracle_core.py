@@ -1,7 +1,7 @@
 """Pins the oracle's restatements of the Evaluation pieces around the fit
 (core.pyx) on hand-computed examples: lambda_max (core.pyx:961-976), the
 group structure and weights (core.pyx:1123-1192), the debias masks
-(core.pyx:1476-1501), the confidence values (core.pyx:1434-1457), the fit
+(core.pyx:1476-1501), the confidence values (core.pyx:1433-1457), the fit
 error maps and statistics (core.pyx:1614-1707), the compartment maps
 (core.pyx:1709-1756) and the set_wLasso_ISO weights (core.pyx:828-843).
 Each expected value is worked out by hand in the comments."""
