@@ -12,7 +12,8 @@
 //   k_vox_walk<true>    the same walk again, storing each piece as
 //                       key = o * nvox + v (v = x + X(y + Y z), so ascending
 //                       keys are the std::map<segKey> order (o, z, y, x), :38-41)
-//                       and its double value w * len, in recording order
+//                       and its value w * len, in recording order (32-bit keys
+//                       when they fit, float values when w is 1: ic_segments)
 //   segmented sort      stable radix sort of each streamline's pieces by key
 //   k_vox_merge         equal keys summed into a float in recording order
 //                       (the map's `+=`, :804), kept flag (:510-515)
@@ -131,17 +132,19 @@ __device__ __forceinline__ bool raybox(D3 o, D3 d, D3 vmin, D3 vmax, double &t)
     return true;
 }
 
-// One streamline's recorder: counts pieces (EMIT = false) or stores them.
-template <bool EMIT>
+// One streamline's recorder: counts pieces (EMIT = false) or stores them,
+// keys as KT (32 bits when every key fits) and values w * len as VT (float
+// when w is 1, so that w * len is the float length itself, exactly).
+template <bool EMIT, class KT, class VT>
 struct Recorder {
     const VoxArgs &a;
-    uint64_t *keys;
-    double *vals;
+    KT *keys;
+    VT *vals;
     uint32_t limit;                    // the count pass's total: stores never pass it
     uint32_t n = 0;
     float flen = 0.0f, flentot = 0.0f;
 
-    __device__ Recorder(const VoxArgs &a_, uint64_t *k, double *v, uint32_t lim)
+    __device__ Recorder(const VoxArgs &a_, KT *k, VT *v, uint32_t lim)
         : a(a_), keys(k), vals(v), limit(lim) {}
 
     // segmentForwardModel (trk2dictionary_c.cpp:761-808)
@@ -170,8 +173,8 @@ struct Recorder {
             double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
             unsigned short o = (unsigned short)hash_index(a.hash, colat, lon);
             uint64_t v = (uint64_t)vx + (uint64_t)a.X * ((uint64_t)vy + (uint64_t)a.Y * vz);
-            keys[n] = (uint64_t)o * a.nvox + v;
-            vals[n] = w * len;
+            keys[n] = (KT)((uint64_t)o * a.nvox + v);
+            vals[n] = (VT)(w * len);
         }
         n++;
         flentot = (float)((double)flentot + w * len);
@@ -181,11 +184,11 @@ struct Recorder {
 
 // fiberForwardModel (trk2dictionary_c.cpp:584-755) for streamline f.  MULTI:
 // more than one replica, their positions kept in Pbuf [nF][n_rep].
-template <bool EMIT, bool MULTI>
+template <bool EMIT, bool MULTI, class KT, class VT>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *__restrict__ pts,
                                                         const int64_t *__restrict__ fptr, int nF,
                                                         D3 *__restrict__ Pbuf, const uint64_t *__restrict__ off,
-                                                        uint64_t *__restrict__ keys, double *__restrict__ vals,
+                                                        KT *__restrict__ keys, VT *__restrict__ vals,
                                                         uint32_t *__restrict__ cnt, float *__restrict__ flen,
                                                         float *__restrict__ flentot, uint32_t *__restrict__ status,
                                                         const uint32_t *__restrict__ order)
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
     const int t = blockIdx.x * kVoxBlock + threadIdx.x;
     if (t >= nF) return;
     const int f = (int)order[t];
-    Recorder<EMIT> rec(a, EMIT ? keys + off[f] : nullptr, EMIT ? vals + off[f] : nullptr,
+    Recorder<EMIT, KT, VT> rec(a, EMIT ? keys + off[f] : nullptr, EMIT ? vals + off[f] : nullptr,
                        EMIT ? (uint32_t)(off[f + 1] - off[f]) : 0u);
     const int64_t p0 = fptr[f];
     const int pts_n = (int)(fptr[f + 1] - p0);
@@ -318,13 +321,14 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_order_keys(const int64_t *__r
 // wavefront per streamline: each lane of a 64-piece chunk that starts a run
 // of equal keys adds the run in order and stores it at its rank among the
 // run heads.
+template <class KT, class VT>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restrict__ off, int nF,
-                                                         const uint64_t *__restrict__ skey,
-                                                         const double *__restrict__ sval, uint64_t nvox,
-                                                         const float *__restrict__ flen, float min_fib, float max_fib,
-                                                         uint64_t *__restrict__ mkey, float *__restrict__ mlen,
-                                                         uint64_t *__restrict__ vkey, uint32_t *__restrict__ mcnt,
-                                                         uint64_t *__restrict__ mend, uint32_t *__restrict__ kept)
+                                                         const KT *__restrict__ skey, const VT *__restrict__ sval,
+                                                         uint64_t nvox, const float *__restrict__ flen, float min_fib,
+                                                         float max_fib, KT *__restrict__ mkey,
+                                                         float *__restrict__ mlen, uint32_t *__restrict__ vkey,
+                                                         uint32_t *__restrict__ mcnt, uint64_t *__restrict__ mend,
+                                                         uint32_t *__restrict__ kept)
 {
     const int f = blockIdx.x * (kVoxBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
@@ -334,16 +338,16 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restr
     for (uint64_t base = b; base < e; base += kWave) {
         const uint64_t i = base + lane;
         const bool valid = i < e;
-        const uint64_t k = valid ? skey[i] : 0;
+        const KT k = valid ? skey[i] : (KT)0;
         const bool head = valid && (i == b || skey[i - 1] != k);
         const uint64_t heads = __ballot(head);
         if (head) {
             float acc = 0.0f;
-            for (uint64_t j = i; j < e && skey[j] == k; j++) acc = (float)((double)acc + sval[j]);
+            for (uint64_t j = i; j < e && skey[j] == k; j++) acc = (float)((double)acc + (double)sval[j]);
             const uint64_t pos = b + m + __popcll(heads & ((1ull << lane) - 1ull));
             mkey[pos] = k;
             mlen[pos] = acc;
-            vkey[pos] = k % nvox;
+            vkey[pos] = (uint32_t)((uint64_t)k % nvox);
         }
         m += __popcll(heads);
     }
@@ -358,7 +362,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restr
 // TRK_norm of each kept streamline from its segments sorted by voxel.
 __global__ __launch_bounds__(kVoxBlock) void k_vox_norm(const uint64_t *__restrict__ off,
                                                         const uint64_t *__restrict__ mend, int nF,
-                                                        const uint64_t *__restrict__ vsort,
+                                                        const uint32_t *__restrict__ vsort,
                                                         const float *__restrict__ lsort, float *__restrict__ norm)
 {
     const int f = blockIdx.x * kVoxBlock + threadIdx.x;
@@ -377,9 +381,10 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_norm(const uint64_t *__restri
 }
 
 // Kept streamlines' segments and per-streamline values, in streamline order.
+template <class KT>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_write(
     const uint64_t *__restrict__ off, int nF, const uint32_t *__restrict__ kept, const uint64_t *__restrict__ kidx,
-    const uint64_t *__restrict__ spos, const uint32_t *__restrict__ mcnt, const uint64_t *__restrict__ mkey,
+    const uint64_t *__restrict__ spos, const uint32_t *__restrict__ mcnt, const KT *__restrict__ mkey,
     const float *__restrict__ mlen, uint64_t nvox, const float *__restrict__ norm, const float *__restrict__ flen,
     const float *__restrict__ flentot, uint32_t *__restrict__ IC_f, uint32_t *__restrict__ IC_v,
     uint16_t *__restrict__ IC_o, float *__restrict__ IC_len, float *__restrict__ TRK_norm,
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_write(
     const uint64_t kk = kidx[f], s = spos[f], b = off[f];
     const uint32_t m = mcnt[f];
     for (uint32_t i = lane; i < m; i += kWave) {
-        const uint64_t key = mkey[b + i];
+        const uint64_t key = (uint64_t)mkey[b + i];
         const uint64_t o = key / nvox;
         IC_f[s + i] = (uint32_t)kk;
         IC_v[s + i] = (uint32_t)(key - o * nvox);
@@ -675,6 +680,112 @@ int upload(commit_hip_vox *V, int i, const T *host, size_t count, hipStream_t s)
     return 0;
 }
 
+// Arguments of the IC stage (ic_segments).
+struct IcArgs {
+    VoxArgs a;
+    const float *pts;
+    const int64_t *ptr;
+    int n_count;
+    D3 *P;
+    const uint64_t *off;
+    uint32_t *status;
+    const uint32_t *order;
+    size_t np;
+    int kbits, vbits;
+    float min_fib, max_fib;
+    hipStream_t s;
+};
+
+// The IC stage from the recording walk to the written segments, with piece
+// keys as KT and piece values as VT (see Recorder): walk, sort of each
+// streamline's pieces by key, merge, sort of each kept streamline's segments
+// by voxel, TRK_norm, renumbering and the IC / TRK outputs.
+template <class KT, class VT>
+int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n_kept_out)
+{
+    using B = commit_hip_vox;
+    const VoxArgs &a = ia.a;
+    const hipStream_t s = ia.s;
+    const int n_count = ia.n_count;
+    const size_t nF = (size_t)n_count, np = ia.np;
+    const uint64_t nvox = a.nvox;
+    const uint64_t *d_off = ia.off;
+    const bool multi = a.n_rep > 1;
+    const unsigned gF = grid_for(nF ? nF : 1);
+    const unsigned gW = (unsigned)((nF + kVoxBlock / kWave - 1) / (kVoxBlock / kWave));   // a wavefront per streamline
+    // V1 later holds the 32-bit voxel keys, V2 the float lengths
+    const size_t vb = np * (sizeof(VT) > 4 ? sizeof(VT) : 4);
+    int rc;
+    if ((rc = V->need(B::K1, np * sizeof(KT))) || (rc = V->need(B::V1, vb)) || (rc = V->need(B::K2, np * sizeof(KT))) ||
+        (rc = V->need(B::V2, vb)) || (rc = V->need(B::L1, np * 4)))
+        return rc;
+    KT *K1 = V->at<KT>(B::K1), *K2 = V->at<KT>(B::K2);
+    VT *V1 = V->at<VT>(B::V1), *V2 = V->at<VT>(B::V2);
+    float *L1 = V->at<float>(B::L1);
+    if (nF) {
+        if (multi)
+            k_vox_walk<true, true, KT, VT><<<gF, kVoxBlock, 0, s>>>(a, ia.pts, ia.ptr, n_count, ia.P, d_off, K1, V1,
+                                                                    nullptr, nullptr, nullptr, ia.status, ia.order);
+        else
+            k_vox_walk<true, false, KT, VT><<<gF, kVoxBlock, 0, s>>>(a, ia.pts, ia.ptr, n_count, nullptr, d_off, K1,
+                                                                     V1, nullptr, nullptr, nullptr, ia.status,
+                                                                     ia.order);
+        HIP_TRY(hipGetLastError());
+    }
+    uint32_t hstatus = 0;
+    HIP_TRY(hipMemcpyAsync(&hstatus, ia.status, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (hstatus) return fail(COMMIT_HIP_EDEVICE, "the recording pass differs from the counting pass");
+    if ((rc = segsort(V, (const KT *)K1, K2, (const VT *)V1, V2, np, n_count, d_off, d_off + 1, ia.kbits, s)))
+        return rc;
+    // merged segments: keys to K1, float lengths to L1, voxel keys into V1's memory
+    uint32_t *VK = (uint32_t *)V1;
+    uint32_t *d_mcnt = V->at<uint32_t>(B::MCNT), *d_kept = V->at<uint32_t>(B::KEPT);
+    uint64_t *d_mend = V->at<uint64_t>(B::MEND);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_mcnt + nF, 1, 0u);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_kept + nF, 1, 0u);
+    if (nF) {
+        k_vox_merge<KT, VT><<<gW, kVoxBlock, 0, s>>>(d_off, n_count, K2, V2, nvox, V->at<float>(B::FLEN), ia.min_fib,
+                                                     ia.max_fib, K1, L1, VK, d_mcnt, d_mend, d_kept);
+        HIP_TRY(hipGetLastError());
+    }
+    // per-voxel order inside each kept streamline: voxel keys into K2's
+    // memory, lengths into V2's
+    uint32_t *VS = (uint32_t *)K2;
+    float *L2 = (float *)V2;
+    if ((rc = segsort(V, (const uint32_t *)VK, VS, (const float *)L1, L2, np, n_count, d_off, d_mend, ia.vbits, s)))
+        return rc;
+    if (nF) {
+        k_vox_norm<<<gF, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, VS, L2, V->at<float>(B::NORM));
+        HIP_TRY(hipGetLastError());
+    }
+    uint64_t *d_kidx = V->at<uint64_t>(B::KIDX), *d_spos = V->at<uint64_t>(B::SPOS);
+    if ((rc = exscan_u32(V, d_kept, d_kidx, nF + 1, s))) return rc;
+    if ((rc = exscan_u32(V, d_mcnt, d_spos, nF + 1, s))) return rc;
+    uint64_t tot[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&tot[0], d_kidx + nF, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&tot[1], d_spos + nF, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const size_t n_kept = (size_t)tot[0], n_ic = (size_t)tot[1];
+    if ((rc = V->need(B::IC_F, n_ic * 4)) || (rc = V->need(B::IC_V, n_ic * 4)) || (rc = V->need(B::IC_O, n_ic * 2)) ||
+        (rc = V->need(B::IC_LEN, n_ic * 4)) || (rc = V->need(B::TRK_NORM, n_kept * 4)) ||
+        (rc = V->need(B::TRK_LEN, n_kept * 4)) || (rc = V->need(B::TRK_LENTOT, n_kept * 4)) ||
+        (rc = V->need(B::TRK_KEPT, nF)) || (rc = V->need(B::TDI_V, n_ic * 4)) || (rc = V->need(B::TDI_L, n_ic * 4)) ||
+        (rc = V->need(B::TDI, nvox * 4)) || (rc = V->need(B::HAS, nvox)))
+        return rc;
+    if (nF) {
+        k_vox_write<KT><<<gW, kVoxBlock, 0, s>>>(
+            d_off, n_count, d_kept, d_kidx, d_spos, d_mcnt, K1, L1, nvox, V->at<float>(B::NORM), V->at<float>(B::FLEN),
+            V->at<float>(B::FLENTOT), V->at<uint32_t>(B::IC_F), V->at<uint32_t>(B::IC_V), V->at<uint16_t>(B::IC_O),
+            V->at<float>(B::IC_LEN), V->at<float>(B::TRK_NORM), V->at<float>(B::TRK_LEN), V->at<float>(B::TRK_LENTOT),
+            V->at<uint8_t>(B::TRK_KEPT));
+        HIP_TRY(hipGetLastError());
+    }
+    *n_ic_out = n_ic;
+    *n_kept_out = n_kept;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" int commit_hip_vox_create(commit_hip_vox **out, int device)
@@ -828,7 +939,6 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_cnt + nF, 1, 0u);
     D3 *d_P = multi ? V->at<D3>(B::PBUF) : nullptr;
     const unsigned gF = grid_for(nF ? nF : 1);
-    const unsigned gW = (unsigned)((nF + kVoxBlock / kWave - 1) / (kVoxBlock / kWave));   // a wavefront per streamline
     if ((rc = V->need(B::OKEY, nF * 4)) || (rc = V->need(B::OIDX, nF * 4)) || (rc = V->need(B::OKEY2, nF * 4)) ||
         (rc = V->need(B::ORDER, nF * 4)))
         return rc;
@@ -840,11 +950,11 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
                                  (const uint32_t *)V->at<uint32_t>(B::OIDX), V->at<uint32_t>(B::ORDER), nF, 14, s)))
             return rc;
         if (multi)
-            k_vox_walk<false, true><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, nullptr, nullptr, nullptr,
+            k_vox_walk<false, true, uint64_t, double><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, nullptr, nullptr, nullptr,
                                                              d_cnt, V->at<float>(B::FLEN), V->at<float>(B::FLENTOT),
                                                              d_status, d_order);
         else
-            k_vox_walk<false, false><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, nullptr, nullptr,
+            k_vox_walk<false, false, uint64_t, double><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, nullptr, nullptr,
                                                               nullptr, d_cnt, V->at<float>(B::FLEN),
                                                               V->at<float>(B::FLENTOT), d_status, d_order);
         HIP_TRY(hipGetLastError());
@@ -860,69 +970,20 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
         return fail(COMMIT_HIP_EUNSUPPORTED, "%llu pieces exceed the 32-bit sort size", (unsigned long long)npieces);
     V->n_pieces = (int64_t)npieces;
     const size_t np = (size_t)npieces;
-    if ((rc = V->need(B::K1, np * 8)) || (rc = V->need(B::V1, np * 8)) || (rc = V->need(B::K2, np * 8)) ||
-        (rc = V->need(B::V2, np * 8)) || (rc = V->need(B::L1, np * 4)))
-        return rc;
-    uint64_t *K1 = V->at<uint64_t>(B::K1), *K2 = V->at<uint64_t>(B::K2);
-    double *V1 = V->at<double>(B::V1), *V2 = V->at<double>(B::V2);
-    float *L1 = V->at<float>(B::L1);
-    if (nF) {
-        if (multi)
-            k_vox_walk<true, true><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, d_off, K1, V1, nullptr,
-                                                            nullptr, nullptr, d_status, d_order);
-        else
-            k_vox_walk<true, false><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, d_off, K1, V1,
-                                                             nullptr, nullptr, nullptr, d_status, d_order);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipMemcpyAsync(&hstatus, d_status, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (hstatus) return fail(COMMIT_HIP_EDEVICE, "the recording pass differs from the counting pass");
     const int kbits = bit_length((uint64_t)(maxo + 1) * nvox - 1);
-    if ((rc = segsort(V, K1, K2, V1, V2, np, n_count, d_off, d_off + 1, kbits > 0 ? kbits : 1, s))) return rc;
-    // merged segments: keys to K1, float lengths to L1, voxel keys into V1's memory
-    uint64_t *VK = (uint64_t *)V1;
-    uint32_t *d_mcnt = V->at<uint32_t>(B::MCNT), *d_kept = V->at<uint32_t>(B::KEPT);
-    uint64_t *d_mend = V->at<uint64_t>(B::MEND);
-    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_mcnt + nF, 1, 0u);
-    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_kept + nF, 1, 0u);
-    if (nF) {
-        k_vox_merge<<<gW, kVoxBlock, 0, s>>>(d_off, n_count, K2, V2, nvox, V->at<float>(B::FLEN), min_fiber_len,
-                                             max_fiber_len, K1, L1, VK, d_mcnt, d_mend, d_kept);
-        HIP_TRY(hipGetLastError());
-    }
-    // per-voxel order inside each kept streamline: voxel keys to K2, lengths to V2's memory
-    float *L2 = (float *)V2;
     const int vbits = bit_length(nvox - 1);
-    if ((rc = segsort(V, (const uint64_t *)VK, K2, (const float *)L1, L2, np, n_count, d_off, d_mend,
-                      vbits > 0 ? vbits : 1, s)))
-        return rc;
-    if (nF) {
-        k_vox_norm<<<gF, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, K2, L2, V->at<float>(B::NORM));
-        HIP_TRY(hipGetLastError());
-    }
-    uint64_t *d_kidx = V->at<uint64_t>(B::KIDX), *d_spos = V->at<uint64_t>(B::SPOS);
-    if ((rc = exscan_u32(V, d_kept, d_kidx, nF + 1, s))) return rc;
-    if ((rc = exscan_u32(V, d_mcnt, d_spos, nF + 1, s))) return rc;
-    uint64_t tot[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(&tot[0], d_kidx + nF, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&tot[1], d_spos + nF, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const size_t n_kept = (size_t)tot[0], n_ic = (size_t)tot[1];
-    if ((rc = V->need(B::IC_F, n_ic * 4)) || (rc = V->need(B::IC_V, n_ic * 4)) || (rc = V->need(B::IC_O, n_ic * 2)) ||
-        (rc = V->need(B::IC_LEN, n_ic * 4)) || (rc = V->need(B::TRK_NORM, n_kept * 4)) ||
-        (rc = V->need(B::TRK_LEN, n_kept * 4)) || (rc = V->need(B::TRK_LENTOT, n_kept * 4)) ||
-        (rc = V->need(B::TRK_KEPT, nF)) || (rc = V->need(B::TDI_V, n_ic * 4)) || (rc = V->need(B::TDI_L, n_ic * 4)) ||
-        (rc = V->need(B::TDI, nvox * 4)) || (rc = V->need(B::HAS, nvox)))
-        return rc;
-    if (nF) {
-        k_vox_write<<<gW, kVoxBlock, 0, s>>>(
-            d_off, n_count, d_kept, d_kidx, d_spos, d_mcnt, K1, L1, nvox, V->at<float>(B::NORM), V->at<float>(B::FLEN),
-            V->at<float>(B::FLENTOT), V->at<uint32_t>(B::IC_F), V->at<uint32_t>(B::IC_V), V->at<uint16_t>(B::IC_O),
-            V->at<float>(B::IC_LEN), V->at<float>(B::TRK_NORM), V->at<float>(B::TRK_LEN), V->at<float>(B::TRK_LENTOT),
-            V->at<uint8_t>(B::TRK_KEPT));
-        HIP_TRY(hipGetLastError());
-    }
+    // 32-bit keys when every key fits; float piece values when the one replica
+    // has weight 1 (then w * len is the float length, exactly)
+    const bool narrow = kbits <= 32;
+    const bool fval = nReplicas == 1 && ptrBlurWeights[0] == 1.0;
+    size_t n_ic = 0, n_kept = 0;
+    IcArgs ia{a, d_pts, d_ptr, n_count, d_P, d_off, d_status, d_order, np, kbits > 0 ? kbits : 1,
+              vbits > 0 ? vbits : 1, min_fiber_len, max_fiber_len, s};
+    rc = narrow ? (fval ? ic_segments<uint32_t, float>(V, ia, &n_ic, &n_kept)
+                        : ic_segments<uint32_t, double>(V, ia, &n_ic, &n_kept))
+                : (fval ? ic_segments<uint64_t, float>(V, ia, &n_ic, &n_kept)
+                        : ic_segments<uint64_t, double>(V, ia, &n_ic, &n_kept));
+    if (rc) return rc;
     // TDI and the voxels holding IC segments
     k_fill_f32<<<grid_for(nvox), kVoxBlock, 0, s>>>(V->at<float>(B::TDI), nvox, 0.0f);
     k_fill_u8<<<grid_for(nvox), kVoxBlock, 0, s>>>(V->at<uint8_t>(B::HAS), nvox, 0);

@@ -76,6 +76,27 @@ def test_blur_replicas(small):
     same(g, o)
 
 
+@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("weight", [1.0, 0.75])
+def test_key_and_value_widths(wide, weight):
+    """The device keeps a piece's key in 32 bits when (largest orientation + 1)
+    x voxels fits, and its value w * len as a float when the one replica has
+    weight 1; each of the four forms against the oracle.  Orientations above
+    32000 on a 163,840-voxel grid need 33-bit keys."""
+    dim = (64, 64, 40)
+    pts, ptr = synth.streamlines(1007, 3000, dim, 1.0, mode=0, step_vox=0.45)
+    h = (H.astype(np.int32) + (32000 if wide else 0)).astype(np.int16)
+    kbits = int((int(h.max()) + 1) * int(np.prod(dim)) - 1).bit_length()
+    assert (kbits > 32) == wide
+    kw = {} if weight == 1.0 else {"blur": (np.zeros(1), np.zeros(1), np.array([weight]))}
+    print(f"voxelise {len(ptr) - 1} streamlines, grid {dim}, {kbits}-bit keys, weight {weight}", flush=True)
+    g = t2d.voxelize(pts, ptr, dim, (1.0, 1.0, 1.0), hash_table=h, **kw)
+    o = vo.trk2dictionary(pts, ptr, dim, (1.0, 1.0, 1.0), h, **kw)
+    assert o["status"] == 0 and g["IC_v"].size > 10_000
+    assert (g["IC_o"].min() >= 32000) == wide
+    same(g, o, (wide, weight))
+
+
 def test_peaks_iso_flips_affine(small):
     pts, ptr, dim = small
     rng = np.random.default_rng(5)
