@@ -3,6 +3,9 @@
 // streamlines already read into memory.
 //
 // Pipeline (one stream, all device-resident):
+//   (one replica: k_vox_bound sizes each streamline's region, and a single
+//   k_vox_walk<true, .., BOUNDED> records and counts; a streamline that
+//   outgrows its region sends the tractogram through the two walks below)
 //   k_vox_walk<false>   one thread per streamline runs fiberForwardModel
 //                       (:584-755) with its replicas, rayBoxIntersection
 //                       (:814-869) and segmentForwardModel (:761-808), and
@@ -184,7 +187,14 @@ struct Recorder {
 
 // fiberForwardModel (trk2dictionary_c.cpp:584-755) for streamline f.  MULTI:
 // more than one replica, their positions kept in Pbuf [nF][n_rep].
-template <bool EMIT, bool MULTI, class KT, class VT>
+//   EMIT = false             counts the pieces (cnt, FiberLen, FiberLenTot)
+//   EMIT, !BOUNDED           stores exactly the counted pieces
+//   EMIT, BOUNDED            stores the pieces into a region of at most
+//                            off[f+1] - off[f] slots and counts them too; a
+//                            streamline with more pieces sets status[2]
+// status[0] = 1: a crossing loop hit its cap, or the recording pass found
+// other counts than the counting pass.
+template <bool EMIT, bool MULTI, class KT, class VT, bool BOUNDED = false>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *__restrict__ pts,
                                                         const int64_t *__restrict__ fptr, int nF,
                                                         D3 *__restrict__ Pbuf, const uint64_t *__restrict__ off,
@@ -295,12 +305,34 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
             }
         }
     }
-    if (!EMIT) {
+    if (!EMIT || BOUNDED) {
         cnt[f] = rec.n;
         flen[f] = rec.flen;
         flentot[f] = rec.flentot;
     }
-    if (capped || (EMIT && rec.n != rec.limit)) *status = 1;
+    if (capped || (EMIT && !BOUNDED && rec.n != rec.limit)) status[0] = 1;
+    if (BOUNDED && rec.n > rec.limit) status[2] = 1;
+}
+
+// Slots of streamline f for the one-pass walk of a single replica: pieces of
+// one polyline step are one more than the voxel boundaries it crosses, at most
+// 4 for a step under a voxel per axis; 16 more for the rest.
+__global__ __launch_bounds__(kVoxBlock) void k_vox_bound(const int64_t *__restrict__ fptr, int nF,
+                                                         uint32_t *__restrict__ bnd)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f >= nF) return;
+    const int64_t n = fptr[f + 1] - fptr[f];
+    bnd[f] = n > 2 ? (uint32_t)(4 * n + 16) : 0u;
+}
+
+// end[f] = off[f] + cnt[f]: the recorded part of streamline f's region
+__global__ __launch_bounds__(kVoxBlock) void k_vox_ends(const uint64_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ cnt, int nF,
+                                                        uint64_t *__restrict__ end)
+{
+    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    if (f < nF) end[f] = off[f] + cnt[f];
 }
 
 // Walk order: the streamlines of one wavefront take similar time when they
@@ -316,13 +348,15 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_order_keys(const int64_t *__r
     idx[f] = (uint32_t)f;
 }
 
-// Equal keys of a streamline's sorted pieces summed into a float in recording
-// order; m[f] merged segments stored from the streamline's first slot.  One
+// Equal keys of a streamline's sorted pieces [off[f], end[f]) summed into a
+// float in recording order; m[f] merged segments stored from the streamline's
+// first slot.  One
 // wavefront per streamline: each lane of a 64-piece chunk that starts a run
 // of equal keys adds the run in order and stores it at its rank among the
 // run heads.
 template <class KT, class VT>
-__global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restrict__ off, int nF,
+__global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restrict__ off,
+                                                         const uint64_t *__restrict__ end, int nF,
                                                          const KT *__restrict__ skey, const VT *__restrict__ sval,
                                                          uint64_t nvox, const float *__restrict__ flen, float min_fib,
                                                          float max_fib, KT *__restrict__ mkey,
@@ -333,7 +367,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restr
     const int f = blockIdx.x * (kVoxBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     if (f >= nF) return;
-    const uint64_t b = off[f], e = off[f + 1];
+    const uint64_t b = off[f], e = end[f];
     uint64_t m = 0;
     for (uint64_t base = b; base < e; base += kWave) {
         const uint64_t i = base + lane;
@@ -592,7 +626,7 @@ struct commit_hip_vox {
         PTS, FPTR, MASK, ISO, PEAKS, HASH, REP, APPLY, PBUF, CNT, FLEN, FLENTOT, OFF, K1, V1, K2, V2, L1,
         MCNT, MEND, KEPT, KIDX, SPOS, NORM, STATUS, TMP,
         IC_F, IC_V, IC_O, IC_LEN, TRK_NORM, TRK_LEN, TRK_LENTOT, TRK_KEPT, TDI_V, TDI_L, TDI, HAS,
-        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, OKEY, OIDX, OKEY2, ORDER, NBUF
+        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, OKEY, OIDX, OKEY2, ORDER, END, POFF, NBUF
     };
     Buf buf[NBUF];
     int64_t n_ic = 0, n_kept = 0, n_ec = 0, n_iso = 0, n_count = 0, n_pieces = 0;
@@ -680,6 +714,10 @@ int upload(commit_hip_vox *V, int i, const T *host, size_t count, hipStream_t s)
     return 0;
 }
 
+// ic_segments with a bounded walk: some streamline had more pieces than its
+// region holds, and the counting walk must size the regions (not an API code)
+constexpr int kRetryExact = 1;
+
 // Arguments of the IC stage (ic_segments).
 struct IcArgs {
     VoxArgs a;
@@ -687,10 +725,13 @@ struct IcArgs {
     const int64_t *ptr;
     int n_count;
     D3 *P;
-    const uint64_t *off;
+    const uint64_t *off;               // streamline regions start here
+    bool bounded;                      // one-pass walk into regions of off[f+1] - off[f] slots
+    uint32_t *cnt;                     // pieces per streamline (written by the one-pass walk)
+    float *flen, *flentot;
     uint32_t *status;
     const uint32_t *order;
-    size_t np;
+    size_t np;                         // slots of all regions
     int kbits, vbits;
     float min_fib, max_fib;
     hipStream_t s;
@@ -723,7 +764,11 @@ int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n
     VT *V1 = V->at<VT>(B::V1), *V2 = V->at<VT>(B::V2);
     float *L1 = V->at<float>(B::L1);
     if (nF) {
-        if (multi)
+        if (ia.bounded)
+            k_vox_walk<true, false, KT, VT, true><<<gF, kVoxBlock, 0, s>>>(a, ia.pts, ia.ptr, n_count, nullptr,
+                                                                           d_off, K1, V1, ia.cnt, ia.flen,
+                                                                           ia.flentot, ia.status, ia.order);
+        else if (multi)
             k_vox_walk<true, true, KT, VT><<<gF, kVoxBlock, 0, s>>>(a, ia.pts, ia.ptr, n_count, ia.P, d_off, K1, V1,
                                                                     nullptr, nullptr, nullptr, ia.status, ia.order);
         else
@@ -732,11 +777,29 @@ int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n
                                                                      ia.order);
         HIP_TRY(hipGetLastError());
     }
-    uint32_t hstatus = 0;
-    HIP_TRY(hipMemcpyAsync(&hstatus, ia.status, 4, hipMemcpyDeviceToHost, s));
+    uint32_t hst[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(hst, ia.status, 12, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (hstatus) return fail(COMMIT_HIP_EDEVICE, "the recording pass differs from the counting pass");
-    if ((rc = segsort(V, (const KT *)K1, K2, (const VT *)V1, V2, np, n_count, d_off, d_off + 1, ia.kbits, s)))
+    const uint64_t *d_end = d_off + 1;
+    if (ia.bounded) {
+        if (hst[0]) return fail(COMMIT_HIP_EUNSUPPORTED, "a polyline step crossed more voxels than its cap allows");
+        if (hst[2]) return kRetryExact;
+        // each region's recorded part, and the pieces in all
+        uint64_t *end = V->at<uint64_t>(B::END), *poff = V->at<uint64_t>(B::POFF);
+        if (nF) {
+            k_vox_ends<<<gF, kVoxBlock, 0, s>>>(d_off, ia.cnt, n_count, end);
+            HIP_TRY(hipGetLastError());
+        }
+        if ((rc = exscan_u32(V, ia.cnt, poff, nF + 1, s))) return rc;
+        uint64_t npieces = 0;
+        HIP_TRY(hipMemcpyAsync(&npieces, poff + nF, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        V->n_pieces = (int64_t)npieces;
+        d_end = end;
+    } else if (hst[0]) {
+        return fail(COMMIT_HIP_EDEVICE, "the recording pass differs from the counting pass");
+    }
+    if ((rc = segsort(V, (const KT *)K1, K2, (const VT *)V1, V2, np, n_count, d_off, d_end, ia.kbits, s)))
         return rc;
     // merged segments: keys to K1, float lengths to L1, voxel keys into V1's memory
     uint32_t *VK = (uint32_t *)V1;
@@ -745,8 +808,8 @@ int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_mcnt + nF, 1, 0u);
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_kept + nF, 1, 0u);
     if (nF) {
-        k_vox_merge<KT, VT><<<gW, kVoxBlock, 0, s>>>(d_off, n_count, K2, V2, nvox, V->at<float>(B::FLEN), ia.min_fib,
-                                                     ia.max_fib, K1, L1, VK, d_mcnt, d_mend, d_kept);
+        k_vox_merge<KT, VT><<<gW, kVoxBlock, 0, s>>>(d_off, d_end, n_count, K2, V2, nvox, V->at<float>(B::FLEN),
+                                                     ia.min_fib, ia.max_fib, K1, L1, VK, d_mcnt, d_mend, d_kept);
         HIP_TRY(hipGetLastError());
     }
     // per-voxel order inside each kept streamline: voxel keys into K2's
@@ -936,11 +999,12 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     uint32_t *d_cnt = V->at<uint32_t>(B::CNT);
     uint64_t *d_off = V->at<uint64_t>(B::OFF);
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_status, 1, 0u);
+    k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_status + 2, 1, 0u);
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_cnt + nF, 1, 0u);
     D3 *d_P = multi ? V->at<D3>(B::PBUF) : nullptr;
     const unsigned gF = grid_for(nF ? nF : 1);
     if ((rc = V->need(B::OKEY, nF * 4)) || (rc = V->need(B::OIDX, nF * 4)) || (rc = V->need(B::OKEY2, nF * 4)) ||
-        (rc = V->need(B::ORDER, nF * 4)))
+        (rc = V->need(B::ORDER, nF * 4)) || (rc = V->need(B::END, nF * 8)) || (rc = V->need(B::POFF, (nF + 1) * 8)))
         return rc;
     const uint32_t *d_order = V->at<uint32_t>(B::ORDER);
     if (nF) {
@@ -949,27 +1013,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
         if ((rc = sort_pairs_u32(V, (const uint32_t *)V->at<uint32_t>(B::OKEY), V->at<uint32_t>(B::OKEY2),
                                  (const uint32_t *)V->at<uint32_t>(B::OIDX), V->at<uint32_t>(B::ORDER), nF, 14, s)))
             return rc;
-        if (multi)
-            k_vox_walk<false, true, uint64_t, double><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, d_P, nullptr, nullptr, nullptr,
-                                                             d_cnt, V->at<float>(B::FLEN), V->at<float>(B::FLENTOT),
-                                                             d_status, d_order);
-        else
-            k_vox_walk<false, false, uint64_t, double><<<gF, kVoxBlock, 0, s>>>(a, d_pts, d_ptr, n_count, nullptr, nullptr, nullptr,
-                                                              nullptr, d_cnt, V->at<float>(B::FLEN),
-                                                              V->at<float>(B::FLENTOT), d_status, d_order);
-        HIP_TRY(hipGetLastError());
     }
-    if ((rc = exscan_u32(V, d_cnt, d_off, nF + 1, s))) return rc;
-    uint64_t npieces = 0;
-    uint32_t hstatus = 0;
-    HIP_TRY(hipMemcpyAsync(&npieces, d_off + nF, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&hstatus, d_status, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (hstatus) return fail(COMMIT_HIP_EUNSUPPORTED, "a polyline step crossed more voxels than its cap allows");
-    if (npieces >= 0xFFFFFFFFull)
-        return fail(COMMIT_HIP_EUNSUPPORTED, "%llu pieces exceed the 32-bit sort size", (unsigned long long)npieces);
-    V->n_pieces = (int64_t)npieces;
-    const size_t np = (size_t)npieces;
     const int kbits = bit_length((uint64_t)(maxo + 1) * nvox - 1);
     const int vbits = bit_length(nvox - 1);
     // 32-bit keys when every key fits; float piece values when the one replica
@@ -977,13 +1021,64 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     const bool narrow = kbits <= 32;
     const bool fval = nReplicas == 1 && ptrBlurWeights[0] == 1.0;
     size_t n_ic = 0, n_kept = 0;
-    IcArgs ia{a, d_pts, d_ptr, n_count, d_P, d_off, d_status, d_order, np, kbits > 0 ? kbits : 1,
-              vbits > 0 ? vbits : 1, min_fiber_len, max_fiber_len, s};
-    rc = narrow ? (fval ? ic_segments<uint32_t, float>(V, ia, &n_ic, &n_kept)
-                        : ic_segments<uint32_t, double>(V, ia, &n_ic, &n_kept))
-                : (fval ? ic_segments<uint64_t, float>(V, ia, &n_ic, &n_kept)
-                        : ic_segments<uint64_t, double>(V, ia, &n_ic, &n_kept));
-    if (rc) return rc;
+    IcArgs ia{a, d_pts, d_ptr, n_count, d_P, d_off, false, d_cnt, V->at<float>(B::FLEN), V->at<float>(B::FLENTOT),
+              d_status, d_order, 0, kbits > 0 ? kbits : 1, vbits > 0 ? vbits : 1, min_fiber_len, max_fiber_len, s};
+    auto ic = [&]() {
+        return narrow ? (fval ? ic_segments<uint32_t, float>(V, ia, &n_ic, &n_kept)
+                              : ic_segments<uint32_t, double>(V, ia, &n_ic, &n_kept))
+                      : (fval ? ic_segments<uint64_t, float>(V, ia, &n_ic, &n_kept)
+                              : ic_segments<uint64_t, double>(V, ia, &n_ic, &n_kept));
+    };
+    // One replica: a single walk into regions sized by k_vox_bound, when they
+    // fit in half the free memory; a streamline that outgrows its region sends
+    // the whole tractogram through the counting walk instead.
+    bool done = false;
+    if (!multi && nF) {
+        k_vox_bound<<<gF, kVoxBlock, 0, s>>>(d_ptr, n_count, d_cnt);
+        HIP_TRY(hipGetLastError());
+        if ((rc = exscan_u32(V, d_cnt, d_off, nF + 1, s))) return rc;
+        uint64_t slots = 0;
+        HIP_TRY(hipMemcpyAsync(&slots, d_off + nF, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        const size_t per_slot = 2 * (narrow ? 4 : 8) + 2 * (fval ? 4 : 8) + 4;
+        if (slots < 0xFFFFFFFFull && slots * per_slot <= free_b / 2) {
+            ia.bounded = true;
+            ia.np = (size_t)slots;
+            rc = ic();
+            if (rc == 0) done = true;
+            else if (rc != kRetryExact) return rc;
+            ia.bounded = false;
+        }
+    }
+    if (!done) {
+        k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_status, 1, 0u);
+        k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_status + 2, 1, 0u);
+        if (nF) {
+            if (multi)
+                k_vox_walk<false, true, uint64_t, double><<<gF, kVoxBlock, 0, s>>>(
+                    a, d_pts, d_ptr, n_count, d_P, nullptr, nullptr, nullptr, d_cnt, V->at<float>(B::FLEN),
+                    V->at<float>(B::FLENTOT), d_status, d_order);
+            else
+                k_vox_walk<false, false, uint64_t, double><<<gF, kVoxBlock, 0, s>>>(
+                    a, d_pts, d_ptr, n_count, nullptr, nullptr, nullptr, nullptr, d_cnt, V->at<float>(B::FLEN),
+                    V->at<float>(B::FLENTOT), d_status, d_order);
+            HIP_TRY(hipGetLastError());
+        }
+        if ((rc = exscan_u32(V, d_cnt, d_off, nF + 1, s))) return rc;
+        uint64_t npieces = 0;
+        uint32_t hstatus = 0;
+        HIP_TRY(hipMemcpyAsync(&npieces, d_off + nF, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&hstatus, d_status, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (hstatus) return fail(COMMIT_HIP_EUNSUPPORTED, "a polyline step crossed more voxels than its cap allows");
+        if (npieces >= 0xFFFFFFFFull)
+            return fail(COMMIT_HIP_EUNSUPPORTED, "%llu pieces exceed the 32-bit sort size", (unsigned long long)npieces);
+        V->n_pieces = (int64_t)npieces;
+        ia.np = (size_t)npieces;
+        if ((rc = ic())) return rc;
+    }
     // TDI and the voxels holding IC segments
     k_fill_f32<<<grid_for(nvox), kVoxBlock, 0, s>>>(V->at<float>(B::TDI), nvox, 0.0f);
     k_fill_u8<<<grid_for(nvox), kVoxBlock, 0, s>>>(V->at<uint8_t>(B::HAS), nvox, 0);

@@ -97,6 +97,21 @@ def test_key_and_value_widths(wide, weight):
     same(g, o, (wide, weight))
 
 
+def test_regions_outgrown():
+    """Single-replica tractograms are walked once into regions of 4 slots per
+    point plus 16; streamlines with steps of tens of voxels outgrow them, and
+    the tractogram goes through the counting walk instead: same outputs."""
+    dim = (64, 64, 64)
+    pts, ptr = synth.streamlines(1011, 500, dim, 1.0, mode=0, step_vox=0.5)
+    rng = np.random.default_rng(12)
+    longs = [rng.uniform(0.5, 63.5, size=(5, 3)).astype(np.float32) for _ in range(20)]
+    pts = np.concatenate([pts] + longs)
+    ptr = np.concatenate([ptr, ptr[-1] + 5 * np.arange(1, 21)]).astype(np.int64)
+    g, o = both(pts, ptr, dim, (1.0, 1.0, 1.0), max_fiber_len=1000.0)
+    assert g["IC_v"].size > 10_000
+    same(g, o)
+
+
 def test_peaks_iso_flips_affine(small):
     pts, ptr, dim = small
     rng = np.random.default_rng(5)
