@@ -23,7 +23,8 @@
 //   segmented sort      the merged segments of each streamline by voxel
 //                       (stable, so each voxel's segments stay in map order)
 //   k_vox_norm          TRK_norm: per-voxel float sums, float sum of their
-//                       squares in (z, y, x) order, float square root (:530-537)
+//                       squares in (z, y, x) order, float square root (:530-537),
+//                       a wavefront per streamline
 //   scans + k_vox_write kept streamlines renumbered (trk2dictionary.pyx:551-563)
 //                       and their segments written in streamline order
 //   radix sort + k_vox_tdi   compute_tdi (pyx:50-59): float sums per voxel in
@@ -393,25 +394,49 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_merge(const uint64_t *__restr
     kept[f] = kf;
 }
 
-// TRK_norm of each kept streamline from its segments sorted by voxel.
+__device__ __forceinline__ double readlane_f64(double x, int l)
+{
+    const long long u = __double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// TRK_norm of each kept streamline from its segments sorted by voxel,
+// [off[f], mend[f]) (:530-537): per voxel the float sum of its lengths in
+// order, then the float sum of their squares in voxel order and a float
+// square root.  A wavefront per streamline: the lane that starts a voxel's
+// run adds the run in order and squares it; the squares are then added one
+// by one in run order (readlane).
 __global__ __launch_bounds__(kVoxBlock) void k_vox_norm(const uint64_t *__restrict__ off,
                                                         const uint64_t *__restrict__ mend, int nF,
                                                         const uint32_t *__restrict__ vsort,
                                                         const float *__restrict__ lsort, float *__restrict__ norm)
 {
-    const int f = blockIdx.x * kVoxBlock + threadIdx.x;
+    const int f = blockIdx.x * (kVoxBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
     if (f >= nF) return;
     const uint64_t b = off[f], e = mend[f];
-    float fn = 0.0f, s = 0.0f;
-    for (uint64_t i = b; i < e; i++) {
-        if (i != b && vsort[i] != vsort[i - 1]) {
-            fn = (float)((double)fn + (double)s * (double)s);
-            s = 0.0f;
+    float fn = 0.0f;
+    for (uint64_t base = b; base < e; base += kWave) {
+        const uint64_t i = base + lane;
+        const bool valid = i < e;
+        const uint32_t v = valid ? vsort[i] : 0u;
+        const bool head = valid && (i == b || vsort[i - 1] != v);
+        double sq = 0.0;
+        if (head) {
+            float r = 0.0f;
+            for (uint64_t j = i; j < e && vsort[j] == v; j++) r = r + lsort[j];
+            sq = (double)r * (double)r;
         }
-        s = s + lsort[i];
+        unsigned long long hm = __ballot(head);
+        while (hm) {
+            const int l = (int)__builtin_ctzll(hm);
+            hm &= hm - 1ull;
+            fn = (float)((double)fn + readlane_f64(sq, l));
+        }
     }
-    if (e > b) fn = (float)((double)fn + (double)s * (double)s);
-    norm[f] = (float)sqrt((double)fn);
+    if (lane == 0) norm[f] = (float)sqrt((double)fn);
 }
 
 // Kept streamlines' segments and per-streamline values, in streamline order.
@@ -819,7 +844,7 @@ int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n
     if ((rc = segsort(V, (const uint32_t *)VK, VS, (const float *)L1, L2, np, n_count, d_off, d_mend, ia.vbits, s)))
         return rc;
     if (nF) {
-        k_vox_norm<<<gF, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, VS, L2, V->at<float>(B::NORM));
+        k_vox_norm<<<gW, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, VS, L2, V->at<float>(B::NORM));
         HIP_TRY(hipGetLastError());
     }
     uint64_t *d_kidx = V->at<uint64_t>(B::KIDX), *d_spos = V->at<uint64_t>(B::SPOS);
