@@ -237,55 +237,148 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
             if (MULTI) Pm[k] = s;
             else P0 = s;
         }
-        for (int i = 1; i < pts_n; i++) {
-            D3 S2 = {(double)(fib[3 * i] + a.sx), (double)(fib[3 * i + 1] + a.sy), (double)(fib[3 * i + 2] + a.sz)};
-            D3 dir1 = {S2.x - (double)(fib[3 * i - 3] + a.sx), S2.y - (double)(fib[3 * i - 2] + a.sy),
-                       S2.z - (double)(fib[3 * i - 1] + a.sz)};
-            dir1 = vnormalize(dir1);
-            if (i == pts_n - 1) {
-                dir2 = dir1;
-            } else {
-                dir2.x = (double)(fib[3 * i + 3] + a.sx) - S2.x;
-                dir2.y = (double)(fib[3 * i + 4] + a.sy) - S2.y;
-                dir2.z = (double)(fib[3 * i + 5] + a.sz) - S2.z;
-                dir2 = vnormalize(dir2);
+        if constexpr (MULTI) {
+            for (int i = 1; i < pts_n; i++) {
+                D3 S2 = {(double)(fib[3 * i] + a.sx), (double)(fib[3 * i + 1] + a.sy), (double)(fib[3 * i + 2] + a.sz)};
+                D3 dir1 = {S2.x - (double)(fib[3 * i - 3] + a.sx), S2.y - (double)(fib[3 * i - 2] + a.sy),
+                           S2.z - (double)(fib[3 * i - 1] + a.sz)};
+                dir1 = vnormalize(dir1);
+                if (i == pts_n - 1) {
+                    dir2 = dir1;
+                } else {
+                    dir2.x = (double)(fib[3 * i + 3] + a.sx) - S2.x;
+                    dir2.y = (double)(fib[3 * i + 4] + a.sy) - S2.y;
+                    dir2.z = (double)(fib[3 * i + 5] + a.sz) - S2.z;
+                    dir2 = vnormalize(dir2);
+                }
+                D3 nn = {0.5 * (dir1.x + dir2.x), 0.5 * (dir1.y + dir2.y), 0.5 * (dir1.z + dir2.z)};
+                double dot = dir1.x * nn.x + dir1.y * nn.y + dir1.z * nn.z;
+                nn.x /= dot;
+                nn.y /= dot;
+                nn.z /= dot;
+                for (int k = 0; k < a.n_rep; k++) {
+                    if (!blur && k > 0) continue;
+                    if (a.wt[k] < 1e-3) continue;
+                    const double wk = a.wt[k];
+                    D3 Po = MULTI ? Pm[k] : P0;
+                    double len = (S2.x - Po.x) * nn.x + (S2.y - Po.y) * nn.y + (S2.z - Po.z) * nn.z;
+                    if (!(len > 0)) continue;
+                    D3 Pn = {Po.x + dir1.x * len, Po.y + dir1.y * len, Po.z + dir1.z * len};
+                    if (MULTI) Pm[k] = Pn;
+                    else P0 = Pn;
+                    if (!a.intersect) {
+                        rec.segment(Po, Pn, k, wk);
+                        continue;
+                    }
+                    D3 S1m = Po;
+                    const D3 S2m = Pn;
+                    long long cap = -1;
+                    for (long long it = 0;; it++) {
+                        double dx = S2m.x - S1m.x, dy = S2m.y - S1m.y, dz = S2m.z - S1m.z;
+                        double sl = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (sl <= a.min_seg) break;
+                        if (cap < 0) cap = max_cross(sl, a);
+                        if (it >= cap) {
+                            capped = true;
+                            break;
+                        }
+                        if (floor(S1m.x / a.px) == floor(S2m.x / a.px) && floor(S1m.y / a.py) == floor(S2m.y / a.py) &&
+                            floor(S1m.z / a.pz) == floor(S2m.z / a.pz)) {
+                            rec.segment(S1m, S2m, k, wk);
+                            break;
+                        }
+                        D3 vmin, vmax;
+                        vmin.x = floor((S1m.x + 1e-6 * dir1.x) / a.px) * a.px;
+                        vmin.y = floor((S1m.y + 1e-6 * dir1.y) / a.py) * a.py;
+                        vmin.z = floor((S1m.z + 1e-6 * dir1.z) / a.pz) * a.pz;
+                        vmax.x = vmin.x + a.px;
+                        vmax.y = vmin.y + a.py;
+                        vmax.z = vmin.z + a.pz;
+                        double t;
+                        if (raybox(S1m, dir1, vmin, vmax, t) && t > 0 && t < sl) {
+                            D3 Pi = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
+                            rec.segment(S1m, Pi, k, wk);
+                            S1m = Pi;
+                        } else {
+                            rec.segment(S1m, S2m, k, wk);
+                            break;
+                        }
+                    }
+                }
             }
-            D3 nn = {0.5 * (dir1.x + dir2.x), 0.5 * (dir1.y + dir2.y), 0.5 * (dir1.z + dir2.z)};
-            double dot = dir1.x * nn.x + dir1.y * nn.y + dir1.z * nn.z;
-            nn.x /= dot;
-            nn.y /= dot;
-            nn.z /= dot;
-            for (int k = 0; k < a.n_rep; k++) {
-                if (!blur && k > 0) continue;
-                if (a.wt[k] < 1e-3) continue;
-                const double wk = a.wt[k];
-                D3 Po = MULTI ? Pm[k] : P0;
-                double len = (S2.x - Po.x) * nn.x + (S2.y - Po.y) * nn.y + (S2.z - Po.z) * nn.z;
-                if (!(len > 0)) continue;
-                D3 Pn = {Po.x + dir1.x * len, Po.y + dir1.y * len, Po.z + dir1.z * len};
-                if (MULTI) Pm[k] = Pn;
-                else P0 = Pn;
-                if (!a.intersect) {
-                    rec.segment(Po, Pn, k, wk);
+        } else if (!(a.wt[0] < 1e-3)) {
+            // One replica: the point loop and each step's voxel-crossing loop
+            // run as one loop of crossing iterations.  A lane takes its next
+            // polyline step only when its current one is done, so the lanes of
+            // a wavefront stay busy while their steps cross different numbers
+            // of voxels.  The operations and their order per streamline are
+            // those of the nested loops above.
+            const double wk = a.wt[0];
+            int i = 1;
+            bool cross = false;
+            D3 S1m = {0, 0, 0}, S2m = {0, 0, 0}, dir1 = {0, 0, 0};
+            double f2x = 0, f2y = 0, f2z = 0;               // floor(S2m / pixdim), fixed for a step
+            long long it = 0, cap = -1;
+            for (;;) {
+                if (!cross) {
+                    while (i < pts_n) {
+                        D3 S2 = {(double)(fib[3 * i] + a.sx), (double)(fib[3 * i + 1] + a.sy),
+                                 (double)(fib[3 * i + 2] + a.sz)};
+                        dir1 = {S2.x - (double)(fib[3 * i - 3] + a.sx), S2.y - (double)(fib[3 * i - 2] + a.sy),
+                                S2.z - (double)(fib[3 * i - 1] + a.sz)};
+                        dir1 = vnormalize(dir1);
+                        if (i == pts_n - 1) {
+                            dir2 = dir1;
+                        } else {
+                            dir2.x = (double)(fib[3 * i + 3] + a.sx) - S2.x;
+                            dir2.y = (double)(fib[3 * i + 4] + a.sy) - S2.y;
+                            dir2.z = (double)(fib[3 * i + 5] + a.sz) - S2.z;
+                            dir2 = vnormalize(dir2);
+                        }
+                        D3 nn = {0.5 * (dir1.x + dir2.x), 0.5 * (dir1.y + dir2.y), 0.5 * (dir1.z + dir2.z)};
+                        double dot = dir1.x * nn.x + dir1.y * nn.y + dir1.z * nn.z;
+                        nn.x /= dot;
+                        nn.y /= dot;
+                        nn.z /= dot;
+                        i++;
+                        const D3 Po = P0;
+                        double len = (S2.x - Po.x) * nn.x + (S2.y - Po.y) * nn.y + (S2.z - Po.z) * nn.z;
+                        if (!(len > 0)) continue;
+                        D3 Pn = {Po.x + dir1.x * len, Po.y + dir1.y * len, Po.z + dir1.z * len};
+                        P0 = Pn;
+                        if (!a.intersect) {
+                            rec.segment(Po, Pn, 0, wk);
+                            continue;
+                        }
+                        S1m = Po;
+                        S2m = Pn;
+                        f2x = floor(S2m.x / a.px);
+                        f2y = floor(S2m.y / a.py);
+                        f2z = floor(S2m.z / a.pz);
+                        it = 0;
+                        cap = -1;
+                        cross = true;
+                        break;
+                    }
+                    if (!cross) break;
+                }
+                // one iteration of the step's crossing loop
+                double dx = S2m.x - S1m.x, dy = S2m.y - S1m.y, dz = S2m.z - S1m.z;
+                double sl = sqrt(dx * dx + dy * dy + dz * dz);
+                if (sl <= a.min_seg) {
+                    cross = false;
                     continue;
                 }
-                D3 S1m = Po;
-                const D3 S2m = Pn;
-                long long cap = -1;
-                for (long long it = 0;; it++) {
-                    double dx = S2m.x - S1m.x, dy = S2m.y - S1m.y, dz = S2m.z - S1m.z;
-                    double sl = sqrt(dx * dx + dy * dy + dz * dz);
-                    if (sl <= a.min_seg) break;
-                    if (cap < 0) cap = max_cross(sl, a);
-                    if (it >= cap) {
-                        capped = true;
-                        break;
-                    }
-                    if (floor(S1m.x / a.px) == floor(S2m.x / a.px) && floor(S1m.y / a.py) == floor(S2m.y / a.py) &&
-                        floor(S1m.z / a.pz) == floor(S2m.z / a.pz)) {
-                        rec.segment(S1m, S2m, k, wk);
-                        break;
-                    }
+                if (cap < 0) cap = max_cross(sl, a);
+                if (it >= cap) {
+                    capped = true;
+                    cross = false;
+                    continue;
+                }
+                it++;
+                D3 Pb = S2m;
+                bool fin = true;
+                if (!(floor(S1m.x / a.px) == f2x && floor(S1m.y / a.py) == f2y && floor(S1m.z / a.pz) == f2z)) {
                     D3 vmin, vmax;
                     vmin.x = floor((S1m.x + 1e-6 * dir1.x) / a.px) * a.px;
                     vmin.y = floor((S1m.y + 1e-6 * dir1.y) / a.py) * a.py;
@@ -295,14 +388,13 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                     vmax.z = vmin.z + a.pz;
                     double t;
                     if (raybox(S1m, dir1, vmin, vmax, t) && t > 0 && t < sl) {
-                        D3 Pi = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
-                        rec.segment(S1m, Pi, k, wk);
-                        S1m = Pi;
-                    } else {
-                        rec.segment(S1m, S2m, k, wk);
-                        break;
+                        Pb = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
+                        fin = false;
                     }
                 }
+                rec.segment(S1m, Pb, 0, wk);
+                if (fin) cross = false;
+                else S1m = Pb;
             }
         }
     }
