@@ -32,8 +32,11 @@ extern "C" {
 
 /* precision of x, y and the accumulation */
 #define COMMIT_HIP_FP64 0         /* drop-in: x, y float64; FP64 accumulation */
-#define COMMIT_HIP_FP32 1         /* x, y (and FISTA vectors) float32; products of the float32 LUT,
-                                     lengths and vectors accumulated in FP64, rounded once on store */
+#define COMMIT_HIP_FP32 1         /* x, y (and FISTA vectors) float32.  LDS-slab products in FP32
+                                     arithmetic: A x sums groups of 4 entries in FP32 into FP64
+                                     accumulators, A^T y forms per-lane FP32 partial dot products and
+                                     reduces them in FP64.  The per-voxel and dense-tile products
+                                     accumulate in FP64.  Outputs rounded once on store. */
 
 typedef struct commit_hip_plan commit_hip_plan;
 
