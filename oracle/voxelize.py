@@ -18,6 +18,7 @@ ISO and TDI definitions.
 """
 import ctypes
 import os
+import sys
 import time
 
 import numpy as np
@@ -229,14 +230,25 @@ def reference_trk2dictionary(workdir, fibers, fib_ptr, dim, pixdim, hash_table, 
         aff = np.ascontiguousarray(np.eye(3) if peaks_affine is None else peaks_affine, np.float64)
     sh = np.broadcast_to(np.asarray(fiber_shift, np.float32), (3,))
     hash_table = np.ascontiguousarray(hash_table, np.int16)
-    t0 = time.perf_counter()
-    ret = _ref_lib()(trk.encode(), 1000, int(dim[0]), int(dim[1]), int(dim[2]), float(pixdim[0]), float(pixdim[1]),
-                     float(pixdim[2]), n, 0, 0, float(sh[0]), float(sh[1]), float(sh[2]), float(min_seg_len),
-                     float(min_fiber_len), float(max_fiber_len), _ptr(peaks), Np, float(vf_THR),
-                     -1 if flip_peaks[0] else 1, -1 if flip_peaks[1] else 1, -1 if flip_peaks[2] else 1,
-                     _ptr(mask), _ptr(iso_mask), tptr, temp.encode(), 1 if do_intersect else 0, _ptr(aff),
-                     int(rho.size), _ptr(rho), _ptr(ang), _ptr(wts), _ptr(apply), None, _ptr(hash_table), nt, 0)
-    seconds = time.perf_counter() - t0
+    # the reference prints its progress on the C stdout: send it to stderr, so
+    # that a caller's own stdout (a bench's one JSON line) stays clean
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        t0 = time.perf_counter()
+        ret = _ref_lib()(trk.encode(), 1000, int(dim[0]), int(dim[1]), int(dim[2]), float(pixdim[0]),
+                         float(pixdim[1]), float(pixdim[2]), n, 0, 0, float(sh[0]), float(sh[1]), float(sh[2]),
+                         float(min_seg_len), float(min_fiber_len), float(max_fiber_len), _ptr(peaks), Np,
+                         float(vf_THR), -1 if flip_peaks[0] else 1, -1 if flip_peaks[1] else 1,
+                         -1 if flip_peaks[2] else 1, _ptr(mask), _ptr(iso_mask), tptr, temp.encode(),
+                         1 if do_intersect else 0, _ptr(aff), int(rho.size), _ptr(rho), _ptr(ang), _ptr(wts),
+                         _ptr(apply), None, _ptr(hash_table), nt, 0)
+        seconds = time.perf_counter() - t0
+    finally:
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
     if ret != 1:
         raise RuntimeError("the reference's trk2dictionary() generated no dictionary")
     rd = lambda name, dt: np.fromfile(name, dtype=dt)
