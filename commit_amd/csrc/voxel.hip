@@ -69,6 +69,7 @@ struct VoxArgs {
     int n_rep;
     uint64_t nvox;
     const float *mask;                 // C order [X][Y][Z] or null
+    const uint64_t *mbits;             // mask != 0, one bit per voxel in the same order (with mask)
     const short *hash;                 // 181 * 181
     const double *rho, *sinh, *cosh, *wt;   // per replica
     const uint8_t *apply;              // per streamline or null
@@ -171,7 +172,10 @@ struct Recorder {
         int vy = (int)floor(0.5 * (P1.y + P2.y) / a.py);
         int vz = (int)floor(0.5 * (P1.z + P2.z) / a.pz);
         if (vx >= a.X || vx < 0 || vy >= a.Y || vy < 0 || vz >= a.Z || vz < 0) return;
-        if (a.mask && a.mask[vz + (size_t)a.Z * (vy + (size_t)a.Y * vx)] == 0) return;
+        if (a.mask) {                  // ptrMASK[...] == 0, from a bit array that stays in L2
+            const uint64_t mv = (uint64_t)vz + (uint64_t)a.Z * ((uint64_t)vy + (uint64_t)a.Y * vx);
+            if (!((a.mbits[mv >> 6] >> (mv & 63)) & 1ull)) return;
+        }
         if (EMIT && n < limit) {
             double lon = atan2(dir.y, dir.x);
             double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
@@ -316,25 +320,33 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
             const double wk = a.wt[0];
             int i = 1;
             bool cross = false;
+            // points i-1, i, i+1 of the next step in registers, point i+2
+            // requested a step before it is needed
+            float3 pa = {fib[0], fib[1], fib[2]}, pb = {fib[3], fib[4], fib[5]}, pc = {fib[6], fib[7], fib[8]};
+            float3 pd = pc;
+            if (pts_n > 3) pd = {fib[9], fib[10], fib[11]};
             D3 S1m = {0, 0, 0}, S2m = {0, 0, 0}, dir1 = {0, 0, 0};
             double f2x = 0, f2y = 0, f2z = 0;               // floor(S2m / pixdim), fixed for a step
             long long it = 0, cap = -1;
             for (;;) {
                 if (!cross) {
                     while (i < pts_n) {
-                        D3 S2 = {(double)(fib[3 * i] + a.sx), (double)(fib[3 * i + 1] + a.sy),
-                                 (double)(fib[3 * i + 2] + a.sz)};
-                        dir1 = {S2.x - (double)(fib[3 * i - 3] + a.sx), S2.y - (double)(fib[3 * i - 2] + a.sy),
-                                S2.z - (double)(fib[3 * i - 1] + a.sz)};
+                        D3 S2 = {(double)(pb.x + a.sx), (double)(pb.y + a.sy), (double)(pb.z + a.sz)};
+                        dir1 = {S2.x - (double)(pa.x + a.sx), S2.y - (double)(pa.y + a.sy),
+                                S2.z - (double)(pa.z + a.sz)};
                         dir1 = vnormalize(dir1);
                         if (i == pts_n - 1) {
                             dir2 = dir1;
                         } else {
-                            dir2.x = (double)(fib[3 * i + 3] + a.sx) - S2.x;
-                            dir2.y = (double)(fib[3 * i + 4] + a.sy) - S2.y;
-                            dir2.z = (double)(fib[3 * i + 5] + a.sz) - S2.z;
+                            dir2.x = (double)(pc.x + a.sx) - S2.x;
+                            dir2.y = (double)(pc.y + a.sy) - S2.y;
+                            dir2.z = (double)(pc.z + a.sz) - S2.z;
                             dir2 = vnormalize(dir2);
                         }
+                        pa = pb;
+                        pb = pc;
+                        pc = pd;
+                        if (i + 3 < pts_n) pd = {fib[3 * i + 9], fib[3 * i + 10], fib[3 * i + 11]};
                         D3 nn = {0.5 * (dir1.x + dir2.x), 0.5 * (dir1.y + dir2.y), 0.5 * (dir1.z + dir2.z)};
                         double dot = dir1.x * nn.x + dir1.y * nn.y + dir1.z * nn.z;
                         nn.x /= dot;
@@ -417,6 +429,16 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_bound(const int64_t *__restri
     if (f >= nF) return;
     const int64_t n = fptr[f + 1] - fptr[f];
     bnd[f] = n > 2 ? (uint32_t)(4 * n + 16) : 0u;
+}
+
+// bits[w] bit l = (mask[64 w + l] != 0): the mask as one bit per voxel, a
+// wavefront per 64-bit word
+__global__ __launch_bounds__(kVoxBlock) void k_vox_mask_bits(const float *__restrict__ mask, uint64_t nvox,
+                                                             uint64_t *__restrict__ bits)
+{
+    const uint64_t v = (uint64_t)blockIdx.x * kVoxBlock + threadIdx.x;
+    const uint64_t m = __ballot(v < nvox && mask[v < nvox ? v : 0] != 0.0f);
+    if ((threadIdx.x & (kWave - 1)) == 0 && v < nvox) bits[v >> 6] = m;
 }
 
 // end[f] = off[f] + cnt[f]: the recorded part of streamline f's region
@@ -743,7 +765,7 @@ struct commit_hip_vox {
         PTS, FPTR, MASK, ISO, PEAKS, HASH, REP, APPLY, PBUF, CNT, FLEN, FLENTOT, OFF, K1, V1, K2, V2, L1,
         MCNT, MEND, KEPT, KIDX, SPOS, NORM, STATUS, TMP,
         IC_F, IC_V, IC_O, IC_LEN, TRK_NORM, TRK_LEN, TRK_LENTOT, TRK_KEPT, TDI_V, TDI_L, TDI, HAS,
-        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, OKEY, OIDX, OKEY2, ORDER, END, POFF, NBUF
+        ECNT, EOFF, EC_V, EC_O, IFLAG, IOFF, ISO_V, OKEY, OIDX, OKEY2, ORDER, END, POFF, MBITS, NBUF
     };
     Buf buf[NBUF];
     int64_t n_ic = 0, n_kept = 0, n_ec = 0, n_iso = 0, n_count = 0, n_pieces = 0;
@@ -1094,6 +1116,14 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     a.n_rep = nReplicas;
     a.nvox = nvox;
     a.mask = d_mask;
+    a.mbits = nullptr;
+    if (d_mask) {
+        if ((rc = V->need(B::MBITS, (nvox + 63) / 64 * 8))) return rc;
+        uint64_t *bits = V->at<uint64_t>(B::MBITS);
+        a.mbits = bits;
+        k_vox_mask_bits<<<grid_for(nvox), kVoxBlock, 0, s>>>(d_mask, nvox, bits);
+        HIP_TRY(hipGetLastError());
+    }
     a.hash = V->at<short>(B::HASH);
     const double *d_rep = V->at<double>(B::REP);
     a.rho = d_rep;
