@@ -40,6 +40,7 @@ SIGNATURES = {
     "commit_hip_At": (_I, [_P, _P, _P, _P]),
     "commit_hip_A_residual": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "commit_hip_residual_combine": (_I, [_P, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P, _P]),
+    "commit_hip_At_combined": (_I, [_P, ctypes.c_double, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P]),
     "commit_hip_At_weighted": (_I, [_P, _P, _P, _P, _P]),
     "commit_hip_At_pass": (_I, [_P, _P, _P, _P, _I, _P]),
     "commit_hip_A_multi": (_I, [_P, _P, _P, _I, _I, _P]),

@@ -803,9 +803,47 @@ int do_combine(commit_hip_plan *plan, double a, const void *r1, double b, const 
     return 0;
 }
 
+// A^T (sqrtw .* (a r1 + b r2)) and ||a r1 + b r2||^2: fused on the slab path,
+// else the combination into r and the plain A^T.
+template <typename T>
+int do_At_combined(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2, const void *sqrtw,
+                   void *r, void *x, double *rsq, hipStream_t st)
+{
+    const DevDict &d = plan->d;
+    const T *p1 = static_cast<const T *>(r1), *p2 = static_cast<const T *>(r2), *w = static_cast<const T *>(sqrtw);
+    T *xo = static_cast<T *>(x);
+    if (vox_blocks(d) > 0 && plan->slab) {
+        int rc = commit_internal::launch_At_slab_comb<T>(plan, p1, p2, a, b, w, xo, st);
+        if (rc == 0) {
+            if ((rc = launch_At<T>(plan, p1, w, xo, st, 2))) return rc;          // pass 2 only
+            const size_t nv = (size_t)d.nVOX;
+            hipLaunchKernelGGL(k_sum_stage1, dim3(kSumBlocks), dim3(kBlock), 0, st, plan->rpart, nv, plan->rpart + nv);
+            hipLaunchKernelGGL(k_sum_stage2, dim3(1), dim3(kBlock), 0, st, plan->rpart + nv, kSumBlocks, rsq);
+            HIP_TRY(hipGetLastError());
+            return 0;
+        }
+        if (rc != COMMIT_HIP_EUNSUPPORTED) return rc;
+    }
+    int rc = do_combine<T>(plan, a, r1, b, r2, r, rsq, st);
+    if (rc) return rc;
+    return do_At<T>(plan, r, sqrtw, x, st, 3);
+}
+
 }  // namespace
 
 extern "C" {
+
+int commit_hip_At_combined(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2,
+                           const void *sqrtw, void *r, void *x, double *rsq, void *stream)
+{
+    if (!plan || !r1 || !r2 || !r || !x || !rsq) return fail(COMMIT_HIP_EINVAL, "NULL argument");
+    if (plan->nolut && plan->nS_ref > 1 && sqrtw)
+        return fail(COMMIT_HIP_EUNSUPPORTED, "row weights on a LUT-free operator with nS > 1");
+    HIP_TRY(hipSetDevice(plan->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return plan->precision == COMMIT_HIP_FP32 ? do_At_combined<float>(plan, a, r1, b, r2, sqrtw, r, x, rsq, st)
+                                              : do_At_combined<double>(plan, a, r1, b, r2, sqrtw, r, x, rsq, st);
+}
 
 int commit_hip_residual_combine(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2, void *r,
                                 double *rsq, void *stream)

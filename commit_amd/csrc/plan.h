@@ -137,6 +137,12 @@ template <typename T>
 int launch_A_slab(commit_hip_plan *p, const T *x, const T *xic, const AOut<T> &out, hipStream_t st);
 template <typename T>
 int launch_At_slab(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStream_t st);
+// Pass 1 of A^T (sqrtw .* (a r1 + b r2)) on the LDS slabs, with each voxel's
+// sum of squares of a r1 + b r2 (FP64) in p->rpart[v]; EUNSUPPORTED when the
+// plan has no slab tables.
+template <typename T>
+int launch_At_slab_comb(commit_hip_plan *p, const T *r1, const T *r2, double a, double b, const T *sqrtw, T *x,
+                        hipStream_t st);
 
 // Dense-tile A x (dense.hip); COMMIT_HIP_EUNSUPPORTED when the plan's sizes
 // are outside what it is built for.

@@ -510,9 +510,20 @@ __device__ __forceinline__ void glob_at(const T (&yv)[NJ], const Comp &cp, uint3
     }
 }
 
-template <int NJ, int V, int W, typename T>
-__global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__restrict__ y, const T *__restrict__ sqrtw,
-                                                      double *__restrict__ z, T *__restrict__ x, SlabArgs sa)
+// COMB: the y rows are formed from two residuals, y = a r1 + b r2 in FP64
+// (as commit_hip_residual_combine forms them, rounded to the vectors' type),
+// and each voxel's sum of the unrounded values' squares goes to rpart[v].
+template <typename T>
+struct CombIn {
+    const T *r2;
+    double a, b;
+    double *rpart;
+};
+
+template <int NJ, int V, int W, typename T, bool COMB>
+__device__ __forceinline__ void at_slab(const DevDict &d, const T *__restrict__ y, const T *__restrict__ sqrtw,
+                                        double *__restrict__ z, T *__restrict__ x, const CombIn<T> &cb,
+                                        const SlabArgs &sa)
 {
     __shared__ __align__(16) float L[SlabLds<W>::floats];
     const int lane = threadIdx.x & (kWave - 1);
@@ -529,18 +540,31 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
 #pragma unroll
     for (int u = 0; u < V; u++) {
         const size_t row = (size_t)(vox[u] < 0 ? 0 : vox[u]) * d.nS;
-        T raw[NJ], wv[NJ];
+        T raw[NJ], wv[NJ], r2v[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
             raw[j] = stream_load(y + row + (s < d.nS ? s : 0));
+            if constexpr (COMB) r2v[j] = stream_load(cb.r2 + row + (s < d.nS ? s : 0));
             wv[j] = sqrtw ? stream_load(sqrtw + row + (s < d.nS ? s : 0)) : (T)1;
         }
+        double ss = 0.0;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
+            const bool in = vox[u] >= 0 && Lanes<NJ, kQuads<T>>::sample(lane, j) < d.nS;
             T t = raw[j];
+            if constexpr (COMB) {
+                const double v = cb.a * (double)raw[j] + cb.b * (double)r2v[j];
+                t = (T)v;
+                if (in) ss += v * v;
+            }
             if (sqrtw) t *= wv[j];
-            yv[u][j] = (vox[u] >= 0 && Lanes<NJ, kQuads<T>>::sample(lane, j) < d.nS) ? t : (T)0;
+            yv[u][j] = in ? t : (T)0;
+        }
+        if constexpr (COMB) {                    // the voxel's sum, in a fixed order
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) ss += __shfl_xor(ss, off);
+            if (lane == 0 && vox[u] >= 0) cb.rpart[vox[u]] = ss;
         }
     }
     const size_t lut_rf = (size_t)d.ndirs * d.nSp;
@@ -611,6 +635,21 @@ __global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__
                             (size_t)d.nSp, 0, lane, tail);
         }
     }
+}
+
+template <int NJ, int V, int W, typename T>
+__global__ __launch_bounds__(W * kWave, 4) void k_At_slab(DevDict d, const T *__restrict__ y, const T *__restrict__ sqrtw,
+                                                      double *__restrict__ z, T *__restrict__ x, SlabArgs sa)
+{
+    at_slab<NJ, V, W, T, false>(d, y, sqrtw, z, x, CombIn<T>{nullptr, 0.0, 0.0, nullptr}, sa);
+}
+
+template <int NJ, int V, int W, typename T>
+__global__ __launch_bounds__(W * kWave, 4) void k_At_slab_comb(DevDict d, const T *__restrict__ y,
+                                                           const T *__restrict__ sqrtw, double *__restrict__ z,
+                                                           T *__restrict__ x, CombIn<T> cb, SlabArgs sa)
+{
+    at_slab<NJ, V, W, T, true>(d, y, sqrtw, z, x, cb, sa);
 }
 
 // ofs[v*(nsl+1) + s] = first IC entry of voxel v with orientation >= s*S
@@ -914,6 +953,17 @@ int launch_At_slab(commit_hip_plan *p, const T *y, const T *sqrtw, T *x, hipStre
     return 0;
 }
 
+template <typename T>
+int launch_At_slab_comb(commit_hip_plan *p, const T *r1, const T *r2, double a, double b, const T *sqrtw, T *x,
+                        hipStream_t st)
+{
+    const DevDict &d = p->d;
+    if (!p->slab || !p->slab_vlist_at || d.nSp / kWave > kSlabMaxNJ) return COMMIT_HIP_EUNSUPPORTED;
+    const CombIn<T> cb{r2, a, b, p->rpart};
+    SLAB_SWITCH(k_At_slab_comb, kSlabVAt, p->slab_vlist_at, p->slab_blocks_at, d, r1, sqrtw, p->zbuf, x, cb);
+    return 0;
+}
+
 #undef SLAB_SWITCH
 
 template int launch_A_slab<double>(commit_hip_plan *, const double *, const double *, const AOut<double> &,
@@ -921,5 +971,9 @@ template int launch_A_slab<double>(commit_hip_plan *, const double *, const doub
 template int launch_A_slab<float>(commit_hip_plan *, const float *, const float *, const AOut<float> &, hipStream_t);
 template int launch_At_slab<double>(commit_hip_plan *, const double *, const double *, double *, hipStream_t);
 template int launch_At_slab<float>(commit_hip_plan *, const float *, const float *, float *, hipStream_t);
+template int launch_At_slab_comb<double>(commit_hip_plan *, const double *, const double *, double, double,
+                                         const double *, double *, hipStream_t);
+template int launch_At_slab_comb<float>(commit_hip_plan *, const float *, const float *, double, double,
+                                        const float *, float *, hipStream_t);
 
 }  // namespace commit_internal

@@ -412,13 +412,14 @@ def _fista_device(y, A, reg, sqrt_W, tol_fun, tol_x, max_iter, verbose, x0, lips
         _lib.check(axpby(n, 1.0, P(x), -1.0, P(prev_x), P(tmp), st), "commit_hip_axpby")
         _lib.check(axpby(n, 1.0, P(x), mom, P(tmp), P(xhat), st), "commit_hip_axpby")
         # r(xhat) = (1 + mom) r(x) - mom r(prev_x), with its norm (read with the
-        # next trial's scalars); the first update has no r(prev_x) and runs A xhat
+        # next trial's scalars), formed while the gradient's A^T loads it; the
+        # first update has no r(prev_x) and runs A xhat
         if it > 1:
-            _lib.check(lib.commit_hip_residual_combine(plan, 1.0 + mom, P(rbuf[0]), -mom, P(rbuf[1]), P(res),
-                                                       P(sc[9:10]), st), "commit_hip_residual_combine")
+            _lib.check(lib.commit_hip_At_combined(plan, 1.0 + mom, P(rbuf[0]), -mom, P(rbuf[1]), P(sw), P(res),
+                                                  P(grad), P(sc[9:10]), st), "commit_hip_At_combined")
         else:
             residual(xhat, out=sc[9:10])
-        gradient()
+            gradient()
         it += 1
         prev_obj = curr_obj
         prev_x.copy_(x)

@@ -164,6 +164,20 @@ int commit_hip_residual_combine(commit_hip_plan *plan, double a, const void *r1,
 int commit_hip_At_weighted(commit_hip_plan *plan, const void *y, const void *sqrtw, void *x, void *stream);
 
 /*
+ * The FISTA gradient at the momentum point in one call: with
+ * rhat = a r1 + b r2 (FP64, rounded to the plan's vectors, as
+ * commit_hip_residual_combine forms it), x = A^T (sqrtw .* rhat) and
+ * *rsq = ||rhat||^2 (FP64, fixed order).  On the LDS-slab path the combination
+ * is formed while A^T loads its rows and rhat is never stored; otherwise the
+ * call is commit_hip_residual_combine into r, then commit_hip_At_weighted.  r:
+ * a row vector the call may overwrite.  Replaces the combination of
+ * solvers.py:363-368 followed by the gradient's A^T (no counterpart in the
+ * reference, which recomputes A xhat).
+ */
+int commit_hip_At_combined(commit_hip_plan *plan, double a, const void *r1, double b, const void *r2,
+                           const void *sqrtw, void *r, void *x, double *rsq, void *stream);
+
+/*
  * One pass of A^T y, for timing and profiling each kernel on its own (no
  * counterpart in the reference): pass 1 = voxel-major products (and the EC/ISO
  * columns of x), pass 2 = per-fibre sums into the IC columns of x; pass 3 = both

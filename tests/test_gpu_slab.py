@@ -193,3 +193,47 @@ def test_slab_runs_of_one_orientation(mode, zero_frac):
     e = (rel(Ax, O.dot(x)), rel(Aty, O.T.dot(y)), rel(res, r_o), rel(g, O.T.dot(w * r_o)))
     print(f"mode={mode} zeros={zero_frac}: " + " ".join(f"{v:.1e}" for v in e), flush=True)
     assert max(e) <= TOL
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_At_combined(mode):
+    """commit_hip_At_combined: A^T (sqrtw .* (a r1 + b r2)) and ||a r1 + b r2||^2
+    in one call.  On the slab path (mode 2) the combination is formed while
+    A^T loads its rows: against the oracle's A^T of the combined residual
+    (1e-12 in FP64, 1e-6 with FP32 storage) and against the two calls it
+    replaces (commit_hip_residual_combine, commit_hip_At_weighted).  Slabs off
+    (mode 0): the two calls themselves, bitwise."""
+    D, K = _dictionary(31 + mode, nVOX=150, n=9000, nS=288, ndirs=300, nIC=1, nEC=1, nISO=2, heavy=(5, 900))
+    T = oracle.set_threads(D, 4)
+    A = LinearOperator(D, K, T)
+    autotune.set_option(A, _lib.OPT_SLAB, mode)
+    O = oracle.OracleOperator(D, K, T)
+    P = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib, h = _lib.load(), A._plan.handle
+    rng = np.random.default_rng(mode + 3)
+    m, n = A.shape
+    a, b = 1.37, -0.37
+    for prec, dt, tol in ((_lib.FP64, torch.float64, TOL), (_lib.FP32, torch.float32, 1e-6)):
+        A.set_precision(prec)
+        for weighted in (True, False):
+            r1 = torch.from_numpy(rng.standard_normal(m)).to("cuda", dt)
+            r2 = torch.from_numpy(rng.standard_normal(m)).to("cuda", dt)
+            w = torch.from_numpy(rng.uniform(0.5, 1.5, m)).to("cuda", dt) if weighted else None
+            rs, rsq = torch.empty(m, dtype=dt, device="cuda"), torch.empty(1, dtype=torch.float64, device="cuda")
+            g = torch.empty(n, dtype=dt, device="cuda")
+            _lib.check(lib.commit_hip_At_combined(h, a, P(r1), b, P(r2), P(w), P(rs), P(g), P(rsq), st), "combined")
+            r, rsq2 = torch.empty(m, dtype=dt, device="cuda"), torch.empty(1, dtype=torch.float64, device="cuda")
+            g2 = torch.empty(n, dtype=dt, device="cuda")
+            _lib.check(lib.commit_hip_residual_combine(h, a, P(r1), b, P(r2), P(r), P(rsq2), st), "combine")
+            _lib.check(lib.commit_hip_At_weighted(h, P(r), P(w), P(g2), st), "At_weighted")
+            u = a * r1.double().cpu().numpy() + b * r2.double().cpu().numpy()
+            wn = np.ones(m) if w is None else w.double().cpu().numpy()
+            g_o = O.T.dot(wn * u.astype(np.float32).astype(np.float64) if prec == _lib.FP32 else wn * u)
+            e = (rel(g.double().cpu().numpy(), g_o), abs(rsq.item() - u @ u) / (u @ u),
+                 rel(g.double().cpu().numpy(), g2.double().cpu().numpy()), abs(rsq.item() - rsq2.item()) / rsq2.item())
+            print(f"mode={mode} prec={prec} weighted={weighted}: " + " ".join(f"{v:.1e}" for v in e), flush=True)
+            assert e[0] <= tol and e[1] <= 1e-13 and e[2] <= tol and e[3] <= 1e-13
+            if mode == 0:
+                assert torch.equal(g, g2) and rsq.item() == rsq2.item()
+    A.set_precision(_lib.FP64)
