@@ -43,6 +43,7 @@
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_segmented_radix_sort.hpp>
 
@@ -553,6 +554,204 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_norm(const uint64_t *__restri
     if (lane == 0) norm[f] = (float)sqrt((double)fn);
 }
 
+// Streamlines of at most kSortCap pieces take k_vox_sortmerge, those of at
+// most kSortSmall in a launch with small LDS buffers.
+constexpr int kSortCap = 2048;
+constexpr int kSortSmall = 512;
+
+// One wavefront's LDS writes done and visible to all its lanes.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Ascending bitonic sort of sk[0, N) (N a power of two, >= 64) by one
+// wavefront, in LDS.
+__device__ __forceinline__ void bitonic_u64(uint64_t *sk, int N, int lane)
+{
+    for (int k = 2; k <= N; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lane; t < (N >> 1); t += kWave) {
+                const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                const int l = i | j;
+                const uint64_t x = sk[i], y = sk[l];
+                if ((x > y) == ((i & k) == 0)) {
+                    sk[i] = y;
+                    sk[l] = x;
+                }
+            }
+            wave_lds_sync();
+        }
+}
+
+// Ascending bitonic sort of N = 64 E words held by one wavefront in
+// registers, word p = 64-bit v[e] of lane p / E, e = p % E: stages whose
+// pairs lie inside a lane compare in registers, the others exchange with the
+// partner lane (lane ^ j / E) by shuffles.
+template <int E, int J>
+__device__ __forceinline__ void bitonic_lane_stage(uint64_t (&v)[8], int lane, int k)
+{
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int pe = e ^ J;
+        if (pe > e) {
+            const bool up = ((lane * E + e) & k) == 0;
+            const uint64_t x = v[e], y = v[pe];
+            const bool sw = (x > y) == up;
+            v[e] = sw ? y : x;
+            v[pe] = sw ? x : y;
+        }
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void bitonic_reg(uint64_t (&v)[8], int lane)
+{
+    constexpr int N = kWave * E;
+    for (int k = 2; k <= N; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= E) {
+                const int lm = j / E;
+                const bool lower = (lane & lm) == 0;
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const uint64_t o = (uint64_t)__shfl_xor((unsigned long long)v[e], lm);
+                    const bool up = ((lane * E + e) & k) == 0;
+                    const bool keep_min = lower == up;
+                    v[e] = keep_min ? (v[e] < o ? v[e] : o) : (v[e] > o ? v[e] : o);
+                }
+            } else if (j == 4) {
+                if constexpr (E > 4) bitonic_lane_stage<E, 4>(v, lane, k);
+            } else if (j == 2) {
+                if constexpr (E > 2) bitonic_lane_stage<E, 2>(v, lane, k);
+            } else {
+                if constexpr (E > 1) bitonic_lane_stage<E, 1>(v, lane, k);
+            }
+        }
+}
+
+// Sort sk[0, N) (N = 64 E, E in 1, 2, 4, 8): in registers, the result back
+// in sk.  Entries past the valid ones are all-ones words (they sort last).
+__device__ __forceinline__ void sort_words(uint64_t *sk, int N, int lane)
+{
+    uint64_t v[8];
+    const int E = N / kWave;
+#define SORT_E(EE)                                                   \
+    {                                                                \
+        _Pragma("unroll") for (int e = 0; e < EE; e++) v[e] = sk[lane * EE + e]; \
+        bitonic_reg<EE>(v, lane);                                    \
+        _Pragma("unroll") for (int e = 0; e < EE; e++) sk[lane * EE + e] = v[e]; \
+    }
+    if (E == 1) SORT_E(1) else if (E == 2) SORT_E(2) else if (E == 4) SORT_E(4) else SORT_E(8)
+#undef SORT_E
+    wave_lds_sync();
+}
+
+// The IC stage between the walk and the output for 32-bit keys, one
+// wavefront per streamline, in LDS: the pieces [off[f],
+// end[f]) sorted by (key, recording position), equal keys summed into a float
+// in recording order and stored from the streamline's first slot (keys over
+// the streamline's own unsorted keys, lengths to mlen), the kept flag, and
+// for a kept streamline its merged segments sorted by (voxel, rank) and
+// TRK_norm: per voxel the float sum of its lengths in order, the float sum of
+// their squares in voxel order, a float square root.  The operations and
+// their order are those of the segmented sorts, k_vox_merge and k_vox_norm.
+// A launch takes the streamlines of nlo < pieces <= nmax (a power of two).
+// Each wavefront of a work-group takes its own streamline and its own LDS
+// slice of nmax x (8-byte sort word, piece value, merged key, merged length).
+template <class VT>
+__global__ __launch_bounds__(kVoxBlock) void k_vox_sortmerge(const uint64_t *__restrict__ off,
+                                                         const uint64_t *__restrict__ end, int nF, uint32_t *keys,
+                                                         const VT *__restrict__ vals, uint64_t nvox,
+                                                         const float *__restrict__ flen, float min_fib,
+                                                         float max_fib, int nmax, int nlo, float *__restrict__ mlen,
+                                                         uint32_t *__restrict__ mcnt, uint64_t *__restrict__ mend,
+                                                         uint32_t *__restrict__ kept, float *__restrict__ norm)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & (kWave - 1));
+    // this wavefront's slice of the work-group's LDS
+    uint64_t *sk = reinterpret_cast<uint64_t *>(smem + (size_t)w * nmax * (sizeof(uint64_t) + sizeof(VT) + 8));
+    VT *sv = reinterpret_cast<VT *>(sk + nmax);
+    uint32_t *mk = reinterpret_cast<uint32_t *>(sv + nmax);
+    float *ml = reinterpret_cast<float *>(mk + nmax);
+    const int f = (int)(blockIdx.x * (blockDim.x >> 6)) + w;
+    if (f >= nF) return;
+    const uint64_t b = off[f];
+    const int n = (int)(end[f] - b);
+    if (n <= nlo || n > nmax) return;                 // another launch's size class
+    int N = kWave;
+    while (N < n) N <<= 1;
+    for (int i = lane; i < N; i += kWave) {
+        sk[i] = i < n ? ((uint64_t)keys[b + i] << 32) | (uint32_t)i : ~0ull;
+        if (i < n) sv[i] = vals[b + i];
+    }
+    wave_lds_sync();
+    if (N <= kWave * 8) sort_words(sk, N, lane);
+    else bitonic_u64(sk, N, lane);
+    // equal keys, adjacent and in recording order: the run's first lane adds
+    // the run into a float (the map's +=) and stores it at its rank
+    int m = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int p = base + lane;
+        const bool valid = p < n;
+        const uint32_t k = valid ? (uint32_t)(sk[p] >> 32) : 0u;
+        const bool head = valid && (p == 0 || (uint32_t)(sk[p - 1] >> 32) != k);
+        const unsigned long long heads = __ballot(head);
+        if (head) {
+            float acc = 0.0f;
+            for (int q = p; q < n && (uint32_t)(sk[q] >> 32) == k; q++)
+                acc = (float)((double)acc + (double)sv[(uint32_t)sk[q]]);
+            const int r = m + __popcll(heads & ((1ull << lane) - 1ull));
+            keys[b + r] = k;
+            mlen[b + r] = acc;
+            mk[r] = k;
+            ml[r] = acc;
+        }
+        m += __popcll(heads);
+    }
+    const float fl = flen[f];
+    const bool kf = m > 0 && fl > min_fib && fl < max_fib;
+    if (lane == 0) {
+        mcnt[f] = kf ? (uint32_t)m : 0u;
+        mend[f] = kf ? b + m : b;
+        kept[f] = kf ? 1u : 0u;
+    }
+    if (!kf) {
+        if (lane == 0) norm[f] = 0.0f;
+        return;
+    }
+    wave_lds_sync();                                  // the merge's reads of sk are done
+    int N2 = kWave;
+    while (N2 < m) N2 <<= 1;
+    for (int i = lane; i < N2; i += kWave)
+        sk[i] = i < m ? ((uint64_t)(uint32_t)((uint64_t)mk[i] % nvox) << 32) | (uint32_t)i : ~0ull;
+    wave_lds_sync();
+    if (N2 <= kWave * 8) sort_words(sk, N2, lane);
+    else bitonic_u64(sk, N2, lane);
+    float fn = 0.0f;
+    for (int base = 0; base < m; base += kWave) {
+        const int p = base + lane;
+        const bool valid = p < m;
+        const uint32_t v = valid ? (uint32_t)(sk[p] >> 32) : 0u;
+        const bool head = valid && (p == 0 || (uint32_t)(sk[p - 1] >> 32) != v);
+        double sq = 0.0;
+        if (head) {
+            float r = 0.0f;
+            for (int q = p; q < m && (uint32_t)(sk[q] >> 32) == v; q++) r = r + ml[(uint32_t)sk[q]];
+            sq = (double)r * (double)r;
+        }
+        unsigned long long hm = __ballot(head);
+        while (hm) {
+            const int l = (int)__builtin_ctzll(hm);
+            hm &= hm - 1ull;
+            fn = (float)((double)fn + readlane_f64(sq, l));
+        }
+    }
+    if (lane == 0) norm[f] = (float)sqrt((double)fn);
+}
+
 // Kept streamlines' segments and per-streamline values, in streamline order.
 template <class KT>
 __global__ __launch_bounds__(kVoxBlock) void k_vox_write(
@@ -938,28 +1137,65 @@ int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n
     } else if (hst[0]) {
         return fail(COMMIT_HIP_EDEVICE, "the recording pass differs from the counting pass");
     }
-    if ((rc = segsort(V, (const KT *)K1, K2, (const VT *)V1, V2, np, n_count, d_off, d_end, ia.kbits, s)))
-        return rc;
-    // merged segments: keys to K1, float lengths to L1, voxel keys into V1's memory
-    uint32_t *VK = (uint32_t *)V1;
     uint32_t *d_mcnt = V->at<uint32_t>(B::MCNT), *d_kept = V->at<uint32_t>(B::KEPT);
     uint64_t *d_mend = V->at<uint64_t>(B::MEND);
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_mcnt + nF, 1, 0u);
     k_fill_u32<<<1, kVoxBlock, 0, s>>>(d_kept + nF, 1, 0u);
-    if (nF) {
-        k_vox_merge<KT, VT><<<gW, kVoxBlock, 0, s>>>(d_off, d_end, n_count, K2, V2, nvox, V->at<float>(B::FLEN),
-                                                     ia.min_fib, ia.max_fib, K1, L1, VK, d_mcnt, d_mend, d_kept);
-        HIP_TRY(hipGetLastError());
+    bool fused = false;
+    if constexpr (sizeof(KT) == 4) {
+        if (nF) {
+            // the longest streamline's pieces decide whether k_vox_sortmerge fits
+            uint32_t *dmax = ia.status + 3;
+            size_t bytes = 0;
+            HIP_TRY(rocprim::reduce(nullptr, bytes, (const uint32_t *)ia.cnt, dmax, 0u, nF,
+                                    rocprim::maximum<uint32_t>(), s));
+            if ((rc = ensure_tmp(V, bytes))) return rc;
+            bytes = V->buf[B::TMP].bytes;
+            HIP_TRY(rocprim::reduce(V->buf[B::TMP].p, bytes, (const uint32_t *)ia.cnt, dmax, 0u, nF,
+                                    rocprim::maximum<uint32_t>(), s));
+            uint32_t maxn = 0;
+            HIP_TRY(hipMemcpyAsync(&maxn, dmax, 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (maxn <= (uint32_t)kSortCap) {
+                // two size classes, so that the common short streamlines get
+                // small LDS buffers and many wavefronts per CU
+                int ntop = kWave;
+                while (ntop < (int)maxn) ntop <<= 1;
+                const int nsmall = ntop < kSortSmall ? ntop : kSortSmall;
+                const size_t slot = sizeof(uint64_t) + sizeof(VT) + 2 * sizeof(uint32_t);
+                for (const int2 c : {int2{-1, nsmall}, int2{nsmall, ntop}}) {
+                    if (c.x >= c.y) continue;
+                    const int wpb = 1;                        // wavefronts per work-group
+                    k_vox_sortmerge<VT><<<(unsigned)((nF + wpb - 1) / wpb), wpb * kWave, (size_t)wpb * c.y * slot, s>>>(
+                        d_off, d_end, n_count, (uint32_t *)K1, V1, nvox, V->at<float>(B::FLEN), ia.min_fib,
+                        ia.max_fib, c.y, c.x, L1, d_mcnt, d_mend, d_kept, V->at<float>(B::NORM));
+                    HIP_TRY(hipGetLastError());
+                }
+                fused = true;
+            }
+        }
     }
-    // per-voxel order inside each kept streamline: voxel keys into K2's
-    // memory, lengths into V2's
-    uint32_t *VS = (uint32_t *)K2;
-    float *L2 = (float *)V2;
-    if ((rc = segsort(V, (const uint32_t *)VK, VS, (const float *)L1, L2, np, n_count, d_off, d_mend, ia.vbits, s)))
-        return rc;
-    if (nF) {
-        k_vox_norm<<<gW, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, VS, L2, V->at<float>(B::NORM));
-        HIP_TRY(hipGetLastError());
+    if (!fused) {
+        if ((rc = segsort(V, (const KT *)K1, K2, (const VT *)V1, V2, np, n_count, d_off, d_end, ia.kbits, s)))
+            return rc;
+        // merged segments: keys to K1, float lengths to L1, voxel keys into V1's memory
+        uint32_t *VK = (uint32_t *)V1;
+        if (nF) {
+            k_vox_merge<KT, VT><<<gW, kVoxBlock, 0, s>>>(d_off, d_end, n_count, K2, V2, nvox, V->at<float>(B::FLEN),
+                                                         ia.min_fib, ia.max_fib, K1, L1, VK, d_mcnt, d_mend, d_kept);
+            HIP_TRY(hipGetLastError());
+        }
+        // per-voxel order inside each kept streamline: voxel keys into K2's
+        // memory, lengths into V2's
+        uint32_t *VS = (uint32_t *)K2;
+        float *L2 = (float *)V2;
+        if ((rc = segsort(V, (const uint32_t *)VK, VS, (const float *)L1, L2, np, n_count, d_off, d_mend, ia.vbits,
+                          s)))
+            return rc;
+        if (nF) {
+            k_vox_norm<<<gW, kVoxBlock, 0, s>>>(d_off, d_mend, n_count, VS, L2, V->at<float>(B::NORM));
+            HIP_TRY(hipGetLastError());
+        }
     }
     uint64_t *d_kidx = V->at<uint64_t>(B::KIDX), *d_spos = V->at<uint64_t>(B::SPOS);
     if ((rc = exscan_u32(V, d_kept, d_kidx, nF + 1, s))) return rc;
