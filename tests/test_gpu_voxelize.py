@@ -112,6 +112,35 @@ def test_regions_outgrown():
     same(g, o)
 
 
+def _helix(npts, dim, turns, seed):
+    """A helix of npts points through the grid, steps of about 0.4 voxel."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0.0, 1.0, npts)
+    c = np.asarray(dim, np.float64) / 2
+    r = min(dim[0], dim[1]) * 0.35
+    ph = rng.uniform(0, 2 * np.pi)
+    pts = np.stack([c[0] + r * np.cos(2 * np.pi * turns * t + ph), c[1] + r * np.sin(2 * np.pi * turns * t + ph),
+                    1.0 + (dim[2] - 2.0) * t], axis=1)
+    return pts.astype(np.float32)
+
+
+@pytest.mark.parametrize("longest,turns", [(900, 3.0), (2600, 9.0)])
+def test_long_streamlines(longest, turns):
+    """Streamlines of hundreds to thousands of pieces: up to 512 pieces the
+    fused sort-merge kernel sorts in registers, up to 2,048 in LDS, and a
+    tractogram with a longer streamline takes the segmented sorts instead."""
+    dim = (64, 64, 48)
+    fibs = [_helix(n, dim, tn, k) for k, (n, tn) in enumerate([(longest, turns), (300, 1.0), (60, 0.2)])]
+    pts0, ptr0 = synth.streamlines(1013, 200, dim, 1.0, mode=0, step_vox=0.5)
+    pts = np.concatenate([pts0] + fibs)
+    ptr = np.concatenate([ptr0, ptr0[-1] + np.cumsum([len(f) for f in fibs])]).astype(np.int64)
+    g, o = both(pts, ptr, dim, (1.0, 1.0, 1.0), max_fiber_len=5000.0)
+    kept_f = np.nonzero(g["TRK_kept"])[0]
+    per = np.bincount(g["IC_f"], minlength=kept_f.size)
+    assert per.max() > (2048 if longest > 2000 else 512) // 2
+    same(g, o, longest)
+
+
 def test_peaks_iso_flips_affine(small):
     pts, ptr, dim = small
     rng = np.random.default_rng(5)
