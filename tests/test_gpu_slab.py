@@ -195,10 +195,11 @@ def test_slab_runs_of_one_orientation(mode, zero_frac):
     assert max(e) <= TOL
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_At_combined(mode):
     """commit_hip_At_combined: A^T (sqrtw .* (a r1 + b r2)) and ||a r1 + b r2||^2
-    in one call.  On the slab path (mode 2) the combination is formed while
+    in one call.  On the slab paths (modes 1 and 2: 8 and 16 wavefronts per
+    work-group) the combination is formed while
     A^T loads its rows: against the oracle's A^T of the combined residual
     (1e-12 in FP64, 1e-6 with FP32 storage) and against the two calls it
     replaces (commit_hip_residual_combine, commit_hip_At_weighted).  Slabs off
