@@ -64,6 +64,7 @@ constexpr int kMaxFibLen = 10000;      // MAX_FIB_LEN, trk2dictionary_c.cpp:16
 struct VoxArgs {
     int X, Y, Z;
     float px, py, pz;
+    double rpx, rpy, rpz;              // 1.0 / (double)pixdim, correctly rounded (floor_div)
     float sx, sy, sz;                  // shift in mm: fiber_shift * pixdim, float (:125-127)
     float min_seg, min_fib, max_fib;
     int intersect;
@@ -94,12 +95,40 @@ __device__ __forceinline__ D3 vnormalize(D3 a)
     return a;
 }
 
+// floor(x / p) with x / p the correctly rounded IEEE quotient, from r =
+// RN(1 / p).  q0 = RN(x r) is within 3.0001 u |x / p| of RN(x / p) (u =
+// 2^-53: one rounding in r, one in the product, one in the quotient), so when
+// q0 lies farther than 2^-50 |q0| from every integer the two have the same
+// floor; otherwise (about one case in 10^13 for these magnitudes) the quotient
+// is formed by IEEE division.  q0 - floor(q0) and ceil(q0) - q0 are exact
+// whenever either is below 1/2.  Needs no overflow or underflow: positions
+// and voxel sizes are millimetres.
+__device__ __forceinline__ double floor_div(double x, double p, double r)
+{
+    const double q0 = x * r;
+    const double f = floor(q0), eps = fabs(q0) * 0x1p-50;
+    if (q0 - f > eps && ceil(q0) - q0 > eps) return f;
+    return floor(x / p);
+}
+
+// round(x / M_PI * 180.0) the same way: m0 = RN(RN(x / pi) 180) is within
+// about 5 u |m| of the IEEE chain's m, and round() changes value only at
+// half-integers; near one (within 2^-48 |m0|) the chain itself is evaluated.
+__device__ __forceinline__ double round_deg(double x)
+{
+    constexpr double kInvPi = 1.0 / M_PI;
+    const double m0 = x * kInvPi * 180.0;
+    const double fr = m0 - floor(m0), eps = fabs(m0) * 0x1p-48;
+    if (fabs(fr - 0.5) > eps) return round(m0);
+    return round(x / M_PI * 180.0);
+}
+
 // ptrHashTable[ox*181 + oy] with the angles rounded to degrees as
 // segmentForwardModel (:799-802) and ECSegments (:371-374) round them
 __device__ __forceinline__ int hash_index(const short *hash, double colat, double lon)
 {
-    int ox = (int)round(colat / M_PI * 180.0);
-    int oy = (int)round(lon / M_PI * 180.0);
+    int ox = (int)round_deg(colat);
+    int oy = (int)round_deg(lon);
     long long idx = (long long)ox * 181 + oy;
     if (idx < 0) idx = 0;              // the reference reads before its table here
     if (idx >= 181 * 181) idx = 181 * 181 - 1;
@@ -116,11 +145,12 @@ __device__ __forceinline__ long long max_cross(double len, const VoxArgs &a)
     return 64 + 4 * (long long)c;
 }
 
-// rayBoxIntersection (trk2dictionary_c.cpp:814-869)
-__device__ __forceinline__ bool raybox(D3 o, D3 d, D3 vmin, D3 vmax, double &t)
+// rayBoxIntersection (trk2dictionary_c.cpp:814-869); inv = {1 / d.x, 1 / d.y,
+// 1 / d.z} of the ray direction, which is fixed for a whole polyline step and
+// is formed once per step by the caller
+__device__ __forceinline__ bool raybox(D3 o, D3 inv, D3 vmin, D3 vmax, double &t)
 {
     double tmin, tmax, tymin, tymax, tzmin, tzmax;
-    D3 inv = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
     if (inv.x >= 0) { tmin = (vmin.x - o.x) * inv.x; tmax = (vmax.x - o.x) * inv.x; }
     else            { tmin = (vmax.x - o.x) * inv.x; tmax = (vmin.x - o.x) * inv.x; }
     if (inv.y >= 0) { tymin = (vmin.y - o.y) * inv.y; tymax = (vmax.y - o.y) * inv.y; }
@@ -138,20 +168,46 @@ __device__ __forceinline__ bool raybox(D3 o, D3 d, D3 vmin, D3 vmax, double &t)
     return true;
 }
 
+// Pieces a lane holds in LDS before it writes them out.  Every lane writes
+// its own streamline's region, so one store instruction touches 64 cache
+// lines; stored one piece per crossing iteration, those lines were evicted
+// from L2 between consecutive pieces and written back many times over.
+// Written in runs of up to kRecBuf consecutive pieces, each line is filled in
+// one burst.
+constexpr int kRecBuf = 16;
+
 // One streamline's recorder: counts pieces (EMIT = false) or stores them,
 // keys as KT (32 bits when every key fits) and values w * len as VT (float
 // when w is 1, so that w * len is the float length itself, exactly).
+// EMIT: the pieces are buffered in this lane's slots of its wavefront's LDS
+// buffer (lk, lv: slot 0; slot e at e * kWave) and written out by flush().
 template <bool EMIT, class KT, class VT>
 struct Recorder {
     const VoxArgs &a;
     KT *keys;
     VT *vals;
+    KT *lk;
+    VT *lv;
     uint32_t limit;                    // the count pass's total: stores never pass it
     uint32_t n = 0;
+    uint32_t nb = 0, nf = 0;           // pieces buffered, pieces written
     float flen = 0.0f, flentot = 0.0f;
 
-    __device__ Recorder(const VoxArgs &a_, KT *k, VT *v, uint32_t lim)
-        : a(a_), keys(k), vals(v), limit(lim) {}
+    __device__ Recorder(const VoxArgs &a_, KT *k, VT *v, KT *lk_, VT *lv_, uint32_t lim)
+        : a(a_), keys(k), vals(v), lk(lk_), lv(lv_), limit(lim) {}
+
+    // every active lane writes its buffered pieces, in recording order, to
+    // slots nf .. nf + nb - 1 of its region (all below limit: only pieces
+    // with n < limit are buffered)
+    __device__ void flush()
+    {
+        for (uint32_t e = 0; e < nb; e++) {
+            keys[nf + e] = lk[e * kWave];
+            vals[nf + e] = lv[e * kWave];
+        }
+        nf += nb;
+        nb = 0;
+    }
 
     // segmentForwardModel (trk2dictionary_c.cpp:761-808)
     __device__ void segment(D3 P1, D3 P2, int k, double w)
@@ -169,9 +225,9 @@ struct Recorder {
         double len = vnorm(dir);
         if (w * len <= a.min_seg) return;
         dir = vnormalize(dir);
-        int vx = (int)floor(0.5 * (P1.x + P2.x) / a.px);
-        int vy = (int)floor(0.5 * (P1.y + P2.y) / a.py);
-        int vz = (int)floor(0.5 * (P1.z + P2.z) / a.pz);
+        int vx = (int)floor_div(0.5 * (P1.x + P2.x), a.px, a.rpx);
+        int vy = (int)floor_div(0.5 * (P1.y + P2.y), a.py, a.rpy);
+        int vz = (int)floor_div(0.5 * (P1.z + P2.z), a.pz, a.rpz);
         if (vx >= a.X || vx < 0 || vy >= a.Y || vy < 0 || vz >= a.Z || vz < 0) return;
         if (a.mask) {                  // ptrMASK[...] == 0, from a bit array that stays in L2
             const uint64_t mv = (uint64_t)vz + (uint64_t)a.Z * ((uint64_t)vy + (uint64_t)a.Y * vx);
@@ -182,12 +238,15 @@ struct Recorder {
             double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
             unsigned short o = (unsigned short)hash_index(a.hash, colat, lon);
             uint64_t v = (uint64_t)vx + (uint64_t)a.X * ((uint64_t)vy + (uint64_t)a.Y * vz);
-            keys[n] = (KT)((uint64_t)o * a.nvox + v);
-            vals[n] = (VT)(w * len);
+            lk[nb * kWave] = (KT)((uint64_t)o * a.nvox + v);
+            lv[nb * kWave] = (VT)(w * len);
+            nb++;
         }
         n++;
         flentot = (float)((double)flentot + w * len);
         if (k == 0) flen = (float)((double)flen + len);
+        // a full buffer in any active lane: the active lanes write theirs
+        if (EMIT && __ballot(nb == (uint32_t)kRecBuf) != 0) flush();
     }
 };
 
@@ -209,11 +268,16 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                                                         float *__restrict__ flentot, uint32_t *__restrict__ status,
                                                         const uint32_t *__restrict__ order)
 {
+    // EMIT: per wavefront, kRecBuf slots of 64 lanes for keys and values
+    constexpr int kBuf = EMIT ? kVoxBlock * kRecBuf : 1;
+    __shared__ KT lk_s[kBuf];
+    __shared__ VT lv_s[kBuf];
     const int t = blockIdx.x * kVoxBlock + threadIdx.x;
     if (t >= nF) return;
     const int f = (int)order[t];
-    Recorder<EMIT, KT, VT> rec(a, EMIT ? keys + off[f] : nullptr, EMIT ? vals + off[f] : nullptr,
-                       EMIT ? (uint32_t)(off[f + 1] - off[f]) : 0u);
+    const int lslot = EMIT ? (int)(threadIdx.x >> 6) * kRecBuf * kWave + (int)(threadIdx.x & (kWave - 1)) : 0;
+    Recorder<EMIT, KT, VT> rec(a, EMIT ? keys + off[f] : nullptr, EMIT ? vals + off[f] : nullptr, lk_s + lslot,
+                               lv_s + lslot, EMIT ? (uint32_t)(off[f + 1] - off[f]) : 0u);
     const int64_t p0 = fptr[f];
     const int pts_n = (int)(fptr[f + 1] - p0);
     const float *fib = pts + 3 * p0;
@@ -261,6 +325,8 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                 nn.x /= dot;
                 nn.y /= dot;
                 nn.z /= dot;
+                D3 inv = {0, 0, 0};             // 1 / dir1, formed at the step's first crossing
+                bool have_inv = false;
                 for (int k = 0; k < a.n_rep; k++) {
                     if (!blur && k > 0) continue;
                     if (a.wt[k] < 1e-3) continue;
@@ -287,20 +353,25 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                             capped = true;
                             break;
                         }
-                        if (floor(S1m.x / a.px) == floor(S2m.x / a.px) && floor(S1m.y / a.py) == floor(S2m.y / a.py) &&
-                            floor(S1m.z / a.pz) == floor(S2m.z / a.pz)) {
+                        if (floor_div(S1m.x, a.px, a.rpx) == floor_div(S2m.x, a.px, a.rpx) &&
+                            floor_div(S1m.y, a.py, a.rpy) == floor_div(S2m.y, a.py, a.rpy) &&
+                            floor_div(S1m.z, a.pz, a.rpz) == floor_div(S2m.z, a.pz, a.rpz)) {
                             rec.segment(S1m, S2m, k, wk);
                             break;
                         }
                         D3 vmin, vmax;
-                        vmin.x = floor((S1m.x + 1e-6 * dir1.x) / a.px) * a.px;
-                        vmin.y = floor((S1m.y + 1e-6 * dir1.y) / a.py) * a.py;
-                        vmin.z = floor((S1m.z + 1e-6 * dir1.z) / a.pz) * a.pz;
+                        vmin.x = floor_div(S1m.x + 1e-6 * dir1.x, a.px, a.rpx) * a.px;
+                        vmin.y = floor_div(S1m.y + 1e-6 * dir1.y, a.py, a.rpy) * a.py;
+                        vmin.z = floor_div(S1m.z + 1e-6 * dir1.z, a.pz, a.rpz) * a.pz;
                         vmax.x = vmin.x + a.px;
                         vmax.y = vmin.y + a.py;
                         vmax.z = vmin.z + a.pz;
+                        if (!have_inv) {
+                            inv = {1.0 / dir1.x, 1.0 / dir1.y, 1.0 / dir1.z};
+                            have_inv = true;
+                        }
                         double t;
-                        if (raybox(S1m, dir1, vmin, vmax, t) && t > 0 && t < sl) {
+                        if (raybox(S1m, inv, vmin, vmax, t) && t > 0 && t < sl) {
                             D3 Pi = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
                             rec.segment(S1m, Pi, k, wk);
                             S1m = Pi;
@@ -328,6 +399,8 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
             if (pts_n > 3) pd = {fib[9], fib[10], fib[11]};
             D3 S1m = {0, 0, 0}, S2m = {0, 0, 0}, dir1 = {0, 0, 0};
             double f2x = 0, f2y = 0, f2z = 0;               // floor(S2m / pixdim), fixed for a step
+            D3 inv = {0, 0, 0};                             // 1 / dir1, formed at the step's first crossing
+            bool have_inv = false;
             long long it = 0, cap = -1;
             for (;;) {
                 if (!cross) {
@@ -365,9 +438,10 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                         }
                         S1m = Po;
                         S2m = Pn;
-                        f2x = floor(S2m.x / a.px);
-                        f2y = floor(S2m.y / a.py);
-                        f2z = floor(S2m.z / a.pz);
+                        f2x = floor_div(S2m.x, a.px, a.rpx);
+                        f2y = floor_div(S2m.y, a.py, a.rpy);
+                        f2z = floor_div(S2m.z, a.pz, a.rpz);
+                        have_inv = false;
                         it = 0;
                         cap = -1;
                         cross = true;
@@ -391,16 +465,21 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
                 it++;
                 D3 Pb = S2m;
                 bool fin = true;
-                if (!(floor(S1m.x / a.px) == f2x && floor(S1m.y / a.py) == f2y && floor(S1m.z / a.pz) == f2z)) {
+                if (!(floor_div(S1m.x, a.px, a.rpx) == f2x && floor_div(S1m.y, a.py, a.rpy) == f2y &&
+                      floor_div(S1m.z, a.pz, a.rpz) == f2z)) {
                     D3 vmin, vmax;
-                    vmin.x = floor((S1m.x + 1e-6 * dir1.x) / a.px) * a.px;
-                    vmin.y = floor((S1m.y + 1e-6 * dir1.y) / a.py) * a.py;
-                    vmin.z = floor((S1m.z + 1e-6 * dir1.z) / a.pz) * a.pz;
+                    vmin.x = floor_div(S1m.x + 1e-6 * dir1.x, a.px, a.rpx) * a.px;
+                    vmin.y = floor_div(S1m.y + 1e-6 * dir1.y, a.py, a.rpy) * a.py;
+                    vmin.z = floor_div(S1m.z + 1e-6 * dir1.z, a.pz, a.rpz) * a.pz;
                     vmax.x = vmin.x + a.px;
                     vmax.y = vmin.y + a.py;
                     vmax.z = vmin.z + a.pz;
+                    if (!have_inv) {
+                        inv = {1.0 / dir1.x, 1.0 / dir1.y, 1.0 / dir1.z};
+                        have_inv = true;
+                    }
                     double t;
-                    if (raybox(S1m, dir1, vmin, vmax, t) && t > 0 && t < sl) {
+                    if (raybox(S1m, inv, vmin, vmax, t) && t > 0 && t < sl) {
                         Pb = {S1m.x + t * dir1.x, S1m.y + t * dir1.y, S1m.z + t * dir1.z};
                         fin = false;
                     }
@@ -411,6 +490,7 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_walk(VoxArgs a, const float *
             }
         }
     }
+    if constexpr (EMIT) rec.flush();
     if (!EMIT || BOUNDED) {
         cnt[f] = rec.n;
         flen[f] = rec.flen;
@@ -1346,6 +1426,7 @@ extern "C" int commit_hip_trk2dictionary(commit_hip_vox *V, const float *fibers,
     VoxArgs a;
     a.X = Nx; a.Y = Ny; a.Z = Nz;
     a.px = Px; a.py = Py; a.pz = Pz;
+    a.rpx = 1.0 / (double)a.px; a.rpy = 1.0 / (double)a.py; a.rpz = 1.0 / (double)a.pz;
     a.sx = fiber_shiftX * Px; a.sy = fiber_shiftY * Py; a.sz = fiber_shiftZ * Pz;
     a.min_seg = min_seg_len; a.min_fib = min_fiber_len; a.max_fib = max_fiber_len;
     a.intersect = c > 0;
