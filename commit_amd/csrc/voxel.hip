@@ -123,16 +123,39 @@ __device__ __forceinline__ double round_deg(double x)
     return round(x / M_PI * 180.0);
 }
 
-// ptrHashTable[ox*181 + oy] with the angles rounded to degrees as
-// segmentForwardModel (:799-802) and ECSegments (:371-374) round them
-__device__ __forceinline__ int hash_index(const short *hash, double colat, double lon)
+// ptrHashTable[ox*181 + oy] for angles already rounded to degrees
+__device__ __forceinline__ int hash_lookup(const short *hash, int ox, int oy)
 {
-    int ox = (int)round_deg(colat);
-    int oy = (int)round_deg(lon);
     long long idx = (long long)ox * 181 + oy;
     if (idx < 0) idx = 0;              // the reference reads before its table here
     if (idx >= 181 * 181) idx = 181 * 181 - 1;
     return hash[idx];
+}
+
+// ptrHashTable[ox*181 + oy] with the angles rounded to degrees as
+// segmentForwardModel (:799-802) and ECSegments (:371-374) round them
+__device__ __forceinline__ int hash_index(const short *hash, double colat, double lon)
+{
+    return hash_lookup(hash, (int)round_deg(colat), (int)round_deg(lon));
+}
+
+// The two rounded angles of segmentForwardModel (:799-802) for a direction
+// with d.y >= 0, from FP32 arithmetic: the components rounded to float move
+// each angle by at most about 1e-7 rad, atan2f and the float square root add
+// a few ulp (under 1e-6 rad), and the scaling to degrees under 2e-5 degrees,
+// so an angle farther than 1e-3 degrees from a half degree rounds as the FP64
+// one does.  Returns false, leaving the FP64 path to the caller, when either
+// angle is nearer than that (about 0.4% of pieces).
+__device__ __forceinline__ bool degrees_f32(D3 d, int &ox, int &oy)
+{
+    const float x = (float)d.x, y = (float)d.y, z = (float)d.z;
+    const float kDeg = 57.29577951308232f;           // 180 / pi
+    const float mx = atan2f(sqrtf(x * x + y * y), z) * kDeg, my = atan2f(y, x) * kDeg;
+    const float fx = mx - floorf(mx), fy = my - floorf(my);
+    if (!(fabsf(fx - 0.5f) > 1e-3f && fabsf(fy - 0.5f) > 1e-3f)) return false;
+    ox = (int)floorf(mx + 0.5f);
+    oy = (int)floorf(my + 0.5f);
+    return true;
 }
 
 // cap of one sub-step's voxel-crossing loop (the reference loops unbounded)
@@ -234,9 +257,14 @@ struct Recorder {
             if (!((a.mbits[mv >> 6] >> (mv & 63)) & 1ull)) return;
         }
         if (EMIT && n < limit) {
-            double lon = atan2(dir.y, dir.x);
-            double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
-            unsigned short o = (unsigned short)hash_index(a.hash, colat, lon);
+            int ox, oy;
+            if (!degrees_f32(dir, ox, oy)) {
+                double lon = atan2(dir.y, dir.x);
+                double colat = atan2(sqrt(dir.x * dir.x + dir.y * dir.y), dir.z);
+                ox = (int)round_deg(colat);
+                oy = (int)round_deg(lon);
+            }
+            unsigned short o = (unsigned short)hash_lookup(a.hash, ox, oy);
             uint64_t v = (uint64_t)vx + (uint64_t)a.X * ((uint64_t)vy + (uint64_t)a.Y * vz);
             lk[nb * kWave] = (KT)((uint64_t)o * a.nvox + v);
             lv[nb * kWave] = (VT)(w * len);
