@@ -662,10 +662,11 @@ __global__ __launch_bounds__(kVoxBlock) void k_vox_norm(const uint64_t *__restri
     if (lane == 0) norm[f] = (float)sqrt((double)fn);
 }
 
-// Streamlines of at most kSortCap pieces take k_vox_sortmerge, those of at
-// most kSortSmall in a launch with small LDS buffers.
+// Streamlines of at most kSortCap pieces take k_vox_sortmerge, in launches
+// by size class: at most kSortSmall[0] pieces, at most kSortSmall[1], and the
+// rest, each with LDS slices sized to its class.
 constexpr int kSortCap = 2048;
-constexpr int kSortSmall = 512;
+constexpr int kSortSmall[2] = {256, 512};
 
 // One wavefront's LDS writes done and visible to all its lanes.
 __device__ __forceinline__ void wave_lds_sync()
@@ -1298,13 +1299,14 @@ int ic_segments(commit_hip_vox *V, const IcArgs &ia, size_t *n_ic_out, size_t *n
             HIP_TRY(hipMemcpyAsync(&maxn, dmax, 4, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (maxn <= (uint32_t)kSortCap) {
-                // two size classes, so that the common short streamlines get
+                // size classes, so that the common short streamlines get
                 // small LDS buffers and many wavefronts per CU
                 int ntop = kWave;
                 while (ntop < (int)maxn) ntop <<= 1;
-                const int nsmall = ntop < kSortSmall ? ntop : kSortSmall;
+                const int n1 = ntop < kSortSmall[0] ? ntop : kSortSmall[0];
+                const int n2 = ntop < kSortSmall[1] ? ntop : kSortSmall[1];
                 const size_t slot = sizeof(uint64_t) + sizeof(VT) + 2 * sizeof(uint32_t);
-                for (const int2 c : {int2{-1, nsmall}, int2{nsmall, ntop}}) {
+                for (const int2 c : {int2{-1, n1}, int2{n1, n2}, int2{n2, ntop}}) {
                     if (c.x >= c.y) continue;
                     const int wpb = 1;                        // wavefronts per work-group
                     k_vox_sortmerge<VT><<<(unsigned)((nF + wpb - 1) / wpb), wpb * kWave, (size_t)wpb * c.y * slot, s>>>(
