@@ -133,47 +133,69 @@ __device__ __forceinline__ void a_comp(double (&acc)[NJ], const Comp &cp, uint32
     }
 }
 
-// Output stage of A for voxel v: y row (or the residual sqrtw .* (A x - yobs))
-// and the voxel's partial of ||r||^2.                  (solvers.py:306-308)
-// RES false: a plain product (out.yobs, out.sqrtw and out.rpart are null).
-template <int NJ, typename T, bool RES = true>
-__device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &d, int v, const AOut<T> &out, int lane)
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// A row of T (p: its sample 0) moved in the lane layout Lanes<NJ, kQuads<T>>.
+// With the float layout a lane's quad is 4 consecutive samples: moved as 4
+// separate 4-byte accesses, each instruction touches the same 1 KiB of lines
+// at a 16-byte lane stride, and with the streaming hint the lines can leave
+// L2 between the quad's four accesses, so the row is fetched (or written back)
+// several times.  When the row is 16-byte aligned (nS a multiple of 4 and p
+// aligned) a quad is one 16-byte access.
+__device__ __forceinline__ bool quad_ok(const void *p, int nS)
 {
-    const size_t row = (size_t)v * d.nS;
-    if constexpr (!RES) {
+    return (nS & 3) == 0 && ((uintptr_t)p & 15) == 0;
+}
+
+// the lane's NJ samples of the row; a sample past nS reads sample nS - 1
+// (the caller masks it)
+template <int NJ, typename T>
+__device__ __forceinline__ void row_load(T (&v)[NJ], const T *__restrict__ p, int nS, int lane)
+{
+    constexpr bool Q = kQuads<T>;
+    constexpr int NQ = Lanes<NJ, Q>::NQ;
+    const bool vec = Q && quad_ok(p, nS);
 #pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
-            if (s < d.nS) __builtin_nontemporal_store((T)acc[j], out.y + row + s);
-        }
-        return;
-    }
-    double sq = 0.0;
-    // the observations and weights are all requested before the first store:
-    // a load issued after a store would wait for that store as well
-    T yo[NJ], wv[NJ];
+    for (int q = 0; q < NQ; q++) {
+        const int s0 = 256 * q + 4 * lane;
+        if (vec && s0 + 3 < nS) {
+            const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(p + s0));
+            v[4 * q] = (T)t.x, v[4 * q + 1] = (T)t.y, v[4 * q + 2] = (T)t.z, v[4 * q + 3] = (T)t.w;
+        } else {
 #pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
-        const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
-        yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
-        wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
-    }
-#pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
-        if (s < d.nS) {
-            double r = acc[j];
-            if (out.yobs) r -= (double)yo[j];
-            if (out.sqrtw) r *= (double)wv[j];
-            if (out.y) __builtin_nontemporal_store((T)r, out.y + row + s);
-            sq = fma(r, r, sq);
+            for (int e = 0; e < 4; e++) v[4 * q + e] = stream_load(p + (s0 + e < nS ? s0 + e : nS - 1));
         }
     }
-    if (out.rpart) {
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
-        if (lane == 0) out.rpart[v] = sq;
+    for (int j = 4 * NQ; j < NJ; j++) {
+        const int s = Lanes<NJ, Q>::sample(lane, j);
+        v[j] = stream_load(p + (s < nS ? s : nS - 1));
+    }
+}
+
+// the lane's samples below nS of the row, stored
+template <int NJ, typename T>
+__device__ __forceinline__ void row_store(const T (&v)[NJ], T *__restrict__ p, int nS, int lane)
+{
+    constexpr bool Q = kQuads<T>;
+    constexpr int NQ = Lanes<NJ, Q>::NQ;
+    const bool vec = Q && quad_ok(p, nS);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int s0 = 256 * q + 4 * lane;
+        if (vec && s0 + 3 < nS) {
+            const f4v t = {(float)v[4 * q], (float)v[4 * q + 1], (float)v[4 * q + 2], (float)v[4 * q + 3]};
+            __builtin_nontemporal_store(t, reinterpret_cast<f4v *>(p + s0));
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (s0 + e < nS) __builtin_nontemporal_store(v[4 * q + e], p + s0 + e);
+        }
+    }
+#pragma unroll
+    for (int j = 4 * NQ; j < NJ; j++) {
+        const int s = Lanes<NJ, Q>::sample(lane, j);
+        if (s < nS) __builtin_nontemporal_store(v[j], p + s);
     }
 }
 
@@ -185,12 +207,23 @@ __device__ __forceinline__ void res_load(T (&yo)[NJ], T (&wv)[NJ], const DevDict
                                          int lane)
 {
     const size_t row = (size_t)v * d.nS;
+    if constexpr (kQuads<T>) {
+        if (out.yobs) row_load<NJ, T>(yo, out.yobs + row, d.nS, lane);
+        else
 #pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
-        const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
-        yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
-        wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
+            for (int j = 0; j < NJ; j++) yo[j] = (T)0;
+        if (out.sqrtw) row_load<NJ, T>(wv, out.sqrtw + row, d.nS, lane);
+        else
+#pragma unroll
+            for (int j = 0; j < NJ; j++) wv[j] = (T)1;
+    } else {
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int s = Lanes<NJ, false>::sample(lane, j);
+            const size_t i = row + (size_t)(s < d.nS ? s : d.nS - 1);
+            yo[j] = out.yobs ? stream_load(out.yobs + i) : (T)0;
+            wv[j] = out.sqrtw ? stream_load(out.sqrtw + i) : (T)1;
+        }
     }
 }
 
@@ -200,15 +233,29 @@ __device__ __forceinline__ void res_store(const double (&acc)[NJ], const T (&yo)
 {
     const size_t row = (size_t)v * d.nS;
     double sq = 0.0;
+    if constexpr (kQuads<T>) {
+        T rv[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
-        if (s < d.nS) {
+        for (int j = 0; j < NJ; j++) {
+            const int s = Lanes<NJ, true>::sample(lane, j);
             double r = acc[j];
             if (out.yobs) r -= (double)yo[j];
             if (out.sqrtw) r *= (double)wv[j];
-            if (out.y) __builtin_nontemporal_store((T)r, out.y + row + s);
-            sq = fma(r, r, sq);
+            rv[j] = (T)r;
+            if (s < d.nS) sq = fma(r, r, sq);
+        }
+        if (out.y) row_store<NJ, T>(rv, out.y + row, d.nS, lane);
+    } else {
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int s = Lanes<NJ, false>::sample(lane, j);
+            if (s < d.nS) {
+                double r = acc[j];
+                if (out.yobs) r -= (double)yo[j];
+                if (out.sqrtw) r *= (double)wv[j];
+                if (out.y) __builtin_nontemporal_store((T)r, out.y + row + s);
+                sq = fma(r, r, sq);
+            }
         }
     }
     if (out.rpart) {
@@ -216,6 +263,35 @@ __device__ __forceinline__ void res_store(const double (&acc)[NJ], const T (&yo)
         for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
         if (lane == 0) out.rpart[v] = sq;
     }
+}
+
+// Output stage of A for voxel v: y row (or the residual sqrtw .* (A x - yobs))
+// and the voxel's partial of ||r||^2.                  (solvers.py:306-308)
+// RES false: a plain product (out.yobs, out.sqrtw and out.rpart are null).
+template <int NJ, typename T, bool RES = true>
+__device__ __forceinline__ void a_store(const double (&acc)[NJ], const DevDict &d, int v, const AOut<T> &out, int lane)
+{
+    if constexpr (!RES) {
+        if constexpr (kQuads<T>) {
+            T yv[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) yv[j] = (T)acc[j];
+            row_store<NJ, T>(yv, out.y + (size_t)v * d.nS, d.nS, lane);
+        } else {
+            const size_t row = (size_t)v * d.nS;
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                const int s = Lanes<NJ, false>::sample(lane, j);
+                if (s < d.nS) __builtin_nontemporal_store((T)acc[j], out.y + row + s);
+            }
+        }
+        return;
+    }
+    // the observations and weights are all requested before the first store:
+    // a load issued after a store would wait for that store as well
+    T yo[NJ], wv[NJ];
+    res_load<NJ, T>(yo, wv, d, v, out, lane);
+    res_store<NJ, T>(acc, yo, wv, d, v, out, lane);
 }
 
 // ---- cross-lane moves of doubles on the VALU (no LDS traffic) --------------

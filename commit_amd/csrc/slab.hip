@@ -541,12 +541,21 @@ __device__ __forceinline__ void at_slab(const DevDict &d, const T *__restrict__ 
     for (int u = 0; u < V; u++) {
         const size_t row = (size_t)(vox[u] < 0 ? 0 : vox[u]) * d.nS;
         T raw[NJ], wv[NJ], r2v[NJ];
+        if constexpr (kQuads<T>) {
+            row_load<NJ, T>(raw, y + row, d.nS, lane);
+            if constexpr (COMB) row_load<NJ, T>(r2v, cb.r2 + row, d.nS, lane);
+            if (sqrtw) row_load<NJ, T>(wv, sqrtw + row, d.nS, lane);
+            else
 #pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            const int s = Lanes<NJ, kQuads<T>>::sample(lane, j);
-            raw[j] = stream_load(y + row + (s < d.nS ? s : 0));
-            if constexpr (COMB) r2v[j] = stream_load(cb.r2 + row + (s < d.nS ? s : 0));
-            wv[j] = sqrtw ? stream_load(sqrtw + row + (s < d.nS ? s : 0)) : (T)1;
+                for (int j = 0; j < NJ; j++) wv[j] = (T)1;
+        } else {
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                const int s = Lanes<NJ, false>::sample(lane, j);
+                raw[j] = stream_load(y + row + (s < d.nS ? s : 0));
+                if constexpr (COMB) r2v[j] = stream_load(cb.r2 + row + (s < d.nS ? s : 0));
+                wv[j] = sqrtw ? stream_load(sqrtw + row + (s < d.nS ? s : 0)) : (T)1;
+            }
         }
         double ss = 0.0;
 #pragma unroll
