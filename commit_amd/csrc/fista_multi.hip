@@ -44,6 +44,27 @@ struct Seg {
     int S, R;
 };
 
+// t = i R + r (0 <= r < R) for the elementwise kernels: shifts when R is a
+// power of two (sh = log2 R, the usual 16 columns), a 32-bit division while
+// t fits, a 64-bit one otherwise.  A 64-bit division by a run-time R per
+// element made these kernels several times slower than their memory traffic.
+__device__ __forceinline__ int col_shift(int R) { return (R & (R - 1)) == 0 ? __builtin_ctz((unsigned)R) : -1; }
+
+__device__ __forceinline__ void split_rc(int64_t t, int R, int sh, int64_t &i, int &r)
+{
+    if (sh >= 0) {
+        i = t >> sh;
+        r = (int)(t & (int64_t)(R - 1));
+    } else if (t < 0x100000000ll) {
+        const uint32_t q = (uint32_t)t / (uint32_t)R;
+        i = (int64_t)q;
+        r = (int)((uint32_t)t - q * (uint32_t)R);
+    } else {
+        i = t / R;
+        r = (int)(t - i * R);
+    }
+}
+
 __device__ __forceinline__ double soft(double v, double t)
 {
     if (v > t) return v - t;                      // proximals.pyx:25-56
@@ -58,11 +79,12 @@ __global__ __launch_bounds__(kBlock) void k_mprox_elem(MProx P, Seg G, int64_t n
                                                        const float *__restrict__ grad, float *__restrict__ x,
                                                        const Col *__restrict__ col)
 {
-    const int R = G.R;
+    const int R = G.R, sh = col_shift(R);
     const int64_t total = n * R;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % R);
-        const int64_t i = t / R;
+        int64_t i;
+        int r;
+        split_rc(t, R, sh, i, r);
         const Col &c = col[G.segx[i] * R + r];
         if (c.done) continue;
         const double step = c.step;
@@ -146,11 +168,13 @@ __global__ __launch_bounds__(kGroupBlock) void k_mprox_group(MProx P, Seg G, flo
 
 __global__ __launch_bounds__(kBlock) void k_mnn_ic(MProx P, Seg G, float *__restrict__ x, const Col *__restrict__ col)
 {
-    const int R = G.R;
+    const int R = G.R, sh = col_shift(R);
     const int64_t b = P.start[0], total = (P.end[0] - P.start[0]) * R;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % R);
-        const int64_t i = b + t / R;
+        int64_t i;
+        int r;
+        split_rc(t, R, sh, i, r);
+        i += b;
         if (col[G.segx[i] * R + r].done) continue;
         float *p = x + b * R + t;
         *p = (*p <= 0.0f) ? 0.0f : *p;
@@ -393,11 +417,13 @@ __global__ __launch_bounds__(kBlock) void k_mupdate(Seg G, int64_t n, const floa
                                                     float *__restrict__ xhat, float *__restrict__ prev,
                                                     const Col *__restrict__ col)
 {
-    const int R = G.R;
+    const int R = G.R, sh = col_shift(R);
     const int64_t total = n * R;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % R);
-        const Col &c = col[G.segx[t / R] * R + r];
+        int64_t i;
+        int r;
+        split_rc(t, R, sh, i, r);
+        const Col &c = col[G.segx[i] * R + r];
         if (!c.advance) continue;
         const double xv = (double)x[t];
         xhat[t] = (float)(xv + c.coef * (xv - (double)prev[t]));
