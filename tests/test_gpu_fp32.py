@@ -136,3 +136,40 @@ def test_power_iteration_lipschitz():
     L = solvers._lipschitz_power(A, lib, A._plan.handle, sw, g, 500, ctypes.c_void_p(st))
     assert A.precision == _lib.FP32                   # restored after the FP64 iterations
     assert abs(L - lam_max) <= 1e-6 * lam_max, (L, lam_max)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_fp32_rows_unaligned(mode):
+    """The float kernels move a lane's quad of y, sqrtw or residual samples as
+    one 16-byte access only when the row is 16-byte aligned (nS a multiple of
+    4 and the vector aligned); vectors at a 4-byte offset take the per-sample
+    path.  Both give the same residual, norm and A^T, bitwise, on the
+    per-voxel (mode 0) and the slab (mode 2) kernels."""
+    import ctypes
+    from commit_amd import autotune
+    D, K = random_dictionary(12 + mode, nVOX=40, nF=30, n=600, nS=60, ndirs=8, nIC=1, nEC=1, nISO=2)
+    A = LinearOperator(D, K, None, precision=_lib.FP32)
+    autotune.set_option(A, _lib.OPT_SLAB, mode)
+    m, n = A.shape
+    rng = np.random.default_rng(mode)
+    x = torch.from_numpy(rng.random(n).astype(np.float32)).cuda()
+    vals = [rng.standard_normal(m).astype(np.float32), rng.uniform(0.5, 1.5, m).astype(np.float32)]
+    lib, h = _lib.load(), A._plan.handle
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    out = []
+    for off in (0, 1):
+        bufs = [torch.zeros(m + 4, dtype=torch.float32, device="cuda") for _ in range(3)]
+        y, w, r = (b[off:off + m] for b in bufs)
+        y.copy_(torch.from_numpy(vals[0]))
+        w.copy_(torch.from_numpy(vals[1]))
+        assert (y.data_ptr() % 16 == 0) == (off == 0)
+        rsq = torch.empty(1, dtype=torch.float64, device="cuda")
+        g = torch.empty(n, dtype=torch.float32, device="cuda")
+        _lib.check(lib.commit_hip_A_residual(h, P(x), P(y), P(w), P(r), P(rsq), st), "A_residual")
+        _lib.check(lib.commit_hip_At_weighted(h, P(r), P(w), P(g), st), "At_weighted")
+        torch.cuda.synchronize()
+        out.append((r.cpu().numpy().copy(), rsq.item(), g.cpu().numpy().copy()))
+    (r0, s0, g0), (r1, s1, g1) = out
+    print(f"mode={mode}: residual max diff {np.abs(r0 - r1).max():.1e}, norm {s0:.17g} {s1:.17g}", flush=True)
+    assert np.array_equal(r0, r1) and s0 == s1 and np.array_equal(g0, g1)
