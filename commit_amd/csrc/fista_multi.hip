@@ -72,6 +72,19 @@ __device__ __forceinline__ double soft(double v, double t)
     return 0.0;
 }
 
+// The fields of the fit records the elementwise kernels read, staged in LDS
+// by each work-group (S * R records, at most kMaxProblems): an element then
+// waits on one global load (its problem) and one LDS read, not on a chain of
+// global loads into the 152-byte records.
+struct ProxRec {
+    double step, lam[3];
+    int32_t done, pad_;
+};
+struct UpdRec {
+    double coef;
+    int32_t advance, pad_;
+};
+
 // x = prox(xhat - step*grad, step) elementwise for the active problems
 // (soft-thresholding with the problem's lambdas, non-negativity where no
 // group penalty follows).  Finished problems keep x.
@@ -79,13 +92,20 @@ __global__ __launch_bounds__(kBlock) void k_mprox_elem(MProx P, Seg G, int64_t n
                                                        const float *__restrict__ grad, float *__restrict__ x,
                                                        const Col *__restrict__ col)
 {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ProxRec *cs = reinterpret_cast<ProxRec *>(smem);
     const int R = G.R, sh = col_shift(R);
+    for (int q = threadIdx.x; q < G.S * R; q += blockDim.x) {
+        const Col &c = col[q];
+        cs[q] = ProxRec{c.step, {c.lam[0], c.lam[1], c.lam[2]}, c.done, 0};
+    }
+    __syncthreads();
     const int64_t total = n * R;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         int64_t i;
         int r;
         split_rc(t, R, sh, i, r);
-        const Col &c = col[G.segx[i] * R + r];
+        const ProxRec &c = cs[G.segx[i] * R + r];
         if (c.done) continue;
         const double step = c.step;
         double v = (double)xhat[t] - step * (double)grad[t];             // (solvers.py:299)
@@ -417,13 +437,17 @@ __global__ __launch_bounds__(kBlock) void k_mupdate(Seg G, int64_t n, const floa
                                                     float *__restrict__ xhat, float *__restrict__ prev,
                                                     const Col *__restrict__ col)
 {
+    extern __shared__ __align__(16) unsigned char smem[];
+    UpdRec *cs = reinterpret_cast<UpdRec *>(smem);
     const int R = G.R, sh = col_shift(R);
+    for (int q = threadIdx.x; q < G.S * R; q += blockDim.x) cs[q] = UpdRec{col[q].coef, col[q].advance, 0};
+    __syncthreads();
     const int64_t total = n * R;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         int64_t i;
         int r;
         split_rc(t, R, sh, i, r);
-        const Col &c = col[G.segx[i] * R + r];
+        const UpdRec &c = cs[G.segx[i] * R + r];
         if (!c.advance) continue;
         const double xv = (double)x[t];
         xhat[t] = (float)(xv + c.coef * (xv - (double)prev[t]));
@@ -523,8 +547,8 @@ int commit_hip_prox_multi(const commit_hip_prox_params *p, const commit_hip_mult
     const Seg G = seg_of(L);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (L->n == 0) return 0;
-    hipLaunchKernelGGL(k_mprox_elem, dim3(elem_grid(L->n * L->nrhs)), dim3(kBlock), 0, st, a, G, L->n, xhat, grad, x,
-                       col);
+    hipLaunchKernelGGL(k_mprox_elem, dim3(elem_grid(L->n * L->nrhs)), dim3(kBlock),
+                       (size_t)G.S * G.R * sizeof(ProxRec), st, a, G, L->n, xhat, grad, x, col);
     HIP_TRY(hipGetLastError());
     const int r0 = a.reg[0];
     if (r0 == COMMIT_HIP_REG_GROUP_LASSO || r0 == COMMIT_HIP_REG_SPARSE_GROUP_LASSO) {
@@ -610,8 +634,9 @@ int commit_hip_fista_multi_update(const commit_hip_multi_layout *L, const float 
     if (rc) return rc;
     if (!x || !xhat || !prev || !col) return fail(COMMIT_HIP_EINVAL, "NULL argument");
     if (L->n == 0) return 0;
-    hipLaunchKernelGGL(k_mupdate, dim3(elem_grid(L->n * L->nrhs)), dim3(kBlock), 0, static_cast<hipStream_t>(stream),
-                       seg_of(L), L->n, x, xhat, prev, col);
+    const Seg G = seg_of(L);
+    hipLaunchKernelGGL(k_mupdate, dim3(elem_grid(L->n * L->nrhs)), dim3(kBlock), (size_t)G.S * G.R * sizeof(UpdRec),
+                       static_cast<hipStream_t>(stream), G, L->n, x, xhat, prev, col);
     HIP_TRY(hipGetLastError());
     return 0;
 }
